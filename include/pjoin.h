@@ -1,0 +1,167 @@
+/*
+ * pjoin.h -- C ABI of the MI355X hash-join path (libpjoin.so).
+ *
+ * The library replaces the compute behind Pythia's HashJoinOp (with the
+ * HashTable and TupleHasher it uses) and PartitionOp. The caller keeps the
+ * Operator protocol (init / threadInit / scanStart / getNext / scanStop /
+ * threadClose); the C++ operators in pythia-core_amd/host call these
+ * entry points at the matching points of that protocol. Reference citations
+ * are relative to the reference tree.
+ *
+ * Conventions: every function returns PJ_OK (0) or a negative PJ_ERR_*;
+ * the calling thread's pj_last_error() then describes the failure. No C++
+ * exception crosses this boundary. The caller owns every buffer it passes;
+ * the library owns device scratch memory and the handles it returns.
+ * Tuples are packed with no alignment, as the reference's Schema packs them
+ * (schema.cpp:82-117). Output order is unspecified (the reference's own
+ * order depends on thread interleaving); the output multiset is exact.
+ */
+#ifndef PJOIN_H
+#define PJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PJ_OK 0
+#define PJ_ERR_INVALID (-1)     /* bad descriptor or argument (InvalidParameter,
+                                   MissingParameterException, UnknownHashException) */
+#define PJ_ERR_UNSUPPORTED (-2) /* a schema the device path cannot join exactly */
+#define PJ_ERR_DEVICE (-3)      /* a HIP runtime failure, e.g. hipMalloc */
+#define PJ_ERR_CAPACITY (-4)    /* output buffer too small; *out_n = rows needed */
+#define PJ_ERR_STATE (-5)       /* call out of protocol order */
+
+/* Column types, numbered as the reference's ColumnType (schema.h:53-60). */
+enum { PJ_CT_INTEGER = 0, PJ_CT_LONG = 1, PJ_CT_DECIMAL = 2, PJ_CT_CHAR = 3,
+       PJ_CT_DATE = 4, PJ_CT_POINTER = 5 };
+
+#define PJ_MAX_COLS 32
+#define PJ_MAX_KEYS 8
+
+typedef struct {
+    uint32_t ncols;
+    uint32_t type[PJ_MAX_COLS];
+    uint32_t width[PJ_MAX_COLS];  /* read for PJ_CT_CHAR only */
+} pj_schema;
+
+enum { PJ_SIDE_BUILD = 0, PJ_SIDE_PROBE = 1 };
+enum { PJ_PART_RADIX = 0, PJ_PART_LEARNED = 1, PJ_PART_AUTO = 2 };
+
+/* The plan-node parameters of a `hashjoin` that shape its result
+ * (operators/operators.h:2965-3070; parsed by operators/join.cpp:224-326 and
+ * 367-518). The hash function and bucket count of the plan are validated by
+ * the C++ operator but do not change the output multiset; the device path
+ * partitions on its own hash (or the learned model's ranges). */
+typedef struct {
+    pj_schema build;
+    pj_schema probe;
+    uint32_t nkeys;                     /* buildjattr / probejattr length */
+    uint16_t buildjattr[PJ_MAX_KEYS];
+    uint16_t probejattr[PJ_MAX_KEYS];
+    uint32_t nproj;                     /* projection: "B$i" / "P$j" entries */
+    uint8_t proj_side[PJ_MAX_COLS];
+    uint16_t proj_attr[PJ_MAX_COLS];
+    uint32_t partitioner;               /* PJ_PART_* */
+    int32_t radix_bits;                 /* -1: chosen from |R| */
+    uint32_t flags;                     /* PJ_FLAG_* */
+} pj_join_desc;
+
+#define PJ_FLAG_TIMING 1u               /* record per-phase HIP event times */
+
+typedef struct pj_join_s* pj_join_t;
+
+/* Per-phase device times of the last run, in milliseconds (PJ_FLAG_TIMING). */
+typedef struct {
+    float build_partition_ms;  /* histogram + scan + scatter of R */
+    float probe_partition_ms;  /* histogram + scan + scatter of S */
+    float join_ms;             /* LDS build + probe + output (join kernel) */
+    float model_ms;            /* learned partitioner: sample, fit, boundaries */
+    float total_ms;            /* first launch to last, including host planning */
+    uint32_t partitions;       /* partitions of S */
+    uint32_t subpartitions;    /* R sub-partitions per S partition */
+    uint32_t work_items;       /* join work items */
+    uint32_t used_learned;     /* 1 if the learned partitioner was used */
+    uint64_t spilled_items;    /* work items whose R part exceeded one LDS table */
+} pj_timings;
+
+/* Resumable probe cursor: the state HashJoinOp keeps between getNext calls
+ * (HashJoinState, operators/operators.h:3105-3114). Zero-initialise it;
+ * release it with pj_cursor_release. */
+typedef struct {
+    void* impl;
+    uint64_t pending;   /* rows produced and not yet handed out */
+} pj_cursor;
+
+const char* pj_last_error(void);
+const char* pj_version(void);
+
+/* Replaces HashJoinOp::init (operators/join.cpp:367-518): validates the
+ * schemas, key columns and projection; computes the output schema. */
+int pj_join_create(const pj_join_desc* desc, pj_join_t* out);
+/* Output tuple width in bytes (HashJoinOp's output Schema::getTupleSize). */
+int pj_join_out_width(pj_join_t h, uint32_t* width);
+/* Replaces HashJoinOp::destroy + HashTable::destroy (join.cpp:828-836,
+ * util/hashtable.cpp:243-252). */
+int pj_join_destroy(pj_join_t h);
+
+/* One whole join on device-resident relations: the compute of
+ * HashJoinOp::scanStart (build, join.cpp:548-637) followed by every
+ * HashJoinOp::getNext (probe, join.cpp:681-757). d_build/d_probe hold packed
+ * build/probe tuples in HBM; d_out receives up to out_cap output tuples.
+ * *out_n gets the number of output tuples; if it exceeds out_cap the call
+ * returns PJ_ERR_CAPACITY and the buffer holds an unspecified subset.
+ * stream: a hipStream_t, or NULL for the default stream. Synchronous. */
+int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild,
+                       const void* d_probe, uint64_t nprobe, void* d_out,
+                       uint64_t out_cap, uint64_t* out_n, void* stream);
+
+/* Host-page protocol used by the C++ HashJoinOp. */
+/* Stages build tuples (HashJoinOp::buildFromPage, join.cpp:838-884).
+ * Thread-safe; call from every build thread. */
+int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n);
+/* Uploads and partitions the staged build side once the group barrier has
+ * been passed (join.cpp:618). Exactly once per scan. */
+int pj_join_build_finish(pj_join_t h);
+/* Probes a batch of host probe tuples and hands out up to out_cap output
+ * tuples per call (HashJoinOp::getNext, join.cpp:681-757, which returns
+ * Ready when its output page fills and resumes from HashJoinState). Pass the
+ * batch once (tuples != NULL); then call again with tuples == NULL and the
+ * same cursor while cur->pending > 0. Thread-safe across cursors. */
+int pj_join_probe(pj_join_t h, const void* tuples, uint64_t n, pj_cursor* cur,
+                  void* out, uint64_t out_cap, uint64_t* out_n);
+int pj_cursor_release(pj_cursor* cur);
+/* Drops the staged and device-resident build side (HashTable::bucketclear +
+ * destroy, join.cpp:802-826) so the handle can take another scan. */
+int pj_join_reset(pj_join_t h);
+
+int pj_join_last_timings(pj_join_t h, pj_timings* t);
+
+/* PartitionOp (operators/partition.cpp:44-404) on device-resident tuples:
+ * the exactrange hash on column attr (an INTEGER, LONG or DATE column) over
+ * [range_min, range_max] into `buckets` partitions (hash.h:452-487).
+ * d_out receives partition 0, then 1, ...; part_sizes (host, `buckets`
+ * entries) gets the sizes. Within a partition the order is unspecified. */
+int pj_partition_exactrange_device(const pj_schema* schema, uint32_t attr,
+                                   int64_t range_min, int64_t range_max,
+                                   uint32_t buckets, const void* d_in, uint64_t n,
+                                   void* d_out, uint64_t* part_sizes, void* stream);
+
+/* Order-independent 64-bit checksum of device-resident tuples: the sum mod
+ * 2^64 of a splitmix64-folded mix of each tuple's bytes, plus the XOR of all
+ * 32-bit words that ConsumeOp computes (operators/consume.cpp:68-100). */
+int pj_checksum_device(const void* d_tuples, uint64_t n, uint32_t width,
+                       uint64_t* sum, uint32_t* xor32, void* stream);
+
+/* Workload generators (pythia-core_amd/csrc/pj_datagen.h), on the device
+ * (dst in HBM) or on host threads (dst in host memory). */
+int pj_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_probe,
+                uint64_t n_miss, void* build_dst, void* probe_dst, int on_device,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

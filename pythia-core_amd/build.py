@@ -1,0 +1,68 @@
+"""Builds libpjoin.so (HIP kernels + C ABI) for gfx950, in-tree.
+
+Usage: python pythia-core_amd/build.py
+The library lands in pythia-core_amd/lib/. hipcc cross-compiles, so this
+runs on a machine without a GPU.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip"]
+HOST_SOURCES = ["schema.cpp", "plan.cpp", "operators.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
+                    and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    for s in KERNEL_SOURCES:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(LIB, s.replace(".hip", ".o"))
+        _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+        objs.append(obj)
+    for s in HOST_SOURCES:
+        src = os.path.join(HOST, s)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(LIB, "host_" + s.replace(".cpp", ".o"))
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+              "-c", src, "-o", obj])
+        objs.append(obj)
+    out = os.path.join(LIB, "libpjoin.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, out)
+    for o in objs:
+        os.remove(o)
+    if verbose:
+        print("built", out)
+    return out
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,1016 @@
+// pj_join.hip -- the radix-partitioned hash join for gfx950.
+//
+// What it replaces in the reference (paths relative to its tree):
+//   HashJoinOp::scanStart / buildFromPage  operators/join.cpp:548-637, 838-884
+//   HashJoinOp::getNext / constructOutput  operators/join.cpp:648-757
+//   HashTable allocate / iterator          util/hashtable.cpp:254-297, util/hashtable.h:126-171
+//   ConjunctionEqualsEvaluator on keys     conjunctionevaluator.h:65-73, rawcompfns.cpp:48-180
+// The reference shares one chained hash table among T threads. Here both
+// relations are radix-partitioned on the top bits of a 64-bit mix of the key
+// (histogram, exclusive scan, LDS-sorted scatter), every build partition is
+// loaded into an LDS-resident bucket table, and the probe partition streams
+// past it; a whole workgroup reserves output space with one atomic per tile
+// and lanes write adjacent output tuples.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "pj_internal.h"
+#include "pj_scan.h"
+
+namespace pj {
+
+// ======================================================================
+// device helpers
+// ======================================================================
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Streaming (non-temporal) 16-byte load and 8-byte store.
+__device__ __forceinline__ uint4 ld_stream(const uint4* p)
+{
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_stream2(uint32_t* p, uint32_t a, uint32_t b)
+{
+    u32x2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(p));
+}
+
+__device__ __forceinline__ uint64_t rec_key(uint4 r) { return (uint64_t)r.x | ((uint64_t)r.y << 32); }
+
+// Key mix used for partition, sub-partition and LDS bucket bits.
+__device__ __forceinline__ uint64_t key_mix(uint64_t k) { return pj_fin64(k); }
+
+__device__ __forceinline__ uint32_t radix_part(uint64_t h, uint32_t bits)
+{
+    return bits ? (uint32_t)(h >> (64 - bits)) : 0u;
+}
+
+template <bool IDENT>
+__device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint64_t i, const SideLayout& L)
+{
+    if (IDENT) {
+        return ld_stream(reinterpret_cast<const uint4*>(in) + i);
+    } else {
+        const uint32_t* t = in + i * L.tw;
+        uint4 r;
+        r.x = t[L.k0];
+        r.y = L.k1 >= 0 ? t[L.k1] : (uint32_t)((int32_t)r.x >> 31);
+        r.z = L.p0 >= 0 ? t[L.p0] : 0u;
+        r.w = L.p1 >= 0 ? t[L.p1] : 0u;
+        return r;
+    }
+}
+
+// ======================================================================
+// partition pass: histogram, scan, scatter
+// ======================================================================
+
+constexpr int kHistThreads = 512;
+constexpr int kHistItems = 8;
+
+// hist[p * nblk + blk] = tuples of block blk's chunk in partition p.
+template <bool IDENT>
+__global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ in, uint64_t n,
+                                                        SideLayout L, uint32_t bits, uint64_t chunk,
+                                                        uint32_t nblk, uint32_t* __restrict__ hist)
+{
+    extern __shared__ uint32_t cnt[];
+    const uint32_t F = 1u << bits;
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+        cnt[p] = 0;
+    __syncthreads();
+    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = beg + chunk < n ? beg + chunk : n;
+    for (uint64_t base = beg; base < end; base += (uint64_t)kHistThreads * kHistItems) {
+        uint4 r[kHistItems];
+#pragma unroll
+        for (int k = 0; k < kHistItems; ++k) {
+            uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
+            if (i < end)
+                r[k] = load_rec<IDENT>(in, i, L);
+        }
+#pragma unroll
+        for (int k = 0; k < kHistItems; ++k) {
+            uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
+            if (i < end)
+                atomicAdd(&cnt[radix_part(key_mix(rec_key(r[k])), bits)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+        hist[(uint64_t)p * nblk + blockIdx.x] = cnt[p];
+}
+
+// part_start[p] = first record of partition p; part_start[F] = n.
+__global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uint32_t nblk, uint32_t n,
+                              uint32_t* __restrict__ part_start)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < F)
+        part_start[p] = offs[(uint64_t)p * nblk];
+    if (p == F)
+        part_start[F] = n;
+}
+
+constexpr int kScatterItems = 16;
+
+// Scatter with an LDS-sorted tile: each tile of up to ts tuples is counting-
+// sorted by partition in LDS, then written out so that lanes of a wave store
+// consecutive records of one partition.
+template <bool IDENT>
+__global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
+                                                              SideLayout L, uint32_t bits, uint64_t chunk,
+                                                              uint32_t nblk, const uint32_t* __restrict__ offs,
+                                                              uint4* __restrict__ out, uint32_t ts)
+{
+    extern __shared__ uint4 sm4[];
+    const uint32_t F = 1u << bits;
+    uint4* tile = sm4;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(tile + ts);  // F + 1
+    uint32_t* goff = cnt + F + 1;                            // F
+    uint32_t* wsum = goff + F;                               // 9
+    for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
+        goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
+    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = beg + chunk < n ? beg + chunk : n;
+    for (uint64_t t0 = beg; t0 < end; t0 += ts) {
+        const uint32_t tn = (uint32_t)(end - t0 < ts ? end - t0 : ts);
+        for (uint32_t p = threadIdx.x; p <= F; p += kScatterThreads)
+            cnt[p] = 0;
+        __syncthreads();
+        uint4 r[kScatterItems];
+        uint32_t part[kScatterItems], rank[kScatterItems];
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) {
+            uint32_t i = k * kScatterThreads + threadIdx.x;
+            if (i < tn)
+                r[k] = load_rec<IDENT>(in, t0 + i, L);
+        }
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) {
+            uint32_t i = k * kScatterThreads + threadIdx.x;
+            if (i < tn) {
+                part[k] = radix_part(key_mix(rec_key(r[k])), bits);
+                rank[k] = atomicAdd(&cnt[part[k]], 1u);
+            }
+        }
+        __syncthreads();
+        block_scan_array<kScatterThreads>(cnt, F, wsum);
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) {
+            uint32_t i = k * kScatterThreads + threadIdx.x;
+            if (i < tn)
+                tile[cnt[part[k]] + rank[k]] = r[k];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < tn; i += kScatterThreads) {
+            uint4 v = tile[i];
+            uint32_t p = radix_part(key_mix(rec_key(v)), bits);
+            out[goff[p] + (i - cnt[p])] = v;
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
+            goff[p] += cnt[p + 1] - cnt[p];
+        __syncthreads();
+    }
+}
+
+// ======================================================================
+// join: LDS-resident bucket table per build partition, streaming probe
+// ======================================================================
+
+constexpr int kProbeItems = 4;
+constexpr uint32_t kBuildPerThread = (kTableCap + kJoinThreads - 1) / kJoinThreads;
+
+__device__ __forceinline__ uint32_t pick_word(const uint4& b, const uint4& s, uint32_t src)
+{
+    switch (src) {
+    case SRC_BK0: return b.x;
+    case SRC_BK1: return b.y;
+    case SRC_BP0: return b.z;
+    case SRC_BP1: return b.w;
+    case SRC_PK0: return s.x;
+    case SRC_PK1: return s.y;
+    case SRC_PP0: return s.z;
+    default: return s.w;
+    }
+}
+
+__device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t row, const OutFormat& f,
+                                          const uint4& b, const uint4& s)
+{
+    uint32_t* o = out + row * f.words;
+    if (f.words == 6 && f.src[0] == SRC_BK0 && f.src[1] == SRC_BK1 && f.src[2] == SRC_BP0 &&
+        f.src[3] == SRC_BP1 && f.src[4] == SRC_PP0 && f.src[5] == SRC_PP1) {
+        st_stream2(o, b.x, b.y);
+        st_stream2(o + 2, b.z, b.w);
+        st_stream2(o + 4, s.z, s.w);
+        return;
+    }
+#pragma unroll
+    for (int w = 0; w < 8; ++w)
+        if ((uint32_t)w < f.words)
+            o[w] = pick_word(b, s, f.src[w]);
+}
+
+__global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
+                                                        const Item* __restrict__ items, uint32_t nitems,
+                                                        uint32_t* __restrict__ item_ctr,
+                                                        uint32_t* __restrict__ out, uint64_t out_cap,
+                                                        unsigned long long* __restrict__ out_cursor,
+                                                        OutFormat fmt, uint32_t sub_shift, uint32_t bkt_shift)
+{
+    extern __shared__ uint4 lds[];
+    uint4* rec = lds;                                             // kTableCap
+    uint32_t* dir = reinterpret_cast<uint32_t*>(rec + kTableCap);  // kTableBuckets + 1
+    uint32_t* wsum = dir + kTableBuckets + 1;                     // 17
+    uint32_t* misc = wsum + 17;                                   // [0] item, [1..2] base
+    const uint32_t tid = threadIdx.x;
+
+    for (;;) {
+        if (tid == 0)
+            misc[0] = atomicAdd(item_ctr, 1u);
+        __syncthreads();
+        const uint32_t it = misc[0];
+        __syncthreads();
+        if (it >= nitems)
+            break;
+        const Item I = items[it];
+
+        // ---- build: counting sort of the partition's records by bucket
+        for (uint32_t b = tid; b <= kTableBuckets; b += kJoinThreads)
+            dir[b] = 0;
+        __syncthreads();
+        uint4 br[kBuildPerThread];
+        uint32_t bb[kBuildPerThread], brank[kBuildPerThread];
+#pragma unroll
+        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
+            uint32_t i = k * kJoinThreads + tid;
+            if (i < I.r_cnt)
+                br[k] = R[I.r_off + i];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
+            uint32_t i = k * kJoinThreads + tid;
+            if (i < I.r_cnt) {
+                bb[k] = (uint32_t)(key_mix(rec_key(br[k])) >> bkt_shift) & (kTableBuckets - 1);
+                brank[k] = atomicAdd(&dir[bb[k]], 1u);
+            }
+        }
+        __syncthreads();
+        block_scan_array<kJoinThreads>(dir, kTableBuckets, wsum);
+#pragma unroll
+        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
+            uint32_t i = k * kJoinThreads + tid;
+            if (i < I.r_cnt)
+                rec[dir[bb[k]] + brank[k]] = br[k];
+        }
+        __syncthreads();
+
+        // ---- probe
+        const uint4* Sp = S + I.s_off;
+        for (uint32_t t0 = 0; t0 < I.s_cnt; t0 += kJoinThreads * kProbeItems) {
+            uint4 s[kProbeItems];
+            uint32_t nm[kProbeItems], first[kProbeItems];
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                uint32_t i = t0 + k * kJoinThreads + tid;
+                if (i < I.s_cnt)
+                    s[k] = ld_stream(Sp + i);
+            }
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                uint32_t i = t0 + k * kJoinThreads + tid;
+                nm[k] = 0;
+                first[k] = 0;
+                if (i < I.s_cnt) {
+                    const uint64_t key = rec_key(s[k]);
+                    const uint64_t h = key_mix(key);
+                    if (I.sub_mask == 0 || (((uint32_t)(h >> sub_shift)) & I.sub_mask) == I.sub_val) {
+                        const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
+                        const uint32_t lo = dir[b], hi = dir[b + 1];
+                        for (uint32_t j = lo; j < hi; ++j) {
+                            const uint4 c = rec[j];
+                            if (rec_key(c) == key) {
+                                if (nm[k] == 0)
+                                    first[k] = j;
+                                nm[k]++;
+                            }
+                        }
+                    }
+                }
+            }
+            // one output reservation per probe slot k for the whole block,
+            // so lanes of a wave write adjacent tuples
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                uint32_t total;
+                uint32_t pre = block_excl_scan<kJoinThreads>(nm[k], wsum, &total);
+                if (total == 0)
+                    continue;
+                if (tid == 0) {
+                    unsigned long long base = atomicAdd(out_cursor, (unsigned long long)total);
+                    misc[1] = (uint32_t)base;
+                    misc[2] = (uint32_t)(base >> 32);
+                }
+                __syncthreads();
+                const uint64_t base = (uint64_t)misc[1] | ((uint64_t)misc[2] << 32);
+                __syncthreads();
+                if (nm[k]) {
+                    uint64_t row = base + pre;
+                    const uint64_t key = rec_key(s[k]);
+                    uint32_t j = first[k];
+                    uint32_t left = nm[k];
+                    while (left) {
+                        const uint4 c = rec[j];
+                        if (rec_key(c) == key) {
+                            if (row < out_cap)
+                                write_row(out, row, fmt, c, s[k]);
+                            row++;
+                            left--;
+                        }
+                        j++;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ======================================================================
+// host side
+// ======================================================================
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+static int col_width(const pj_schema& s, uint32_t c)
+{
+    switch (s.type[c]) {
+    case PJ_CT_INTEGER: return 4;
+    case PJ_CT_LONG:
+    case PJ_CT_DECIMAL:
+    case PJ_CT_DATE:
+    case PJ_CT_POINTER: return 8;
+    case PJ_CT_CHAR: return (int)s.width[c];
+    default: return -1;
+    }
+}
+
+static int schema_offsets(const pj_schema& s, int* off, int* size)
+{
+    if (s.ncols == 0 || s.ncols > PJ_MAX_COLS)
+        return fail(PJ_ERR_INVALID, "schema must have 1..%d columns", PJ_MAX_COLS);
+    int o = 0;
+    for (uint32_t c = 0; c < s.ncols; ++c) {
+        int w = col_width(s, c);
+        if (w < 0 || (s.type[c] == PJ_CT_CHAR && w < 2))
+            return fail(PJ_ERR_INVALID, "column %u: bad type %u / width %u", c, s.type[c], s.width[c]);
+        off[c] = o;
+        o += w;
+    }
+    *size = o;
+    return PJ_OK;
+}
+
+static bool int_key(uint32_t t) { return t == PJ_CT_INTEGER || t == PJ_CT_LONG || t == PJ_CT_DATE; }
+
+// Mirrors the checks and schema construction of JoinOp::init and
+// HashJoinOp::init (operators/join.cpp:224-326, 367-461), then maps the
+// key, payload and output columns onto 32-bit words.
+int derive_layout(const pj_join_desc* d, Layout* L)
+{
+    int boff[PJ_MAX_COLS], poff[PJ_MAX_COLS], bsize, psize;
+    int rc;
+    if ((rc = schema_offsets(d->build, boff, &bsize)) || (rc = schema_offsets(d->probe, poff, &psize)))
+        return rc;
+    if (d->nkeys == 0 || d->nkeys > PJ_MAX_KEYS)
+        return fail(PJ_ERR_INVALID, "buildjattr/probejattr must name 1..%d columns", PJ_MAX_KEYS);
+    for (uint32_t k = 0; k < d->nkeys; ++k)
+        if (d->buildjattr[k] >= d->build.ncols || d->probejattr[k] >= d->probe.ncols)
+            return fail(PJ_ERR_INVALID, "join attribute %u out of range", k);
+    if (d->nproj == 0 || d->nproj > PJ_MAX_COLS)
+        return fail(PJ_ERR_INVALID, "projection must have 1..%d entries", PJ_MAX_COLS);
+    for (uint32_t j = 0; j < d->nproj; ++j) {
+        const pj_schema& s = d->proj_side[j] == PJ_SIDE_BUILD ? d->build : d->probe;
+        if (d->proj_side[j] > PJ_SIDE_PROBE || d->proj_attr[j] >= s.ncols)
+            return fail(PJ_ERR_INVALID, "projection entry %u out of range", j);
+    }
+    if (bsize % 4 || psize % 4)
+        return fail(PJ_ERR_UNSUPPORTED, "device path needs tuple widths that are multiples of 4 bytes");
+
+    // keys: one INTEGER/LONG/DATE column a side, or two INTEGER columns
+    SideLayout* sl[2] = {&L->build, &L->probe};
+    const pj_schema* sc[2] = {&d->build, &d->probe};
+    const int* so[2] = {boff, poff};
+    const uint16_t* ja[2] = {d->buildjattr, d->probejattr};
+    for (int side = 0; side < 2; ++side) {
+        SideLayout& s = *sl[side];
+        s.tw = (uint32_t)((side ? psize : bsize) / 4);
+        s.p0 = s.p1 = -1;
+        if (d->nkeys == 1) {
+            uint32_t c = ja[side][0];
+            if (!int_key(sc[side]->type[c]))
+                return fail(PJ_ERR_UNSUPPORTED, "device path joins INTEGER, LONG or DATE keys");
+            if (so[side][c] % 4)
+                return fail(PJ_ERR_UNSUPPORTED, "key column not 4-byte aligned");
+            s.k0 = so[side][c] / 4;
+            s.k1 = sc[side]->type[c] == PJ_CT_INTEGER ? -1 : s.k0 + 1;
+        } else if (d->nkeys == 2) {
+            uint32_t a = ja[side][0], b = ja[side][1];
+            if (sc[side]->type[a] != PJ_CT_INTEGER || sc[side]->type[b] != PJ_CT_INTEGER)
+                return fail(PJ_ERR_UNSUPPORTED, "device path joins composite keys of two INTEGER columns");
+            if (so[side][a] % 4 || so[side][b] % 4)
+                return fail(PJ_ERR_UNSUPPORTED, "key column not 4-byte aligned");
+            s.k0 = so[side][a] / 4;
+            s.k1 = so[side][b] / 4;
+        } else {
+            return fail(PJ_ERR_UNSUPPORTED, "device path joins on at most two key columns");
+        }
+    }
+    if (d->nkeys == 1) {
+        bool bl = d->build.type[d->buildjattr[0]] == PJ_CT_DATE, pl = d->probe.type[d->probejattr[0]] == PJ_CT_DATE;
+        if (bl != pl)
+            return fail(PJ_ERR_UNSUPPORTED, "DATE keys join only DATE keys");
+    }
+
+    // output words
+    OutFormat& f = L->out;
+    memset(&f, 0, sizeof(f));
+    uint32_t words = 0;
+    for (uint32_t j = 0; j < d->nproj; ++j) {
+        const int side = d->proj_side[j];
+        const pj_schema& s = *sc[side];
+        const uint32_t c = d->proj_attr[j];
+        const int w = col_width(s, c);
+        if (s.type[c] == PJ_CT_CHAR)
+            return fail(PJ_ERR_UNSUPPORTED, "device path does not project CHAR columns");
+        const int nw = w / 4;
+        if (words + nw > 8)
+            return fail(PJ_ERR_UNSUPPORTED, "device path emits output tuples of at most 32 bytes");
+        int keyidx = -1;
+        for (uint32_t k = 0; k < d->nkeys; ++k)
+            if (ja[side][k] == c)
+                keyidx = (int)k;
+        const uint8_t kbase = side ? SRC_PK0 : SRC_BK0, pbase = side ? SRC_PP0 : SRC_BP0;
+        if (keyidx >= 0) {
+            if (d->nkeys == 2)
+                f.src[words++] = (uint8_t)(kbase + keyidx);
+            else {
+                f.src[words++] = kbase;
+                if (nw == 2)
+                    f.src[words++] = kbase + 1;
+            }
+            continue;
+        }
+        // a non-key column travels in the record's payload words
+        SideLayout& sdl = *sl[side];
+        if (so[side][c] % 4)
+            return fail(PJ_ERR_UNSUPPORTED, "payload column not 4-byte aligned");
+        for (int q = 0; q < nw; ++q) {
+            const int32_t word = so[side][c] / 4 + q;
+            int slot;
+            if (sdl.p0 == word) slot = 0;
+            else if (sdl.p1 == word) slot = 1;
+            else if (sdl.p0 < 0) { sdl.p0 = word; slot = 0; }
+            else if (sdl.p1 < 0) { sdl.p1 = word; slot = 1; }
+            else
+                return fail(PJ_ERR_UNSUPPORTED, "device path carries at most 8 payload bytes a side");
+            f.src[words++] = (uint8_t)(pbase + slot);
+        }
+    }
+    f.words = words;
+    L->out_width = words * 4;
+    for (int side = 0; side < 2; ++side) {
+        SideLayout& s = *sl[side];
+        s.ident = (s.tw == 4 && s.k0 == 0 && s.k1 == 1 && s.p0 == 2 && s.p1 == 3) ? 1u : 0u;
+    }
+    return PJ_OK;
+}
+
+}  // namespace pj
+
+// ======================================================================
+// handle and orchestration
+// ======================================================================
+
+using namespace pj;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap)
+            return PJ_OK;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = bytes < 256 ? 256 : bytes;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(PJ_ERR_DEVICE, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+        }
+        cap = want;
+        return PJ_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// One relation partitioned on the device.
+struct Partitioned {
+    uint32_t bits = 0;
+    uint64_t n = 0;
+    std::vector<uint32_t> start;   // F + 1 partition starts (host copy)
+};
+
+struct CursorImpl {
+    std::vector<uint8_t> rows;
+    uint64_t next = 0, total = 0;
+};
+
+}  // namespace
+
+struct pj_join_s {
+    pj_join_desc desc;
+    Layout lay;
+    std::mutex mu;            // serialises device work on this handle
+    std::mutex stage_mu;      // guards staged build tuples
+    std::vector<uint8_t> staged;
+    bool built = false;
+    DevBuf d_raw_build, d_raw_probe, d_out_batch;
+    DevBuf rpart, spart, hist, bsum, starts, items, counters;
+    Partitioned R, S;
+    uint32_t bR = 0, bS = 0;  // radix bits of R and of S (bR >= bS)
+    pj_timings last;
+    hipEvent_t ev[8];
+    bool ev_ok = false;
+};
+
+namespace {
+
+constexpr uint32_t kMaxBits = 14;
+
+uint32_t choose_build_bits(uint64_t nr, int32_t forced)
+{
+    if (forced >= 0)
+        return (uint32_t)std::min<int32_t>(forced, (int32_t)kMaxBits);
+    for (uint32_t b = 0; b < kMaxBits; ++b) {
+        double avg = (double)nr / (double)(1ull << b);
+        if (avg + 5.0 * std::sqrt(avg) + 16.0 <= (double)kTableCap)
+            return b;
+    }
+    return kMaxBits;
+}
+
+uint32_t scatter_tile(uint32_t bits)
+{
+    const uint32_t F = 1u << bits;
+    const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4;
+    int64_t ts = avail / 16;
+    ts = (ts / kScatterThreads) * kScatterThreads;
+    ts = std::min<int64_t>(ts, (int64_t)kScatterThreads * kScatterItems);
+    return ts > 0 ? (uint32_t)ts : 0;
+}
+
+uint32_t hist_blocks(uint64_t n)
+{
+    uint64_t b = (n + 65535) / 65536;  // ~64K tuples per block
+    if (b < 1)
+        b = 1;
+    if (b > 2048)
+        b = 2048;
+    return (uint32_t)b;
+}
+
+// Partitions raw tuples of one side into dst (canonical records).
+int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits,
+                   DevBuf& dst, Partitioned& P, hipStream_t st)
+{
+    int rc;
+    P.bits = bits;
+    P.n = n;
+    const uint32_t F = 1u << bits;
+    P.start.assign(F + 1, 0);
+    if (n == 0)
+        return PJ_OK;
+    if (n >= (1ull << 32))
+        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^32 tuples or more");
+    const uint32_t nblk = hist_blocks(n);
+    const uint64_t chunk = (n + nblk - 1) / nblk;
+    const uint64_t m = (uint64_t)F * nblk;
+    const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
+    if ((rc = dst.ensure(n * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)) ||
+        (rc = h->starts.ensure((F + 1) * 4)))
+        return rc;
+    const uint32_t* in = (const uint32_t*)d_in;
+    const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
+    uint32_t* hist = (uint32_t*)h->hist.p;
+    if (ident)
+        hipLaunchKernelGGL(k_hist<true>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, hist);
+    else
+        hipLaunchKernelGGL(k_hist<false>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk,
+                           hist);
+    PJ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (uint32_t*)h->bsum.p);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, (uint32_t*)h->bsum.p, nsb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)h->bsum.p);
+    hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk, (uint32_t)n,
+                       (uint32_t*)h->starts.p);
+    PJ_HIP(hipGetLastError());
+    const uint32_t ts = scatter_tile(bits);
+    if (ts == 0)
+        return fail(PJ_ERR_UNSUPPORTED, "radix bits %u exceed the scatter's LDS budget", bits);
+    const size_t lds = (size_t)ts * 16 + (2 * F + 1) * 4 + 16 * 4;
+    if (ident) {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_scatter<true>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk, nblk,
+                           hist, (uint4*)dst.p, ts);
+    } else {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_scatter<false>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk, nblk,
+                           hist, (uint4*)dst.p, ts);
+    }
+    PJ_HIP(hipGetLastError());
+    PJ_HIP(hipMemcpyAsync(P.start.data(), h->starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
+    return PJ_OK;
+}
+
+int device_count_cus()
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// Builds the work list: for every probe partition and every build
+// sub-partition under it, the build records (split by position when they
+// exceed one LDS table) times chunks of the probe records.
+void plan_items(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
+{
+    items.clear();
+    *spilled = 0;
+    const uint32_t FS = 1u << h->bS, Q = 1u << (h->bR - h->bS);
+    const uint64_t ns = h->S.n;
+    const uint64_t waves = (uint64_t)device_count_cus() * 8;
+    uint64_t chunk = std::max<uint64_t>(ns / std::max<uint64_t>(waves, 1), 32768);
+    for (uint32_t ps = 0; ps < FS; ++ps) {
+        const uint32_t s0 = h->S.start[ps], s1 = h->S.start[ps + 1];
+        if (s1 == s0)
+            continue;
+        for (uint32_t q = 0; q < Q; ++q) {
+            const uint32_t pr = ps * Q + q;
+            const uint32_t r0 = h->R.start[pr], r1 = h->R.start[pr + 1];
+            if (r1 == r0)
+                continue;
+            const uint32_t nsplit = (r1 - r0 + kTableCap - 1) / kTableCap;
+            if (nsplit > 1)
+                *spilled += nsplit;
+            for (uint32_t rs = 0; rs < nsplit; ++rs) {
+                const uint32_t ro = r0 + rs * kTableCap;
+                const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
+                for (uint64_t so = s0; so < s1; so += chunk) {
+                    Item it;
+                    it.r_off = ro;
+                    it.r_cnt = rc;
+                    it.s_off = (uint32_t)so;
+                    it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
+                    it.sub_mask = Q > 1 ? Q - 1 : 0;
+                    it.sub_val = q;
+                    items.push_back(it);
+                }
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& a, const Item& b) { return a.s_cnt > b.s_cnt; });
+}
+
+hipStream_t as_stream(void* s) { return (hipStream_t)s; }
+
+void record(pj_join_s* h, int i, hipStream_t st)
+{
+    if (h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING))
+        (void)hipEventRecord(h->ev[i], st);
+}
+
+float elapsed(pj_join_s* h, int a, int b)
+{
+    float ms = 0.f;
+    if (h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING))
+        (void)hipEventElapsedTime(&ms, h->ev[a], h->ev[b]);
+    return ms;
+}
+
+int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st)
+{
+    h->bR = choose_build_bits(nb, h->desc.radix_bits);
+    h->bS = std::min<uint32_t>(h->bR, 11);
+    record(h, 0, st);
+    int rc = partition_side(h, d_build, nb, h->lay.build, h->bR, h->rpart, h->R, st);
+    record(h, 1, st);
+    return rc;
+}
+
+int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
+                 hipStream_t st)
+{
+    int rc;
+    *out_n = 0;
+    record(h, 2, st);
+    if ((rc = partition_side(h, d_probe, np, h->lay.probe, h->bS, h->spart, h->S, st)))
+        return rc;
+    record(h, 3, st);
+    PJ_HIP(hipStreamSynchronize(st));  // partition starts are on the host now
+    if (h->R.n == 0 || np == 0) {
+        memset(&h->last, 0, sizeof(h->last));
+        return PJ_OK;
+    }
+    std::vector<Item> items;
+    uint64_t spilled = 0;
+    plan_items(h, items, &spilled);
+    const uint32_t nitems = (uint32_t)items.size();
+    if ((rc = h->items.ensure(std::max<size_t>(nitems, 1) * sizeof(Item))) || (rc = h->counters.ensure(64)))
+        return rc;
+    if (nitems)
+        PJ_HIP(hipMemcpyAsync(h->items.p, items.data(), nitems * sizeof(Item), hipMemcpyHostToDevice, st));
+    PJ_HIP(hipMemsetAsync(h->counters.p, 0, 64, st));
+    uint32_t* item_ctr = (uint32_t*)h->counters.p;
+    unsigned long long* cursor = (unsigned long long*)((char*)h->counters.p + 8);
+    const size_t lds = (size_t)kTableCap * 16 + (kTableBuckets + 1) * 4 + 17 * 4 + 4 * 4;
+    PJ_HIP(hipFuncSetAttribute((const void*)k_join, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const uint32_t sub_shift = 64 - h->bR;
+    const uint32_t bkt_shift = 64 - h->bR - 12;
+    record(h, 4, st);
+    if (nitems) {
+        const uint32_t grid = std::min<uint32_t>(nitems, (uint32_t)device_count_cus());
+        hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                           (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr, (uint32_t*)d_out,
+                           out_cap, cursor, h->lay.out, sub_shift, bkt_shift);
+        PJ_HIP(hipGetLastError());
+    }
+    record(h, 5, st);
+    unsigned long long produced = 0;
+    PJ_HIP(hipMemcpyAsync(&produced, cursor, 8, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipStreamSynchronize(st));
+    *out_n = produced;
+    pj_timings& t = h->last;
+    t.build_partition_ms = elapsed(h, 0, 1);
+    t.probe_partition_ms = elapsed(h, 2, 3);
+    t.join_ms = elapsed(h, 4, 5);
+    t.model_ms = 0.f;
+    t.total_ms = elapsed(h, 0, 5);
+    t.partitions = 1u << h->bS;
+    t.subpartitions = 1u << (h->bR - h->bS);
+    t.work_items = nitems;
+    t.used_learned = 0;
+    t.spilled_items = spilled;
+    if (produced > out_cap)
+        return fail(PJ_ERR_CAPACITY, "join produced %llu tuples, output holds %llu", produced,
+                    (unsigned long long)out_cap);
+    return PJ_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+
+extern "C" {
+
+const char* pj_last_error(void) { return pj::last_error(); }
+const char* pj_version(void) { return "pjoin 0.1 gfx950"; }
+
+int pj_join_create(const pj_join_desc* desc, pj_join_t* out)
+{
+    if (!desc || !out)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    Layout L;
+    int rc = derive_layout(desc, &L);
+    if (rc)
+        return rc;
+    if (desc->partitioner > PJ_PART_AUTO)
+        return fail(PJ_ERR_INVALID, "unknown partitioner %u", desc->partitioner);
+    pj_join_s* h = new (std::nothrow) pj_join_s();
+    if (!h)
+        return fail(PJ_ERR_INVALID, "out of host memory");
+    h->desc = *desc;
+    h->lay = L;
+    memset(&h->last, 0, sizeof(h->last));
+    h->ev_ok = true;
+    for (int i = 0; i < 8; ++i)
+        if (hipEventCreate(&h->ev[i]) != hipSuccess)
+            h->ev_ok = false;
+    *out = h;
+    return PJ_OK;
+}
+
+int pj_join_out_width(pj_join_t h, uint32_t* width)
+{
+    if (!h || !width)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *width = h->lay.out_width;
+    return PJ_OK;
+}
+
+int pj_join_destroy(pj_join_t h)
+{
+    if (!h)
+        return PJ_OK;
+    for (DevBuf* b : {&h->d_raw_build, &h->d_raw_probe, &h->d_out_batch, &h->rpart, &h->spart, &h->hist, &h->bsum,
+                      &h->starts, &h->items, &h->counters})
+        b->release();
+    if (h->ev_ok)
+        for (int i = 0; i < 8; ++i)
+            (void)hipEventDestroy(h->ev[i]);
+    delete h;
+    return PJ_OK;
+}
+
+int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
+                       void* d_out, uint64_t out_cap, uint64_t* out_n, void* stream)
+{
+    if (!h || !out_n || (nbuild && !d_build) || (nprobe && !d_probe) || (out_cap && !d_out))
+        return fail(PJ_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    hipStream_t st = as_stream(stream);
+    int rc = prepare_build(h, d_build, nbuild, st);
+    if (rc)
+        return rc;
+    return probe_device(h, d_probe, nprobe, d_out, out_cap, out_n, st);
+}
+
+int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n)
+{
+    if (!h || (n && !tuples))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (h->built)
+        return fail(PJ_ERR_STATE, "build side already finished");
+    const size_t w = (size_t)h->lay.build.tw * 4;
+    std::lock_guard<std::mutex> g(h->stage_mu);
+    const uint8_t* p = (const uint8_t*)tuples;
+    h->staged.insert(h->staged.end(), p, p + n * w);
+    return PJ_OK;
+}
+
+int pj_join_build_finish(pj_join_t h)
+{
+    if (!h)
+        return fail(PJ_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (h->built)
+        return fail(PJ_ERR_STATE, "build side already finished");
+    const size_t w = (size_t)h->lay.build.tw * 4;
+    const uint64_t n = h->staged.size() / w;
+    int rc;
+    if ((rc = h->d_raw_build.ensure(std::max<size_t>(h->staged.size(), 16))))
+        return rc;
+    if (n)
+        PJ_HIP(hipMemcpy(h->d_raw_build.p, h->staged.data(), h->staged.size(), hipMemcpyHostToDevice));
+    if ((rc = prepare_build(h, h->d_raw_build.p, n, nullptr)))
+        return rc;
+    PJ_HIP(hipStreamSynchronize(nullptr));
+    h->built = true;
+    return PJ_OK;
+}
+
+int pj_join_probe(pj_join_t h, const void* tuples, uint64_t n, pj_cursor* cur, void* out, uint64_t out_cap,
+                  uint64_t* out_n)
+{
+    if (!h || !cur || !out_n || (out_cap && !out) || (n && !tuples))
+        return fail(PJ_ERR_INVALID, "null argument");
+    *out_n = 0;
+    CursorImpl* ci = (CursorImpl*)cur->impl;
+    const uint32_t ow = h->lay.out_width;
+    if (tuples) {
+        if (!h->built)
+            return fail(PJ_ERR_STATE, "probe before pj_join_build_finish");
+        if (ci && ci->next < ci->total)
+            return fail(PJ_ERR_STATE, "cursor still holds %llu rows", (unsigned long long)(ci->total - ci->next));
+        if (!ci) {
+            ci = new CursorImpl();
+            cur->impl = ci;
+        }
+        ci->rows.clear();
+        ci->next = ci->total = 0;
+        if (n) {
+            std::lock_guard<std::mutex> g(h->mu);
+            const size_t w = (size_t)h->lay.probe.tw * 4;
+            int rc;
+            if ((rc = h->d_raw_probe.ensure(n * w)))
+                return rc;
+            PJ_HIP(hipMemcpy(h->d_raw_probe.p, tuples, n * w, hipMemcpyHostToDevice));
+            uint64_t cap = std::max<uint64_t>(n, 1024), produced = 0;
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                if ((rc = h->d_out_batch.ensure(cap * ow)))
+                    return rc;
+                rc = probe_device(h, h->d_raw_probe.p, n, h->d_out_batch.p, cap, &produced, nullptr);
+                if (rc != PJ_ERR_CAPACITY)
+                    break;
+                cap = produced;
+            }
+            if (rc)
+                return rc;
+            ci->rows.resize(produced * ow);
+            if (produced)
+                PJ_HIP(hipMemcpy(ci->rows.data(), h->d_out_batch.p, produced * ow, hipMemcpyDeviceToHost));
+            ci->total = produced;
+        }
+    }
+    if (!ci) {
+        cur->pending = 0;
+        return PJ_OK;
+    }
+    const uint64_t take = std::min<uint64_t>(out_cap, ci->total - ci->next);
+    if (take)
+        memcpy(out, ci->rows.data() + ci->next * ow, take * ow);
+    ci->next += take;
+    *out_n = take;
+    cur->pending = ci->total - ci->next;
+    return PJ_OK;
+}
+
+int pj_cursor_release(pj_cursor* cur)
+{
+    if (cur && cur->impl) {
+        delete (CursorImpl*)cur->impl;
+        cur->impl = nullptr;
+        cur->pending = 0;
+    }
+    return PJ_OK;
+}
+
+int pj_join_reset(pj_join_t h)
+{
+    if (!h)
+        return fail(PJ_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    std::lock_guard<std::mutex> g2(h->stage_mu);
+    h->staged.clear();
+    h->staged.shrink_to_fit();
+    h->built = false;
+    h->R = Partitioned();
+    return PJ_OK;
+}
+
+int pj_join_last_timings(pj_join_t h, pj_timings* t)
+{
+    if (!h || !t)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *t = h->last;
+    return PJ_OK;
+}
+
+}  // extern "C"

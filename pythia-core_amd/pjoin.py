@@ -1,0 +1,189 @@
+"""ctypes binding of libpjoin.so (include/pjoin.h).
+
+This is the binding a Python caller (tests, bench.py) uses; the C++ operators
+in host/ call the same C ABI directly. Loading fails loudly when the library
+has not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpjoin.so")
+
+PJ_OK = 0
+PJ_ERR_INVALID = -1
+PJ_ERR_UNSUPPORTED = -2
+PJ_ERR_DEVICE = -3
+PJ_ERR_CAPACITY = -4
+PJ_ERR_STATE = -5
+
+CT = {"int": 0, "long": 1, "dec": 2, "char": 3, "date": 4, "pointer": 5}
+WIDTH = {"int": 4, "long": 8, "dec": 8, "date": 8, "pointer": 8}
+PJ_MAX_COLS = 32
+PJ_MAX_KEYS = 8
+PART = {"radix": 0, "learned": 1, "auto": 2}
+PJ_FLAG_TIMING = 1
+
+
+class Schema(C.Structure):
+    _fields_ = [("ncols", C.c_uint32), ("type", C.c_uint32 * PJ_MAX_COLS),
+                ("width", C.c_uint32 * PJ_MAX_COLS)]
+
+
+class JoinDesc(C.Structure):
+    _fields_ = [("build", Schema), ("probe", Schema), ("nkeys", C.c_uint32),
+                ("buildjattr", C.c_uint16 * PJ_MAX_KEYS), ("probejattr", C.c_uint16 * PJ_MAX_KEYS),
+                ("nproj", C.c_uint32), ("proj_side", C.c_uint8 * PJ_MAX_COLS),
+                ("proj_attr", C.c_uint16 * PJ_MAX_COLS), ("partitioner", C.c_uint32),
+                ("radix_bits", C.c_int32), ("flags", C.c_uint32)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("build_partition_ms", C.c_float), ("probe_partition_ms", C.c_float),
+                ("join_ms", C.c_float), ("model_ms", C.c_float), ("total_ms", C.c_float),
+                ("partitions", C.c_uint32), ("subpartitions", C.c_uint32),
+                ("work_items", C.c_uint32), ("used_learned", C.c_uint32),
+                ("spilled_items", C.c_uint64)]
+
+
+class Cursor(C.Structure):
+    _fields_ = [("impl", C.c_void_p), ("pending", C.c_uint64)]
+
+
+class PJError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pjoin error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+_SIGS = {
+    "pj_last_error": (C.c_char_p, []),
+    "pj_version": (C.c_char_p, []),
+    "pj_join_create": (C.c_int, [C.POINTER(JoinDesc), C.POINTER(C.c_void_p)]),
+    "pj_join_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_join_destroy": (C.c_int, [C.c_void_p]),
+    "pj_join_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                     C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pj_join_build_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "pj_join_build_finish": (C.c_int, [C.c_void_p]),
+    "pj_join_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(Cursor), C.c_void_p,
+                                C.c_uint64, C.POINTER(C.c_uint64)]),
+    "pj_cursor_release": (C.c_int, [C.POINTER(Cursor)]),
+    "pj_join_reset": (C.c_int, [C.c_void_p]),
+    "pj_join_last_timings": (C.c_int, [C.c_void_p, C.POINTER(Timings)]),
+    "pj_partition_exactrange_device": (C.c_int, [C.POINTER(Schema), C.c_uint32, C.c_int64, C.c_int64,
+                                                 C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p,
+                                                 C.POINTER(C.c_uint64), C.c_void_p]),
+    "pj_checksum_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint32), C.c_void_p]),
+    "pj_gen_pkfk": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                              C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+}
+
+# Entry points added by later components; bound when the library has them.
+_OPTIONAL_SIGS = {}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpjoin.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in list(_SIGS.items()) + list(_OPTIONAL_SIGS.items()):
+            if not hasattr(L, name):
+                if name in _SIGS:
+                    raise RuntimeError("libpjoin.so lacks %s" % name)
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != PJ_OK:
+        raise PJError(rc, lib().pj_last_error().decode())
+    return rc
+
+
+def make_schema(cols):
+    """cols: list of type names ("long", "int", "dec", "date", "pointer") or
+    ("char", width) tuples."""
+    s = Schema()
+    s.ncols = len(cols)
+    for i, c in enumerate(cols):
+        if isinstance(c, tuple):
+            s.type[i] = CT[c[0]]
+            s.width[i] = c[1]
+        else:
+            s.type[i] = CT[c]
+            s.width[i] = WIDTH.get(c, 0)
+    return s
+
+
+def make_desc(build_cols, probe_cols, buildjattr, probejattr, projection,
+              partitioner="radix", radix_bits=-1, timing=False):
+    """projection: list of "B$i" / "P$j" strings, as in a plan file."""
+    d = JoinDesc()
+    d.build = make_schema(build_cols)
+    d.probe = make_schema(probe_cols)
+    d.nkeys = len(buildjattr)
+    for i, (a, b) in enumerate(zip(buildjattr, probejattr)):
+        d.buildjattr[i] = a
+        d.probejattr[i] = b
+    d.nproj = len(projection)
+    for i, p in enumerate(projection):
+        side, attr = p.split("$")
+        d.proj_side[i] = 0 if side[-1] == "B" else 1
+        d.proj_attr[i] = int(attr)
+    d.partitioner = PART[partitioner]
+    d.radix_bits = radix_bits
+    d.flags = PJ_FLAG_TIMING if timing else 0
+    return d
+
+
+class Join:
+    """Owns a pj_join_t handle."""
+
+    def __init__(self, desc):
+        self.h = C.c_void_p()
+        check(lib().pj_join_create(C.byref(desc), C.byref(self.h)))
+        w = C.c_uint32()
+        check(lib().pj_join_out_width(self.h, C.byref(w)))
+        self.out_width = w.value
+
+    def run_device(self, d_build, nbuild, d_probe, nprobe, d_out, out_cap, stream=None):
+        n = C.c_uint64()
+        rc = lib().pj_join_run_device(self.h, C.c_void_p(d_build), nbuild, C.c_void_p(d_probe), nprobe,
+                                      C.c_void_p(d_out), out_cap, C.byref(n), C.c_void_p(stream))
+        if rc == PJ_ERR_CAPACITY:
+            raise PJError(rc, "need %d output tuples" % n.value)
+        check(rc)
+        return n.value
+
+    def timings(self):
+        t = Timings()
+        check(lib().pj_join_last_timings(self.h, C.byref(t)))
+        return t
+
+    def close(self):
+        if self.h:
+            lib().pj_join_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def checksum_device(d_ptr, n, width, stream=None):
+    s = C.c_uint64()
+    x = C.c_uint32()
+    check(lib().pj_checksum_device(C.c_void_p(d_ptr), n, width, C.byref(s), C.byref(x), C.c_void_p(stream)))
+    return s.value, x.value

@@ -1,0 +1,79 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry point
+include/pjoin.h declares, and rejects bad descriptors before touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import pjoin as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            names |= set(re.findall(r"^\s*(?:int|const char\*|uint64_t|void)\s+(pj_\w+)\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = P.lib()
+    names = declared_entry_points()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in P.lib().pj_version()
+
+
+def _desc(**kw):
+    args = dict(build_cols=["long", "long"], probe_cols=["long", "long"], buildjattr=[0], probejattr=[0],
+                projection=["B$0", "B$1", "P$1"])
+    args.update(kw)
+    return P.make_desc(**args)
+
+
+def test_out_width_of_the_headline_projection():
+    j = P.Join(_desc())
+    assert j.out_width == 24
+    j.close()
+    j = P.Join(_desc(build_cols=["int", "int", "long"], probe_cols=["int", "int", "long"],
+                     buildjattr=[0, 1], probejattr=[0, 1], projection=["B$0", "B$1", "B$2", "P$2"]))
+    assert j.out_width == 24
+    j.close()
+    j = P.Join(_desc(build_cols=["long", "int"], probe_cols=["long", "int"], projection=["B$0", "B$1", "P$1"]))
+    assert j.out_width == 16
+    j.close()
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(projection=["B$5"]), P.PJ_ERR_INVALID),                      # attribute out of range
+    (dict(buildjattr=[3]), P.PJ_ERR_INVALID),
+    (dict(build_cols=["dec", "long"]), P.PJ_ERR_UNSUPPORTED),          # DECIMAL key
+    (dict(build_cols=[("char", 6), "long"]), P.PJ_ERR_UNSUPPORTED),    # CHAR key, odd width
+    (dict(build_cols=["long", "long", "long"], projection=["B$1", "B$2", "P$1", "B$0"] * 1 + ["P$0"]),
+     P.PJ_ERR_UNSUPPORTED),                                           # > 8 payload bytes a side
+])
+def test_bad_descriptors_are_rejected(kw, code):
+    h = C.c_void_p()
+    d = _desc(**kw)
+    rc = P.lib().pj_join_create(C.byref(d), C.byref(h))
+    assert rc == code
+    assert P.lib().pj_last_error()
+
+
+def test_probe_before_build_finish_is_a_state_error():
+    j = P.Join(_desc())
+    cur = P.Cursor()
+    out = (C.c_uint8 * 24)()
+    n = C.c_uint64()
+    tup = (C.c_uint64 * 2)(1, 2)
+    rc = P.lib().pj_join_probe(j.h, tup, 1, C.byref(cur), out, 1, C.byref(n))
+    assert rc == P.PJ_ERR_STATE
+    j.close()

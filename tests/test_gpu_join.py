@@ -1,0 +1,282 @@
+"""GPU parity of the device hash join against the CPU oracle.
+
+Every case compares the HIP path, called through the C ABI, with the restated
+HashJoinOp (oracle/oracle.c) or the brute-force nested-loop join on the same
+seeded inputs: bit-exact output multisets (sorted row compare) at small
+sizes, count + 64-bit checksum against the generator's closed form at the
+full 1M x 16M and 16M x 256M shapes.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pjoin as P
+from _util import pack, to_gpu, unpack
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    P.lib()
+    yield
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_join(desc, build, probe, nb, npr, cap=None):
+    """Host uint8 arrays in, output rows (n x width uint8) out."""
+    j = P.Join(desc)
+    db = to_gpu(build) if build.size else torch.zeros(16, dtype=torch.uint8, device="cuda")
+    dp = to_gpu(probe) if probe.size else torch.zeros(16, dtype=torch.uint8, device="cuda")
+    cap = cap if cap is not None else max(npr, 1)
+    out = torch.empty(max(cap, 1) * j.out_width, dtype=torch.uint8, device="cuda")
+    n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), cap,
+                     torch.cuda.current_stream().cuda_stream)
+    rows = out[: n * j.out_width].cpu().numpy().reshape(-1, j.out_width)
+    t = j.timings()
+    j.close()
+    return rows, t
+
+
+def oracle_desc(pdesc_args):
+    a = dict(pdesc_args)
+    return O.join_desc(a["build_cols"], a["probe_cols"], a["buildjattr"], a["probejattr"], a["projection"],
+                       buckets=1 << 12)
+
+
+def compare(args, build, probe, nb, npr, **kw):
+    got, t = run_join(P.make_desc(**args, **kw), build, probe, nb, npr)
+    exp, _ = O.hashjoin(oracle_desc(args), build, probe)
+    assert got.shape == exp.shape, (got.shape, exp.shape)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+    return got, t
+
+
+HEADLINE = dict(build_cols=["long", "long"], probe_cols=["long", "long"], buildjattr=[0], probejattr=[0],
+                projection=["B$0", "B$1", "P$1"])
+
+
+def test_device_generator_equals_host_generator():
+    _say("case: generator device == host")
+    nb, npr, miss = 5000, 40000, 4000
+    db = torch.empty(nb * 16, dtype=torch.uint8, device="cuda")
+    dp = torch.empty(npr * 16, dtype=torch.uint8, device="cuda")
+    P.check(P.lib().pj_gen_pkfk(9, 1 << 40, nb, npr, miss, C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()),
+                                1, None))
+    hb, hp = O.gen_pkfk(9, 1 << 40, nb, npr, miss)
+    assert np.array_equal(db.cpu().numpy(), hb.view(np.uint8))
+    assert np.array_equal(dp.cpu().numpy(), hp.view(np.uint8))
+
+
+def test_queryhashjoin_shape():
+    # unit_tests/queryhashjoin.cpp: (k, k) x (k, k + 0.1), projection B$1, P$1
+    _say("case: queryhashjoin 20 keys")
+    T = 20
+    args = dict(build_cols=["long", "long"], probe_cols=["long", "dec"], buildjattr=[0], probejattr=[0],
+                projection=["B$1", "P$1"])
+    build = pack(["long", "long"], [(k, k) for k in range(1, T + 1)])
+    probe = pack(["long", "dec"], [(k, k + 0.1) for k in range(1, T + 1)])
+    got, _ = compare(args, build, probe, T, T)
+    rows = unpack(["long", "dec"], got)
+    assert sorted(int(v) for v, _ in rows) == list(range(1, T + 1))
+    assert all(d == v + 0.1 for v, d in rows)
+
+
+@pytest.mark.parametrize("nb,npr,bits", [(1 << 16, 1 << 20, -1), (100000, 300000, 0), (100000, 300000, 14),
+                                         (3, 1000, -1), (9000, 50000, 0)])
+def test_pkfk_vs_oracle(nb, npr, bits):
+    _say("case: pkfk nb=%d np=%d bits=%d" % (nb, npr, bits))
+    b, p = O.gen_pkfk(nb + npr, 1 << 40, nb, npr, npr // 10)
+    got, t = compare(HEADLINE, b.view(np.uint8), p.view(np.uint8), nb, npr, radix_bits=bits, timing=True)
+    assert got.shape[0] == npr - npr // 10
+
+
+def test_many_to_many_cross_width_keys():
+    _say("case: many-to-many int/long keys")
+    rng = np.random.default_rng(31)
+    nb, npr = 3000, 20000
+    bk = rng.integers(-300, 300, nb)
+    pk = rng.integers(-300, 300, npr)
+    build = pack(["int", "long", "int"], [(int(k), int(v), int(w)) for k, v, w in
+                                          zip(bk, rng.integers(0, 1 << 62, nb), rng.integers(0, 1 << 31, nb))])
+    probe = pack(["long", "int"], [(int(k), int(v)) for k, v in zip(pk, rng.integers(-(1 << 31), 1 << 31, npr))])
+    args = dict(build_cols=["int", "long", "int"], probe_cols=["long", "int"], buildjattr=[0], probejattr=[0],
+                projection=["P$1", "B$1", "B$0", "P$0"])
+    got, t = compare(args, build, probe, nb, npr)
+    assert got.shape[0] > 100000
+
+
+def test_composite_int_keys():
+    _say("case: composite (int, int) keys")
+    rng = np.random.default_rng(32)
+    n = 40000
+    cols = ["int", "int", "long"]
+    build = pack(cols, [(int(a), int(b), int(c)) for a, b, c in
+                        zip(rng.integers(0, 50, n), rng.integers(0, 50, n), rng.integers(0, 1 << 62, n))])
+    probe = pack(cols, [(int(a), int(b), int(c)) for a, b, c in
+                        zip(rng.integers(0, 60, n), rng.integers(0, 60, n), rng.integers(0, 1 << 62, n))])
+    args = dict(build_cols=cols, probe_cols=cols, buildjattr=[0, 1], probejattr=[0, 1],
+                projection=["B$0", "B$1", "B$2", "P$2"])
+    got, _ = compare(args, build, probe, n, n)
+    assert got.shape[0] > 100000
+
+
+def test_twelve_byte_tuples():
+    _say("case: 12-byte tuples (long key, int payload)")
+    rng = np.random.default_rng(33)
+    nb, npr = 50000, 200000
+    keys = rng.choice(1 << 40, nb, replace=False)
+    build = pack(["long", "int"], [(int(k), int(v)) for k, v in zip(keys, rng.integers(0, 1 << 31, nb))])
+    fk = keys[rng.integers(0, nb, npr)]
+    probe = pack(["long", "int"], [(int(k), int(v)) for k, v in zip(fk, rng.integers(0, 1 << 31, npr))])
+    args = dict(build_cols=["long", "int"], probe_cols=["long", "int"], buildjattr=[0], probejattr=[0],
+                projection=["B$0", "B$1", "P$1"])
+    got, _ = compare(args, build, probe, nb, npr)
+    assert got.shape[0] == npr
+
+
+def test_empty_and_all_miss():
+    _say("case: empty / all-miss")
+    b, p = O.gen_pkfk(3, 1 << 40, 1000, 5000, 5000)
+    e = np.zeros(0, np.uint8)
+    assert run_join(P.make_desc(**HEADLINE), e, p.view(np.uint8), 0, 5000)[0].shape[0] == 0
+    assert run_join(P.make_desc(**HEADLINE), b.view(np.uint8), e, 1000, 0)[0].shape[0] == 0
+    assert run_join(P.make_desc(**HEADLINE), b.view(np.uint8), p.view(np.uint8), 1000, 5000)[0].shape[0] == 0
+
+
+def test_heavy_duplicate_build_key_spills_lds():
+    # one key 20000 times: one partition larger than an LDS table, joined by
+    # splitting the build side; every probe of that key meets all copies
+    _say("case: heavy duplicate build key")
+    nb, npr = 20000, 12
+    build = np.stack([np.full(nb, 77, np.int64), np.arange(nb, dtype=np.int64)], 1).reshape(-1)
+    probe = np.stack([np.array([77] * 10 + [5, 6], np.int64), np.arange(npr, dtype=np.int64)], 1).reshape(-1)
+    got, t = run_join(P.make_desc(**HEADLINE, timing=True), build.view(np.uint8), probe.view(np.uint8), nb, npr,
+                      cap=nb * 10)
+    assert got.shape[0] == nb * 10
+    assert t.spilled_items > 0
+    exp, _ = O.hashjoin(oracle_desc(HEADLINE), build.view(np.uint8), probe.view(np.uint8), out_cap=nb * 10)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+def test_output_capacity_error_reports_need():
+    _say("case: capacity error")
+    b, p = O.gen_pkfk(4, 1 << 40, 1000, 5000, 0)
+    with pytest.raises(P.PJError) as e:
+        run_join(P.make_desc(**HEADLINE), b.view(np.uint8), p.view(np.uint8), 1000, 5000, cap=100)
+    assert e.value.code == P.PJ_ERR_CAPACITY and "5000" in str(e.value)
+
+
+def test_checksum_device_matches_oracle():
+    _say("case: device checksum")
+    rng = np.random.default_rng(5)
+    for w in (16, 24, 12):
+        rows = rng.integers(0, 256, size=(100003, w), dtype=np.uint8)
+        d = to_gpu(rows)
+        assert P.checksum_device(d.data_ptr(), rows.shape[0], w) == O.checksum(rows, w)
+
+
+def test_partition_exactrange_vs_oracle():
+    # PartitionOp: per-partition multisets equal the restated PartitionOp's
+    _say("case: exactrange partition")
+    T = 128 * 1024
+    rng = np.random.default_rng(6)
+    vals = rng.integers(1, T + 1, size=2 * T)
+    rows = np.stack([vals, vals ^ 0xABCDEF], axis=1).astype(np.int64)
+    for threads in (1, 5, 15, 127):
+        exp = O.partition(O.schema(["long", "long"]), 0, 1, T, [rows.view(np.uint8).reshape(-1)])
+        exp_multi = O.partition(O.schema(["long", "long"]), 0, 1, T,
+                                [s.view(np.uint8).reshape(-1) for s in np.array_split(rows, threads)])
+        din = to_gpu(rows)
+        dout = torch.empty_like(din)
+        sizes = (C.c_uint64 * threads)()
+        sch = P.make_schema(["long", "long"])
+        P.check(P.lib().pj_partition_exactrange_device(C.byref(sch), 0, 1, T, threads, C.c_void_p(din.data_ptr()),
+                                                       rows.shape[0], C.c_void_p(dout.data_ptr()), sizes, None))
+        out = dout.cpu().numpy()
+        o = 0
+        for j in range(threads):
+            part = out[o * 16:(o + sizes[j]) * 16].reshape(-1, 16)
+            ref = exp_multi[j].reshape(-1, 16)
+            assert np.array_equal(O.sorted_rows(part), O.sorted_rows(ref))
+            o += sizes[j]
+        assert o == rows.shape[0]
+        del exp
+
+
+def test_page_protocol_resumes_like_getnext():
+    # HashJoinOp::getNext returns full pages and resumes (join.cpp:716-719):
+    # staged build, batched probe, output drained a few tuples at a time
+    _say("case: page protocol")
+    b, p = O.gen_pkfk(8, 1 << 40, 30000, 100000, 5000)
+    j = P.Join(P.make_desc(**HEADLINE))
+    L = P.lib()
+    bb = b.view(np.uint8)
+    for chunk in np.array_split(bb.reshape(-1, 16), 7):
+        c = np.ascontiguousarray(chunk)
+        P.check(L.pj_join_build_append(j.h, c.ctypes.data_as(C.c_void_p), c.shape[0]))
+    P.check(L.pj_join_build_finish(j.h))
+    rows = []
+    cur = P.Cursor()
+    page = np.zeros(24 * 1000, np.uint8)
+    n = C.c_uint64()
+    for batch in np.array_split(p.view(np.uint8).reshape(-1, 16), 3):
+        bt = np.ascontiguousarray(batch)
+        P.check(L.pj_join_probe(j.h, bt.ctypes.data_as(C.c_void_p), bt.shape[0], C.byref(cur),
+                                page.ctypes.data_as(C.c_void_p), 1000, C.byref(n)))
+        rows.append(page[: n.value * 24].copy())
+        while cur.pending:
+            P.check(L.pj_join_probe(j.h, None, 0, C.byref(cur), page.ctypes.data_as(C.c_void_p), 1000,
+                                    C.byref(n)))
+            assert n.value <= 1000
+            rows.append(page[: n.value * 24].copy())
+    L.pj_cursor_release(C.byref(cur))
+    j.close()
+    got = np.concatenate(rows).reshape(-1, 24)
+    exp, _ = O.hashjoin(oracle_desc(HEADLINE), bb, p.view(np.uint8))
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+def _full_pkfk(seed, dom, nb, npr, nmiss):
+    db = torch.empty(nb * 16, dtype=torch.uint8, device="cuda")
+    dp = torch.empty(npr * 16, dtype=torch.uint8, device="cuda")
+    P.check(P.lib().pj_gen_pkfk(seed, dom, nb, npr, nmiss, C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()),
+                                1, None))
+    j = P.Join(P.make_desc(**HEADLINE, timing=True))
+    out = torch.empty(npr * 24, dtype=torch.uint8, device="cuda")
+    n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr)
+    s, x = P.checksum_device(out.data_ptr(), n, 24)
+    j.close()
+    del db, dp, out
+    torch.cuda.empty_cache()
+    return n, s
+
+
+def test_full_1m_x_16m_dense_keys():
+    # queryhashjoin shape at 1M x 16M: keys 0..1M-1, every probe matches
+    _say("case: 1M x 16M")
+    nb, npr = 1 << 20, 1 << 24
+    n, s = _full_pkfk(101, nb, nb, npr, 0)
+    es, ec = O.expected_pkfk(101, nb, nb, npr, 0, threads=16)
+    assert (n, s) == (ec, es)
+
+
+def test_full_16m_x_256m_uniform():
+    # 16M x 256M, keys from [0, 2^40), 10% of probes miss
+    _say("case: 16M x 256M")
+    nb, npr = 1 << 24, 1 << 28
+    nmiss = round(0.1 * npr)
+    n, s = _full_pkfk(202, 1 << 40, nb, npr, nmiss)
+    es, ec = O.expected_pkfk(202, 1 << 40, nb, npr, nmiss, threads=16)
+    assert ec == npr - nmiss
+    assert (n, s) == (ec, es)

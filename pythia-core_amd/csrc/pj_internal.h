@@ -58,7 +58,7 @@ int derive_layout(const pj_join_desc* d, Layout* L);
 
 // ---------------------------------------------------------------- device
 constexpr uint32_t kJoinThreads = 1024;
-constexpr uint32_t kTableCap = 8960;       // records per LDS table (140 KiB)
+constexpr uint32_t kTableCap = 8944;       // records per LDS table (~140 KiB)
 constexpr uint32_t kTableBuckets = 4096;   // bucket directory entries
 constexpr uint32_t kLdsBytes = 163840;     // gfx950 LDS per CU
 constexpr uint32_t kScatterThreads = 512;
@@ -66,7 +66,8 @@ constexpr uint32_t kScatterThreads = 512;
 struct Item {           // one join work item
     uint32_t r_off, r_cnt;   // build records [r_off, r_off + r_cnt) of R_part
     uint32_t s_off, s_cnt;   // probe records of S_part
-    uint32_t sub_mask, sub_val;  // probe filter on the R sub-partition bits
+    uint32_t part;           // build partition id; records with other ids are skipped
+    uint32_t unused;
 };
 
 }  // namespace pj

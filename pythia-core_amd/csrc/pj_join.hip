@@ -74,15 +74,22 @@ __device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint6
 // ======================================================================
 // partition pass: histogram, scan, scatter
 // ======================================================================
+//
+// A partition's records from one block form a segment padded to a whole
+// number of B-record chunks, so the write-combining scatter below only ever
+// writes whole, aligned chunks. Padding records carry a key of another
+// partition; the join drops any record whose partition bits are not the
+// work item's.
 
 constexpr int kHistThreads = 512;
 constexpr int kHistItems = 8;
 
-// hist[p * nblk + blk] = tuples of block blk's chunk in partition p.
+// hist[p * nblk + blk] = tuples of block blk's chunk in partition p, rounded
+// up to a multiple of pad.
 template <bool IDENT>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ in, uint64_t n,
                                                         SideLayout L, uint32_t bits, uint64_t chunk,
-                                                        uint32_t nblk, uint32_t* __restrict__ hist)
+                                                        uint32_t nblk, uint32_t pad, uint32_t* __restrict__ hist)
 {
     extern __shared__ uint32_t cnt[];
     const uint32_t F = 1u << bits;
@@ -108,25 +115,103 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     }
     __syncthreads();
     for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
-        hist[(uint64_t)p * nblk + blockIdx.x] = cnt[p];
+        hist[(uint64_t)p * nblk + blockIdx.x] = (cnt[p] + pad - 1) / pad * pad;
 }
 
-// part_start[p] = first record of partition p; part_start[F] = n.
-__global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uint32_t nblk, uint32_t n,
-                              uint32_t* __restrict__ part_start)
+// part_start[p] = first record of partition p; part_start[F] = total.
+__global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uint32_t nblk,
+                              const uint32_t* __restrict__ total, uint32_t* __restrict__ part_start)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < F)
         part_start[p] = offs[(uint64_t)p * nblk];
     if (p == F)
-        part_start[F] = n;
+        part_start[F] = *total;
+}
+
+constexpr int kWcThreads = 1024;
+constexpr int kWcItems = 4;
+
+// Software write-combining scatter: every partition owns a B-record buffer
+// in LDS; a full buffer leaves as one aligned B * 16-byte chunk, the lanes of
+// a B-lane group writing one record each. Threads whose record finds its
+// buffer full retry after the flush.
+template <bool IDENT, int B>
+__global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
+                                                            SideLayout L, uint32_t bits, uint64_t chunk,
+                                                            uint32_t nblk, const uint32_t* __restrict__ offs,
+                                                            uint4* __restrict__ out, uint4 pad_other, uint4 pad_zero)
+{
+    extern __shared__ uint4 sm4[];
+    const uint32_t F = 1u << bits;
+    uint4* buf = sm4;                                           // F * B
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + F * B);  // F
+    uint32_t* goff = cnt + F;                                   // F
+    for (uint32_t p = threadIdx.x; p < F; p += kWcThreads) {
+        goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
+        cnt[p] = 0;
+    }
+    __syncthreads();
+    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint64_t end = beg + chunk < n ? beg + chunk : n;
+    const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
+    constexpr uint32_t ngrp = kWcThreads / B;
+    for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * kWcItems) {
+        uint4 r[kWcItems];
+        uint32_t p[kWcItems];
+        bool pend[kWcItems];
+#pragma unroll
+        for (int k = 0; k < kWcItems; ++k) {
+            const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
+            pend[k] = i < end;
+            if (pend[k])
+                r[k] = load_rec<IDENT>(in, i, L);
+        }
+#pragma unroll
+        for (int k = 0; k < kWcItems; ++k)
+            if (pend[k])
+                p[k] = radix_part(key_mix(rec_key(r[k])), bits);
+        for (;;) {
+            bool more = false;
+#pragma unroll
+            for (int k = 0; k < kWcItems; ++k)
+                if (pend[k]) {
+                    const uint32_t s = atomicAdd(&cnt[p[k]], 1u);
+                    if (s < (uint32_t)B) {
+                        buf[p[k] * B + s] = r[k];
+                        pend[k] = false;
+                    } else {
+                        more = true;
+                    }
+                }
+            const int again = __syncthreads_or(more);
+            for (uint32_t q = grp; q < F; q += ngrp)
+                if (cnt[q] >= (uint32_t)B)
+                    out[goff[q] + gl] = buf[q * B + gl];
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < F; q += kWcThreads)
+                if (cnt[q] >= (uint32_t)B) {
+                    cnt[q] = 0;
+                    goff[q] += B;
+                }
+            __syncthreads();
+            if (!again)
+                break;
+        }
+    }
+    // partial buffers leave padded to a whole chunk
+    for (uint32_t q = grp; q < F; q += ngrp) {
+        const uint32_t c = cnt[q];
+        if (c > 0)
+            out[goff[q] + gl] = gl < c ? buf[q * B + gl] : (q == 0 ? pad_zero : pad_other);
+    }
 }
 
 constexpr int kScatterItems = 16;
 
-// Scatter with an LDS-sorted tile: each tile of up to ts tuples is counting-
-// sorted by partition in LDS, then written out so that lanes of a wave store
-// consecutive records of one partition.
+// Scatter with an LDS-sorted tile, for fan-outs too wide for one LDS buffer
+// per partition: each tile is counting-sorted by partition in LDS, then
+// written so that lanes of a wave store consecutive records of a partition.
 template <bool IDENT>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
                                                               SideLayout L, uint32_t bits, uint64_t chunk,
@@ -189,16 +274,15 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
 // join: LDS-resident bucket table per build partition, streaming probe
 // ======================================================================
 
-constexpr int kProbeItems = 4;
-constexpr uint32_t kBuildPerThread = (kTableCap + kJoinThreads - 1) / kJoinThreads;
+constexpr int kProbeItems = 8;
 
-__device__ __forceinline__ uint32_t pick_word(const uint4& b, const uint4& s, uint32_t src)
+__device__ __forceinline__ uint32_t pick_word(uint64_t bk, uint2 bp, const uint4& s, uint32_t src)
 {
     switch (src) {
-    case SRC_BK0: return b.x;
-    case SRC_BK1: return b.y;
-    case SRC_BP0: return b.z;
-    case SRC_BP1: return b.w;
+    case SRC_BK0: return (uint32_t)bk;
+    case SRC_BK1: return (uint32_t)(bk >> 32);
+    case SRC_BP0: return bp.x;
+    case SRC_BP1: return bp.y;
     case SRC_PK0: return s.x;
     case SRC_PK1: return s.y;
     case SRC_PP0: return s.z;
@@ -206,142 +290,212 @@ __device__ __forceinline__ uint32_t pick_word(const uint4& b, const uint4& s, ui
     }
 }
 
-__device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t row, const OutFormat& f,
-                                          const uint4& b, const uint4& s)
+__device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t row, const OutFormat& f, bool kbp,
+                                          uint64_t bk, uint2 bp, const uint4& s)
 {
     uint32_t* o = out + row * f.words;
-    if (f.words == 6 && f.src[0] == SRC_BK0 && f.src[1] == SRC_BK1 && f.src[2] == SRC_BP0 &&
-        f.src[3] == SRC_BP1 && f.src[4] == SRC_PP0 && f.src[5] == SRC_PP1) {
-        st_stream2(o, b.x, b.y);
-        st_stream2(o + 2, b.z, b.w);
+    if (kbp) {   // (build key, build payload, probe payload), 24 bytes
+        st_stream2(o, (uint32_t)bk, (uint32_t)(bk >> 32));
+        st_stream2(o + 2, bp.x, bp.y);
         st_stream2(o + 4, s.z, s.w);
         return;
     }
 #pragma unroll
     for (int w = 0; w < 8; ++w)
         if ((uint32_t)w < f.words)
-            o[w] = pick_word(b, s, f.src[w]);
+            o[w] = pick_word(bk, bp, s, f.src[w]);
 }
 
+// Work items come from a shared counter. Per item the block builds the LDS
+// table of its build records: a counting sort by bucket, keys and payloads
+// in separate arrays, bucket starts as 16-bit offsets. The probe runs in
+// rounds of 8192 records: every wave probes a 512-record slice, the block
+// reserves the round's output rows with one atomic, and the lanes of each
+// wave write adjacent output rows.
 __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
                                                         const Item* __restrict__ items, uint32_t nitems,
                                                         uint32_t* __restrict__ item_ctr,
                                                         uint32_t* __restrict__ out, uint64_t out_cap,
                                                         unsigned long long* __restrict__ out_cursor,
-                                                        OutFormat fmt, uint32_t sub_shift, uint32_t bkt_shift)
+                                                        OutFormat fmt, uint32_t bitsR, uint32_t bkt_shift)
 {
-    extern __shared__ uint4 lds[];
-    uint4* rec = lds;                                             // kTableCap
-    uint32_t* dir = reinterpret_cast<uint32_t*>(rec + kTableCap);  // kTableBuckets + 1
-    uint32_t* wsum = dir + kTableBuckets + 1;                     // 17
-    uint32_t* misc = wsum + 17;                                   // [0] item, [1..2] base
-    const uint32_t tid = threadIdx.x;
+    extern __shared__ uint64_t lds64[];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint64_t* tkey = lds64;                                           // kTableCap
+    uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);         // kTableCap
+    uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);   // kTableBuckets / 2 + 1
+    const uint16_t* dir = reinterpret_cast<const uint16_t*>(dirw);    // bucket ends
+    uint32_t* wsum = dirw + kTableBuckets / 2 + 1;                    // 17
+    uint32_t* misc = wsum + 17;                                       // [0] item, [2..33] wave bases
+    const bool kbp = fmt.words == 6 && fmt.src[0] == SRC_BK0 && fmt.src[1] == SRC_BK1 && fmt.src[2] == SRC_BP0 &&
+                     fmt.src[3] == SRC_BP1 && fmt.src[4] == SRC_PP0 && fmt.src[5] == SRC_PP1;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 
     for (;;) {
         if (tid == 0)
             misc[0] = atomicAdd(item_ctr, 1u);
         __syncthreads();
         const uint32_t it = misc[0];
-        __syncthreads();
         if (it >= nitems)
             break;
         const Item I = items[it];
 
-        // ---- build: counting sort of the partition's records by bucket
-        for (uint32_t b = tid; b <= kTableBuckets; b += kJoinThreads)
-            dir[b] = 0;
+        // ---- build: counting sort of the item's build records by bucket.
+        // Pass 1 counts, a scan turns counts into bucket starts, pass 2
+        // places each record at its bucket's cursor; afterwards entry b holds
+        // the end of bucket b. Two 16-bit entries share a 32-bit word.
+        for (uint32_t w = tid; w <= kTableBuckets / 2; w += kJoinThreads)
+            dirw[w] = 0;
         __syncthreads();
-        uint4 br[kBuildPerThread];
-        uint32_t bb[kBuildPerThread], brank[kBuildPerThread];
-#pragma unroll
-        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
-            uint32_t i = k * kJoinThreads + tid;
-            if (i < I.r_cnt)
-                br[k] = R[I.r_off + i];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
-            uint32_t i = k * kJoinThreads + tid;
-            if (i < I.r_cnt) {
-                bb[k] = (uint32_t)(key_mix(rec_key(br[k])) >> bkt_shift) & (kTableBuckets - 1);
-                brank[k] = atomicAdd(&dir[bb[k]], 1u);
+        for (uint32_t i = tid; i < I.r_cnt; i += kJoinThreads) {
+            const uint64_t h = key_mix(rec_key(R[I.r_off + i]));
+            if (radix_part(h, bitsR) == I.part) {   // drops padding records
+                const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
+                atomicAdd(&dirw[b >> 1], 1u << ((b & 1u) * 16u));
             }
         }
         __syncthreads();
-        block_scan_array<kJoinThreads>(dir, kTableBuckets, wsum);
-#pragma unroll
-        for (uint32_t k = 0; k < kBuildPerThread; ++k) {
-            uint32_t i = k * kJoinThreads + tid;
-            if (i < I.r_cnt)
-                rec[dir[bb[k]] + brank[k]] = br[k];
+        {
+            static_assert(kTableBuckets == 4 * kJoinThreads, "scan layout");
+            const uint32_t w0 = dirw[2 * tid], w1 = dirw[2 * tid + 1];
+            const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16, c2 = w1 & 0xFFFFu, c3 = w1 >> 16;
+            uint32_t total;
+            const uint32_t pre = block_excl_scan<kJoinThreads>(c0 + c1 + c2 + c3, wsum, &total);
+            dirw[2 * tid] = pre | ((pre + c0) << 16);
+            dirw[2 * tid + 1] = (pre + c0 + c1) | ((pre + c0 + c1 + c2) << 16);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < I.r_cnt; i += kJoinThreads) {
+            const uint4 r = R[I.r_off + i];
+            const uint64_t h = key_mix(rec_key(r));
+            if (radix_part(h, bitsR) == I.part) {
+                const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
+                const uint32_t sh = (b & 1u) * 16u;
+                const uint32_t j = (atomicAdd(&dirw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+                tkey[j] = rec_key(r);
+                tpay[j] = make_uint2(r.z, r.w);
+            }
         }
         __syncthreads();
 
-        // ---- probe
+        // ---- probe, software-pipelined: a record's slot is refilled with
+        // the next round's record as soon as its lookup is done
         const uint4* Sp = S + I.s_off;
-        for (uint32_t t0 = 0; t0 < I.s_cnt; t0 += kJoinThreads * kProbeItems) {
+        constexpr uint32_t kSlice = 64 * kProbeItems;
+        constexpr uint32_t kRound = (kJoinThreads / 64) * kSlice;
+        uint4 nxt[kProbeItems];
+#pragma unroll
+        for (int k = 0; k < kProbeItems; ++k) {
+            const uint32_t i = wave * kSlice + k * 64 + lane;
+            if (i < I.s_cnt)
+                nxt[k] = ld_stream(Sp + i);
+        }
+        for (uint32_t r0 = 0; r0 < I.s_cnt; r0 += kRound) {
+            const uint32_t t0 = r0 + wave * kSlice;
             uint4 s[kProbeItems];
             uint32_t nm[kProbeItems], first[kProbeItems];
+            bool multi = false;
 #pragma unroll
             for (int k = 0; k < kProbeItems; ++k) {
-                uint32_t i = t0 + k * kJoinThreads + tid;
-                if (i < I.s_cnt)
-                    s[k] = ld_stream(Sp + i);
-            }
-#pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
-                uint32_t i = t0 + k * kJoinThreads + tid;
+                const uint32_t i = t0 + k * 64 + lane;
+                s[k] = nxt[k];
                 nm[k] = 0;
                 first[k] = 0;
                 if (i < I.s_cnt) {
                     const uint64_t key = rec_key(s[k]);
                     const uint64_t h = key_mix(key);
-                    if (I.sub_mask == 0 || (((uint32_t)(h >> sub_shift)) & I.sub_mask) == I.sub_val) {
+                    if (radix_part(h, bitsR) == I.part) {
                         const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
-                        const uint32_t lo = dir[b], hi = dir[b + 1];
-                        for (uint32_t j = lo; j < hi; ++j) {
-                            const uint4 c = rec[j];
-                            if (rec_key(c) == key) {
+                        const uint32_t lo = b ? dir[b - 1] : 0u, hi = dir[b];
+                        for (uint32_t j = lo; j < hi; ++j)
+                            if (tkey[j] == key) {
                                 if (nm[k] == 0)
                                     first[k] = j;
                                 nm[k]++;
                             }
-                        }
                     }
                 }
+                multi |= nm[k] > 1;
+                if (i + kRound < I.s_cnt)
+                    nxt[k] = ld_stream(Sp + i + kRound);
             }
-            // one output reservation per probe slot k for the whole block,
-            // so lanes of a wave write adjacent tuples
+            const bool wmulti = __any(multi);
+            uint64_t m[kProbeItems];
+            uint32_t mine = 0, pre = 0, wtotal = 0;
+            if (!wmulti) {
 #pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
-                uint32_t total;
-                uint32_t pre = block_excl_scan<kJoinThreads>(nm[k], wsum, &total);
-                if (total == 0)
-                    continue;
-                if (tid == 0) {
-                    unsigned long long base = atomicAdd(out_cursor, (unsigned long long)total);
-                    misc[1] = (uint32_t)base;
-                    misc[2] = (uint32_t)(base >> 32);
+                for (int k = 0; k < kProbeItems; ++k) {
+                    m[k] = __ballot(nm[k] != 0);
+                    wtotal += (uint32_t)__popcll(m[k]);
                 }
-                __syncthreads();
-                const uint64_t base = (uint64_t)misc[1] | ((uint64_t)misc[2] << 32);
-                __syncthreads();
-                if (nm[k]) {
-                    uint64_t row = base + pre;
+            } else {
+#pragma unroll
+                for (int k = 0; k < kProbeItems; ++k)
+                    mine += nm[k];
+                uint32_t x = mine;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t y = __shfl_up(x, d, 64);
+                    if (lane >= (uint32_t)d)
+                        x += y;
+                }
+                pre = x - mine;
+                wtotal = __shfl(x, 63);
+            }
+            if (lane == 0)
+                wsum[wave] = wtotal;
+            __syncthreads();
+            if (wave == 0) {
+                const uint32_t v = lane < kJoinThreads / 64 ? wsum[lane] : 0u;
+                uint32_t x = v;
+#pragma unroll
+                for (int d = 1; d < (int)(kJoinThreads / 64); d <<= 1) {
+                    const uint32_t y = __shfl_up(x, d, 64);
+                    if (lane >= (uint32_t)d)
+                        x += y;
+                }
+                const uint32_t btotal = __shfl(x, kJoinThreads / 64 - 1);
+                unsigned long long base = 0;
+                if (lane == 0 && btotal)
+                    base = atomicAdd(out_cursor, (unsigned long long)btotal);
+                base = __shfl(base, 0);
+                if (lane < kJoinThreads / 64) {
+                    const uint64_t wb = base + (x - v);
+                    misc[2 + 2 * lane] = (uint32_t)wb;
+                    misc[3 + 2 * lane] = (uint32_t)(wb >> 32);
+                }
+            }
+            __syncthreads();
+            if (!wtotal)
+                continue;
+            const uint64_t wbase = (uint64_t)misc[2 + 2 * wave] | ((uint64_t)misc[3 + 2 * wave] << 32);
+            if (wmulti) {
+                // many-to-many: a lane writes all matches of its probes
+                uint64_t row = wbase + pre;
+#pragma unroll
+                for (int k = 0; k < kProbeItems; ++k) {
                     const uint64_t key = rec_key(s[k]);
-                    uint32_t j = first[k];
                     uint32_t left = nm[k];
-                    while (left) {
-                        const uint4 c = rec[j];
-                        if (rec_key(c) == key) {
+                    for (uint32_t j = first[k]; left; ++j)
+                        if (tkey[j] == key) {
                             if (row < out_cap)
-                                write_row(out, row, fmt, c, s[k]);
+                                write_row(out, row, fmt, false, tkey[j], tpay[j], s[k]);
                             row++;
                             left--;
                         }
-                        j++;
-                    }
                 }
+                continue;
+            }
+            // at most one match a probe: lanes of a wave write adjacent rows
+            uint64_t row0 = wbase;
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                if (nm[k]) {
+                    const uint64_t row = row0 + (uint64_t)__popcll(m[k] & below);
+                    if (row < out_cap)
+                        write_row(out, row, fmt, kbp, tkey[first[k]], tpay[first[k]], s[k]);
+                }
+                row0 += (uint64_t)__popcll(m[k]);
             }
         }
         __syncthreads();
@@ -616,6 +770,20 @@ uint32_t scatter_tile(uint32_t bits)
     return ts > 0 ? (uint32_t)ts : 0;
 }
 
+// Records per write-combining chunk for a fan-out of 2^bits: the largest
+// power of two (at most 16) whose buffers fit in 128 KiB of LDS; 1 selects
+// the tile-sorted scatter (no padding).
+uint32_t wc_chunk(uint32_t bits)
+{
+    if (bits == 0)
+        return 1;
+    const uint32_t F = 1u << bits;
+    uint32_t B = 16;
+    while (B > 1 && (uint64_t)F * B * 16 > 131072)
+        B >>= 1;
+    return B < 2 ? 1 : B;
+}
+
 uint32_t hist_blocks(uint64_t n)
 {
     uint64_t b = (n + 65535) / 65536;  // ~64K tuples per block
@@ -626,9 +794,43 @@ uint32_t hist_blocks(uint64_t n)
     return (uint32_t)b;
 }
 
-// Partitions raw tuples of one side into dst (canonical records).
+// A key whose top bit is set: its partition is never 0 (for bits >= 1), so
+// it pads partition 0; key 0 mixes to 0 and pads every other partition.
+uint64_t pad_key_nonzero_part()
+{
+    for (uint64_t k = 1;; ++k)
+        if (pj_fin64(k) >> 63)
+            return k;
+}
+
+template <int B>
+int launch_wc(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, uint32_t bits, uint64_t chunk,
+              uint32_t nblk, const uint32_t* offs, uint4* out, hipStream_t st)
+{
+    const uint32_t F = 1u << bits;
+    const size_t lds = (size_t)F * B * 16 + 2 * F * 4;
+    const uint64_t pk = pad_key_nonzero_part();
+    const uint4 pad_other = make_uint4(0, 0, 0, 0);
+    const uint4 pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
+    if (ident) {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL((k_scatter_wc<true, B>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, bits, chunk,
+                           nblk, offs, out, pad_other, pad_zero);
+    } else {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+        hipLaunchKernelGGL((k_scatter_wc<false, B>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, bits, chunk,
+                           nblk, offs, out, pad_other, pad_zero);
+    }
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
+}
+
+// Partitions raw tuples of one side into dst (canonical records, each
+// block's share of a partition padded to whole write-combining chunks).
 int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits,
-                   DevBuf& dst, Partitioned& P, hipStream_t st)
+                   bool write_combine, DevBuf& dst, Partitioned& P, hipStream_t st)
 {
     int rc;
     P.bits = bits;
@@ -637,44 +839,63 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     P.start.assign(F + 1, 0);
     if (n == 0)
         return PJ_OK;
-    if (n >= (1ull << 32))
-        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^32 tuples or more");
+    if (n >= (1ull << 31))
+        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
     const uint32_t nblk = hist_blocks(n);
     const uint64_t chunk = (n + nblk - 1) / nblk;
     const uint64_t m = (uint64_t)F * nblk;
     const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
-    if ((rc = dst.ensure(n * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)) ||
+    const uint32_t B = write_combine ? wc_chunk(bits) : 1;
+    const uint64_t cap = n + (uint64_t)F * nblk * (B - 1);
+    if (cap >= (1ull << 32))
+        return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");
+    if ((rc = dst.ensure(cap * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)) ||
         (rc = h->starts.ensure((F + 1) * 4)))
         return rc;
     const uint32_t* in = (const uint32_t*)d_in;
     const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
     uint32_t* hist = (uint32_t*)h->hist.p;
+    uint32_t* bsum = (uint32_t*)h->bsum.p;
     if (ident)
-        hipLaunchKernelGGL(k_hist<true>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, hist);
+        hipLaunchKernelGGL(k_hist<true>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, B,
+                           hist);
     else
-        hipLaunchKernelGGL(k_hist<false>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk,
+        hipLaunchKernelGGL(k_hist<false>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, B,
                            hist);
     PJ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (uint32_t*)h->bsum.p);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, (uint32_t*)h->bsum.p, nsb);
-    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)h->bsum.p);
-    hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk, (uint32_t)n,
-                       (uint32_t*)h->starts.p);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)bsum);
+    hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk,
+                       (const uint32_t*)(bsum + nsb), (uint32_t*)h->starts.p);
     PJ_HIP(hipGetLastError());
-    const uint32_t ts = scatter_tile(bits);
-    if (ts == 0)
-        return fail(PJ_ERR_UNSUPPORTED, "radix bits %u exceed the scatter's LDS budget", bits);
-    const size_t lds = (size_t)ts * 16 + (2 * F + 1) * 4 + 16 * 4;
-    if (ident) {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_scatter<true>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk, nblk,
-                           hist, (uint4*)dst.p, ts);
-    } else {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_scatter<false>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk, nblk,
-                           hist, (uint4*)dst.p, ts);
+    uint4* out = (uint4*)dst.p;
+    switch (B) {
+    case 16: rc = launch_wc<16>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
+    case 8: rc = launch_wc<8>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
+    case 4: rc = launch_wc<4>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
+    case 2: rc = launch_wc<2>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
+    default: {
+        const uint32_t ts = scatter_tile(bits);
+        if (ts == 0)
+            return fail(PJ_ERR_UNSUPPORTED, "radix bits %u exceed the scatter's LDS budget", bits);
+        const size_t lds = (size_t)ts * 16 + (2 * F + 1) * 4 + 16 * 4;
+        if (ident) {
+            PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
+            hipLaunchKernelGGL(k_scatter<true>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk,
+                               nblk, hist, out, ts);
+        } else {
+            PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds));
+            hipLaunchKernelGGL(k_scatter<false>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk,
+                               nblk, hist, out, ts);
+        }
+        PJ_HIP(hipGetLastError());
     }
-    PJ_HIP(hipGetLastError());
+    }
+    if (rc)
+        return rc;
     PJ_HIP(hipMemcpyAsync(P.start.data(), h->starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
     return PJ_OK;
 }
@@ -693,42 +914,39 @@ int device_count_cus()
     return cus;
 }
 
-// Builds the work list: for every probe partition and every build
-// sub-partition under it, the build records (split by position when they
-// exceed one LDS table) times chunks of the probe records.
+// Builds the work list: for every build partition (a sub-partition of a
+// probe partition when R uses more radix bits than S), its records (split
+// by position when they exceed one LDS table) times chunks of the probe
+// partition's records.
 void plan_items(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
 {
     items.clear();
     *spilled = 0;
-    const uint32_t FS = 1u << h->bS, Q = 1u << (h->bR - h->bS);
-    const uint64_t ns = h->S.n;
-    const uint64_t waves = (uint64_t)device_count_cus() * 8;
-    uint64_t chunk = std::max<uint64_t>(ns / std::max<uint64_t>(waves, 1), 32768);
-    for (uint32_t ps = 0; ps < FS; ++ps) {
+    const uint32_t FR = 1u << h->bR, shift = h->bR - h->bS;
+    const uint64_t ns = h->S.start.back();
+    const uint64_t slices = (uint64_t)device_count_cus() * 16;
+    const uint64_t chunk = std::max<uint64_t>((ns + slices - 1) / slices, 32768);
+    for (uint32_t pr = 0; pr < FR; ++pr) {
+        const uint32_t ps = pr >> shift;
         const uint32_t s0 = h->S.start[ps], s1 = h->S.start[ps + 1];
-        if (s1 == s0)
+        const uint32_t r0 = h->R.start[pr], r1 = h->R.start[pr + 1];
+        if (s1 == s0 || r1 == r0)
             continue;
-        for (uint32_t q = 0; q < Q; ++q) {
-            const uint32_t pr = ps * Q + q;
-            const uint32_t r0 = h->R.start[pr], r1 = h->R.start[pr + 1];
-            if (r1 == r0)
-                continue;
-            const uint32_t nsplit = (r1 - r0 + kTableCap - 1) / kTableCap;
-            if (nsplit > 1)
-                *spilled += nsplit;
-            for (uint32_t rs = 0; rs < nsplit; ++rs) {
-                const uint32_t ro = r0 + rs * kTableCap;
-                const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
-                for (uint64_t so = s0; so < s1; so += chunk) {
-                    Item it;
-                    it.r_off = ro;
-                    it.r_cnt = rc;
-                    it.s_off = (uint32_t)so;
-                    it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
-                    it.sub_mask = Q > 1 ? Q - 1 : 0;
-                    it.sub_val = q;
-                    items.push_back(it);
-                }
+        const uint32_t nsplit = (r1 - r0 + kTableCap - 1) / kTableCap;
+        if (nsplit > 1)
+            *spilled += nsplit;
+        for (uint32_t rs = 0; rs < nsplit; ++rs) {
+            const uint32_t ro = r0 + rs * kTableCap;
+            const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
+            for (uint64_t so = s0; so < s1; so += chunk) {
+                Item it;
+                it.r_off = ro;
+                it.r_cnt = rc;
+                it.s_off = (uint32_t)so;
+                it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
+                it.part = pr;
+                it.unused = 0;
+                items.push_back(it);
             }
         }
     }
@@ -757,7 +975,7 @@ int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     h->bR = choose_build_bits(nb, h->desc.radix_bits);
     h->bS = std::min<uint32_t>(h->bR, 11);
     record(h, 0, st);
-    int rc = partition_side(h, d_build, nb, h->lay.build, h->bR, h->rpart, h->R, st);
+    int rc = partition_side(h, d_build, nb, h->lay.build, h->bR, false, h->rpart, h->R, st);
     record(h, 1, st);
     return rc;
 }
@@ -768,7 +986,7 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
     int rc;
     *out_n = 0;
     record(h, 2, st);
-    if ((rc = partition_side(h, d_probe, np, h->lay.probe, h->bS, h->spart, h->S, st)))
+    if ((rc = partition_side(h, d_probe, np, h->lay.probe, h->bS, true, h->spart, h->S, st)))
         return rc;
     record(h, 3, st);
     PJ_HIP(hipStreamSynchronize(st));  // partition starts are on the host now
@@ -787,16 +1005,15 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
     PJ_HIP(hipMemsetAsync(h->counters.p, 0, 64, st));
     uint32_t* item_ctr = (uint32_t*)h->counters.p;
     unsigned long long* cursor = (unsigned long long*)((char*)h->counters.p + 8);
-    const size_t lds = (size_t)kTableCap * 16 + (kTableBuckets + 1) * 4 + 17 * 4 + 4 * 4;
+    const size_t lds = (size_t)kTableCap * 16 + (kTableBuckets / 2 + 1) * 4 + 17 * 4 + 34 * 4;
     PJ_HIP(hipFuncSetAttribute((const void*)k_join, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const uint32_t sub_shift = 64 - h->bR;
     const uint32_t bkt_shift = 64 - h->bR - 12;
     record(h, 4, st);
     if (nitems) {
         const uint32_t grid = std::min<uint32_t>(nitems, (uint32_t)device_count_cus());
         hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
                            (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr, (uint32_t*)d_out,
-                           out_cap, cursor, h->lay.out, sub_shift, bkt_shift);
+                           out_cap, cursor, h->lay.out, h->bR, bkt_shift);
         PJ_HIP(hipGetLastError());
     }
     record(h, 5, st);

@@ -34,20 +34,19 @@ def cpu_baseline(seed, dom, nb, npr, nmiss):
     import numpy as np
     import oracle as O
 
-    sample = min(npr, 1 << 23)
+    sample = min(npr, 1 << 25)
     b, p = O.gen_pkfk(seed, dom, nb, npr, nmiss, probe_prefix=sample)
     what = "|R|=%d build + first %d of %d probe tuples" % (nb, sample, npr)
-    try:
-        import reference_run as R
-        ref = R if R.available() else None
-    except ImportError:
-        ref = None
+    import reference_run as R
+    ref = R if R.available() else None
     if ref is not None:
         threads = ref.default_threads()
         r = ref.run_pkfk(b, p, threads)
         return {"value": (nb + sample) / r["seconds"], "unit": "tuples/s", "cores": threads,
-                "kind": "reference", "sample": what + "; reference HashJoinOp under MergeOp, modulo hash, "
-                "2 tuples a bucket, %d threads" % threads, "out_tuples": r["out_tuples"]}
+                "kind": "reference", "sample": what + "; reference HashJoinOp built -O3 from its sources, "
+                "under MergeOp, modulo hash over next_pow2(|R|) buckets of 2 tuples, 1 MiB pages, %d threads"
+                % threads, "seconds": r["seconds"], "build_seconds": r["build_seconds"],
+                "out_tuples": r["out_tuples"]}
     d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
                     buckets=1 << max(1, (nb - 1).bit_length()), tuplesperbucket=2)
     t0 = time.perf_counter()

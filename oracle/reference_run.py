@@ -1,0 +1,122 @@
+"""Binding of oracle/_ref/libpythiaref.so: the reference HashJoinOp and
+TupleHasher compiled from the reference's own sources (oracle/ref/Makefile).
+
+Checker and CPU baseline only. The library is built where the reference tree
+is present and travels with the repository as a built file; when it is absent
+every caller falls back to the oracle restatement.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(HERE, "_ref", "libpythiaref.so")
+CT = {"int": 0, "long": 1, "dec": 2, "char": 3, "date": 4, "pointer": 5}
+WIDTH = {"int": 4, "long": 8, "dec": 8, "date": 8, "pointer": 8}
+
+_lib = None
+
+
+def available():
+    return os.path.exists(PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(PATH)
+        L.ref_hashjoin_run.restype = C.c_int
+        L.ref_hashjoin_run.argtypes = [
+            C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+            C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+            C.POINTER(C.c_char_p), C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+            C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+            C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ref_tuple_hash.restype = C.c_uint
+        L.ref_tuple_hash.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.c_int]
+        L.ref_tuple_hash_bytes.restype = C.c_uint
+        L.ref_tuple_hash_bytes.argtypes = [C.c_int, C.c_void_p, C.c_int]
+        _lib = L
+    return _lib
+
+
+def default_threads():
+    """One thread per core the process may use, at most 16 (a GPU box's
+    host share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _ints(v):
+    return (C.c_int * max(len(v), 1))(*v)
+
+
+def _schema(cols):
+    types = [CT[c[0]] if isinstance(c, tuple) else CT[c] for c in cols]
+    widths = [c[1] if isinstance(c, tuple) else WIDTH.get(c, 0) for c in cols]
+    return types, widths, sum(widths)
+
+
+def run_join(build_cols, probe_cols, buildjattr, probejattr, projection, build, probe, threads=4,
+             fn="modulo", buckets=1024, tuplesperbucket=2, buffsize=1 << 20, range_min=0, range_max=0,
+             out_cap=None):
+    """Runs the reference HashJoinOp under MergeOp(threads). build/probe are
+    packed uint8 arrays. Returns (rows uint8 [n, w], build seconds, total
+    seconds)."""
+    bt, bw, bsz = _schema(build_cols)
+    pt, pw, psz = _schema(probe_cols)
+    ow = 0
+    for p in projection:
+        side, a = p.split("$")
+        cols = build_cols if side[-1] == "B" else probe_cols
+        c = cols[int(a)]
+        ow += c[1] if isinstance(c, tuple) else WIDTH[c]
+    nb = build.size // bsz
+    npr = probe.size // psz
+    cap = out_cap if out_cap is not None else max(npr, 1)
+    while True:
+        out = np.zeros(cap * ow, np.uint8)
+        n = C.c_uint64()
+        tb = C.c_double()
+        tt = C.c_double()
+        proj = (C.c_char_p * len(projection))(*[p.encode() for p in projection])
+        rc = lib().ref_hashjoin_run(threads, len(bt), _ints(bt), _ints(bw), len(pt), _ints(pt), _ints(pw),
+                                    len(buildjattr), _ints(buildjattr), _ints(probejattr), len(projection), proj,
+                                    fn.encode(), buckets, tuplesperbucket, buffsize, range_min, range_max,
+                                    build.ctypes.data_as(C.c_void_p) if build.size else None, nb,
+                                    probe.ctypes.data_as(C.c_void_p) if probe.size else None, npr,
+                                    out.ctypes.data_as(C.c_void_p), cap, C.byref(n), C.byref(tb), C.byref(tt))
+        if rc == -2 and out_cap is None:
+            cap = n.value
+            continue
+        if rc != 0:
+            raise RuntimeError("reference HashJoinOp failed: %d" % rc)
+        return out[: n.value * ow].reshape(-1, ow), tb.value, tt.value
+
+
+def run_pkfk(build, probe, threads):
+    """The headline join (int64 key, int64 payload) x (int64 fk, int64
+    payload), projection [B$0, B$1, P$1], modulo hash over next_pow2(|R|)
+    buckets of 2 tuples. Returns seconds and the output count."""
+    nb = build.size // 2
+    buckets = 1 << max(1, (nb - 1).bit_length())
+    rows, tb, tt = run_join(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                            build.view(np.uint8), probe.view(np.uint8), threads=threads, fn="modulo",
+                            buckets=buckets, tuplesperbucket=2, buffsize=1 << 20)
+    return {"seconds": tt, "build_seconds": tb, "out_tuples": rows.shape[0], "rows": rows}
+
+
+def tuple_hash(fn, buckets, coltype, value, range_min=0, range_max=0, offset=0, skipbits=0):
+    a = np.array([value], dtype=np.int32 if coltype == "int" else np.int64)
+    return lib().ref_tuple_hash(fn.encode(), buckets, CT[coltype], a.ctypes.data_as(C.c_void_p), range_min,
+                                range_max, offset, skipbits)
+
+
+def bytes_hash(buckets, data):
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    return lib().ref_tuple_hash_bytes(buckets, data.ctypes.data_as(C.c_void_p), data.size)

@@ -156,6 +156,13 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
+    uint4 nxt[kWcItems];   // the next round's records, loaded one round ahead
+#pragma unroll
+    for (int k = 0; k < kWcItems; ++k) {
+        const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
+        if (i < end)
+            nxt[k] = load_rec<IDENT>(in, i, L);
+    }
     for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * kWcItems) {
         uint4 r[kWcItems];
         uint32_t p[kWcItems];
@@ -164,8 +171,10 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         for (int k = 0; k < kWcItems; ++k) {
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
             pend[k] = i < end;
-            if (pend[k])
-                r[k] = load_rec<IDENT>(in, i, L);
+            r[k] = nxt[k];
+            const uint64_t i2 = i + (uint64_t)kWcThreads * kWcItems;
+            if (i2 < end)
+                nxt[k] = load_rec<IDENT>(in, i2, L);
         }
 #pragma unroll
         for (int k = 0; k < kWcItems; ++k)
@@ -827,6 +836,15 @@ int launch_wc(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, u
     return PJ_OK;
 }
 
+// Blocks of the partition pass: about 512K tuples each, at least one per CU.
+uint32_t part_blocks(uint64_t n)
+{
+    uint64_t b = n >> 19;
+    b = std::max<uint64_t>(b, 256);
+    b = std::min<uint64_t>(b, 2048);
+    return (uint32_t)std::max<uint64_t>(std::min<uint64_t>(b, (n + 255) / 256), 1);
+}
+
 // Partitions raw tuples of one side into dst (canonical records, each
 // block's share of a partition padded to whole write-combining chunks).
 int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits,
@@ -841,11 +859,11 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
         return PJ_OK;
     if (n >= (1ull << 31))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
-    const uint32_t nblk = hist_blocks(n);
+    const uint32_t B = write_combine ? wc_chunk(bits) : 1;
+    const uint32_t nblk = B > 1 ? part_blocks(n) : hist_blocks(n);
     const uint64_t chunk = (n + nblk - 1) / nblk;
     const uint64_t m = (uint64_t)F * nblk;
     const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
-    const uint32_t B = write_combine ? wc_chunk(bits) : 1;
     const uint64_t cap = n + (uint64_t)F * nblk * (B - 1);
     if (cap >= (1ull << 32))
         return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");

@@ -280,3 +280,17 @@ def test_full_16m_x_256m_uniform():
     es, ec = O.expected_pkfk(202, 1 << 40, nb, npr, nmiss, threads=16)
     assert ec == npr - nmiss
     assert (n, s) == (ec, es)
+
+
+def test_gpu_equals_reference_hashjoin():
+    # the device join against the reference HashJoinOp itself (oracle/_ref),
+    # where that build travelled with the tree
+    import reference_run as RR
+    if not RR.available():
+        pytest.skip("oracle/_ref not built")
+    _say("case: GPU vs reference HashJoinOp")
+    b, p = O.gen_pkfk(55, 1 << 40, 200000, 1000000, 100000)
+    ref = RR.run_pkfk(b, p, threads=8)
+    got, _ = run_join(P.make_desc(**HEADLINE), b.view(np.uint8), p.view(np.uint8), 200000, 1000000)
+    assert got.shape == ref["rows"].shape
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(ref["rows"]))

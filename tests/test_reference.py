@@ -1,0 +1,98 @@
+"""Pins the oracle against the reference itself: HashJoinOp and TupleHasher
+compiled from the reference's sources into oracle/_ref (oracle/ref/Makefile).
+Skipped where that build is absent."""
+import numpy as np
+import pytest
+
+import oracle as O
+import reference_run as RR
+from _util import pack, unpack
+
+pytestmark = pytest.mark.skipif(not RR.available(), reason="oracle/_ref not built (reference tree absent)")
+
+
+def test_reference_queryhashjoin_each_key_once():
+    # unit_tests/queryhashjoin.cpp run through the real operator: 4 threads,
+    # 16-byte output pages
+    T = 20
+    build = pack(["long", "long"], [(k, k) for k in range(1, T + 1)])
+    probe = pack(["long", "dec"], [(k, k + 0.1) for k in range(1, T + 1)])
+    rows, _, _ = RR.run_join(["long", "long"], ["long", "dec"], [0], [0], ["B$1", "P$1"], build, probe,
+                             threads=4, fn="modulo", buckets=16, tuplesperbucket=2, buffsize=16)
+    got = sorted((int(v), float(d)) for v, d in unpack(["long", "dec"], rows))
+    assert got == [(k, k + 0.1) for k in range(1, T + 1)]
+
+
+@pytest.mark.parametrize("fn,threads", [("modulo", 1), ("modulo", 4), ("knuth", 3), ("bytes", 2), ("willis", 4)])
+def test_oracle_hashjoin_equals_reference(fn, threads):
+    rng = np.random.default_rng(41)
+    cols = ["long", "long", "int"]
+    nb, npr = 3000, 9000
+    build = pack(cols, [(int(k), int(v), int(w)) for k, v, w in zip(
+        rng.integers(-500, 500, nb), rng.integers(0, 1 << 62, nb), rng.integers(-(1 << 31), 1 << 31, nb))])
+    probe = pack(cols, [(int(k), int(v), int(w)) for k, v, w in zip(
+        rng.integers(-500, 500, npr), rng.integers(0, 1 << 62, npr), rng.integers(-(1 << 31), 1 << 31, npr))])
+    proj = ["P$2", "B$1", "B$0", "P$1", "B$2"]
+    ref, _, _ = RR.run_join(cols, cols, [0], [0], proj, build, probe, threads=threads, fn=fn, buckets=256,
+                            tuplesperbucket=2, buffsize=4096)
+    d = O.join_desc(cols, cols, [0], [0], proj, fn=fn, buckets=256, tuplesperbucket=2, buffsize=4096)
+    ora, _ = O.hashjoin(d, build, probe)
+    assert ref.shape[0] > 10000
+    assert np.array_equal(O.sorted_rows(ref), O.sorted_rows(ora))
+
+
+def test_oracle_composite_equals_reference():
+    # queryhashjoincomposite.cpp shape with duplicates on both sides
+    rng = np.random.default_rng(42)
+    cols = ["int", "int", "long"]
+    build = pack(cols, [(int(a), int(b), int(c)) for a, b, c in zip(
+        rng.integers(0, 30, 2000), rng.integers(0, 30, 2000), rng.integers(0, 1 << 62, 2000))])
+    probe = pack(cols, [(int(a), int(b), int(c)) for a, b, c in zip(
+        rng.integers(0, 35, 2000), rng.integers(0, 35, 2000), rng.integers(0, 1 << 62, 2000))])
+    for fn in ("modulo", "bytes"):
+        ref, _, _ = RR.run_join(cols, cols, [0, 1], [0, 1], ["B$0", "B$1", "B$2", "P$2"], build, probe,
+                                threads=3, fn=fn, buckets=64)
+        d = O.join_desc(cols, cols, [0, 1], [0, 1], ["B$0", "B$1", "B$2", "P$2"], fn=fn, buckets=64)
+        ora, _ = O.hashjoin(d, build, probe)
+        assert ref.shape[0] > 1000
+        assert np.array_equal(O.sorted_rows(ref), O.sorted_rows(ora))
+
+
+@pytest.mark.parametrize("fn", ["modulo", "knuth", "parammodulo", "exactrange", "range", "tpchorderkey", "willis"])
+def test_value_hashers_equal_reference(fn):
+    rng = np.random.default_rng(43)
+    kw = dict(range_min=-1000, range_max=(1 << 30)) if fn in ("exactrange", "range") else {}
+    if fn in ("knuth", "parammodulo"):
+        kw = dict(offset=77, skipbits=3)
+    h = O.hasher(fn, 1000, **kw)
+    for ct in ("long", "int"):
+        if fn in ("exactrange", "range"):
+            vals = rng.integers(-1000, 1 << 30, 500)
+        elif ct == "int":
+            vals = rng.integers(-(1 << 31), 1 << 31, 500)
+        else:
+            vals = rng.integers(-(1 << 62), 1 << 62, 500)
+        for v in vals:
+            a = np.array([v], dtype=np.int32 if ct == "int" else np.int64)
+            mine = O.lib().or_hasher_hash(h, a.ctypes.data, a.itemsize)
+            assert mine == RR.tuple_hash(fn, 1000, ct, int(v), **kw), (fn, ct, int(v))
+
+
+def test_byte_hasher_equals_reference():
+    # ByteHasher's sign-extended bytes and xor-fold: random keys, bytes >= 0x80
+    # included, 8 and 16 byte keys, several bucket counts
+    rng = np.random.default_rng(44)
+    for buckets in (16, 1000, 1 << 20):
+        h = O.hasher("bytes", buckets)
+        for size in (8, 16):
+            for _ in range(300):
+                data = rng.integers(0, 256, size, dtype=np.uint8)
+                assert O.lib().or_hasher_hash(h, data.ctypes.data, size) == RR.bytes_hash(buckets, data)
+
+
+def test_reference_pkfk_matches_expected_checksum():
+    b, p = O.gen_pkfk(77, 1 << 40, 50000, 200000, 20000)
+    r = RR.run_pkfk(b, p, threads=4)
+    s, c = O.expected_pkfk(77, 1 << 40, 50000, 200000, 20000)
+    assert r["out_tuples"] == c
+    assert O.checksum(r["rows"], 24)[0] == s

@@ -19,6 +19,11 @@ sys.path.insert(0, os.path.join(ROOT, "pythia-core_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes one k_join launch moves on each workload, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy
+# (profiles/k_join_uniform_16Mx256M.md).
+JOIN_TRAFFIC_BYTES = {"uniform_16Mx256M": 11.645e9}
+
 WORKLOADS = {
     # name: (seed, key domain, |R|, |S|, misses)
     "uniform_16Mx256M": (202, 1 << 40, 1 << 24, 1 << 28, round(0.1 * (1 << 28))),
@@ -64,6 +69,8 @@ def main():
     ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS))
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB with torch, a kernel of known traffic for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
@@ -75,6 +82,13 @@ def main():
 
     seed, dom, nb, npr, nmiss = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:
+        x = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     stream = torch.cuda.current_stream().cuda_stream
     db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
     dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
@@ -140,7 +154,8 @@ def main():
         "phases_ms": phases,
         "parity": {"count": n, "expected_count": ec, "checksum_ok": parity},
         "roofline": {"bound": "hbm", "kernel": "k_join", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": JOIN_TRAFFIC_BYTES.get(args.workload)},
     }
     j.close()
     del db, dp, out

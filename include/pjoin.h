@@ -149,6 +149,26 @@ int pj_partition_exactrange_device(const pj_schema* schema, uint32_t attr,
                                    uint32_t buckets, const void* d_in, uint64_t n,
                                    void* d_out, uint64_t* part_sizes, void* stream);
 
+/* PartitionOp on host buffers: uploads, partitions on the device as
+ * pj_partition_exactrange_device does, and copies the partitions back. */
+int pj_partition_exactrange(const pj_schema* schema, uint32_t attr, int64_t range_min,
+                            int64_t range_max, uint32_t buckets, const void* in, uint64_t n,
+                            void* out, uint64_t* part_sizes);
+
+/* Runs a query plan (libconfig text, as the reference's executequery reads,
+ * drivers/executequery.cpp:105-154) through the C++ operators of this
+ * library: node types merge, hashjoin, partition, threadidprepend, scan /
+ * parallelscan (raw binary tables) and memscan (the buffers passed here,
+ * by node name). The tree root must be a merge node. Output tuples of every
+ * thread are concatenated into out (in any order); *out_bytes gets the bytes
+ * produced, PJ_ERR_CAPACITY if they exceed out_cap_bytes. */
+int pj_query_run(const char* plan_text, uint32_t nsources, const char* const* names,
+                 const void* const* data, const uint64_t* tuples, void* out,
+                 uint64_t out_cap_bytes, uint64_t* out_bytes, uint32_t* out_tuple_width);
+
+/* Sets the calling thread's pj_last_error() text (used by the C++ layer). */
+void pj_set_error(const char* msg);
+
 /* Order-independent 64-bit checksum of device-resident tuples: the sum mod
  * 2^64 of a splitmix64-folded mix of each tuple's bytes, plus the XOR of all
  * 32-bit words that ConsumeOp computes (operators/consume.cpp:68-100). */

@@ -22,7 +22,7 @@ HIP_FLAGS = [
 ]
 
 KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip"]
-HOST_SOURCES = ["schema.cpp", "plan.cpp", "operators.cpp"]
+HOST_SOURCES = ["schema.cpp", "operators.cpp"]
 
 
 def _run(cmd):

@@ -353,4 +353,32 @@ int pj_partition_exactrange_device(const pj_schema* schema, uint32_t attr, int64
     return PJ_OK;
 }
 
+int pj_partition_exactrange(const pj_schema* schema, uint32_t attr, int64_t range_min, int64_t range_max,
+                            uint32_t buckets, const void* in, uint64_t n, void* out, uint64_t* part_sizes)
+{
+    if (!schema || !part_sizes || (n && (!in || !out)))
+        return fail(PJ_ERR_INVALID, "null argument");
+    uint32_t width = 0;
+    for (uint32_t c = 0; c < schema->ncols && c < PJ_MAX_COLS; ++c)
+        width += schema->type[c] == PJ_CT_INTEGER ? 4 : schema->type[c] == PJ_CT_CHAR ? schema->width[c] : 8;
+    void *din = nullptr, *dout = nullptr;
+    const size_t bytes = std::max<size_t>(n * width, 16);
+    if (hipMalloc(&din, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess) {
+        if (din)
+            (void)hipFree(din);
+        return fail(PJ_ERR_DEVICE, "hipMalloc of %zu bytes failed", bytes);
+    }
+    int rc = PJ_OK;
+    if (n && hipMemcpy(din, in, n * width, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(PJ_ERR_DEVICE, "upload failed");
+    if (rc == PJ_OK)
+        rc = pj_partition_exactrange_device(schema, attr, range_min, range_max, buckets, din, n, dout, part_sizes,
+                                            nullptr);
+    if (rc == PJ_OK && n && hipMemcpy(out, dout, n * width, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(PJ_ERR_DEVICE, "download failed");
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
+}
+
 }  // extern "C"

@@ -1065,6 +1065,7 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
 extern "C" {
 
 const char* pj_last_error(void) { return pj::last_error(); }
+void pj_set_error(const char* msg) { pj::set_error("%s", msg ? msg : ""); }
 const char* pj_version(void) { return "pjoin 0.1 gfx950"; }
 
 int pj_join_create(const pj_join_desc* desc, pj_join_t* out)

@@ -80,6 +80,12 @@ _SIGS = {
                                      C.POINTER(C.c_uint32), C.c_void_p]),
     "pj_gen_pkfk": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                               C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "pj_partition_exactrange": (C.c_int, [C.POINTER(Schema), C.c_uint32, C.c_int64, C.c_int64, C.c_uint32,
+                                          C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "pj_query_run": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p),
+                               C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_uint32)]),
+    "pj_set_error": (None, [C.c_char_p]),
 }
 
 # Entry points added by later components; bound when the library has them.
@@ -180,6 +186,28 @@ class Join:
             self.close()
         except Exception:
             pass
+
+
+def run_plan(plan, sources, out_cap_bytes=1 << 20):
+    """Runs a plan through the C++ operators. sources: {node name: (uint8
+    array of packed tuples, tuple count)}. Returns (uint8 rows [n, w])."""
+    import numpy as np
+    names = list(sources)
+    arrs = [np.ascontiguousarray(sources[n][0]) for n in names]
+    cn = (C.c_char_p * max(len(names), 1))(*[n.encode() for n in names])
+    cd = (C.c_void_p * max(len(names), 1))(*[a.ctypes.data for a in arrs])
+    ct = (C.c_uint64 * max(len(names), 1))(*[sources[n][1] for n in names])
+    while True:
+        out = np.zeros(max(out_cap_bytes, 1), np.uint8)
+        nb = C.c_uint64()
+        w = C.c_uint32()
+        rc = lib().pj_query_run(plan.encode(), len(names), cn, cd, ct, out.ctypes.data_as(C.c_void_p),
+                                out_cap_bytes, C.byref(nb), C.byref(w))
+        if rc == PJ_ERR_CAPACITY:
+            out_cap_bytes = nb.value
+            continue
+        check(rc)
+        return out[: nb.value].reshape(-1, w.value) if w.value else out[:0]
 
 
 def checksum_device(d_ptr, n, width, stream=None):

@@ -29,6 +29,11 @@ WORKLOADS = {
     "uniform_16Mx256M": (202, 1 << 40, 1 << 24, 1 << 28, round(0.1 * (1 << 28))),
     "dense_1Mx16M": (101, 1 << 20, 1 << 20, 1 << 24, 0),
 }
+# Composite (int32 a, int32 b) keys, probe side Zipf(z) over build ranks.
+ZIPF_WORKLOADS = {
+    # name: (seed, |R|, |S|)
+    "zipf_composite_8Mx128M": (303, 1 << 23, 1 << 27),
+}
 
 
 def cpu_baseline(seed, dom, nb, npr, nmiss):
@@ -61,12 +66,42 @@ def cpu_baseline(seed, dom, nb, npr, nmiss):
             "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
 
 
+def cpu_baseline_zipf(seed, nb, npr, cdf):
+    """The reference HashJoinOp (bytes hash over the two key columns, as
+    queryhashjoincomposite.cpp runs it) on the whole build side and the first
+    probe tuples; the oracle restatement when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import reference_run as R
+    sample = min(npr, 1 << 24)
+    b, p = O.gen_pairs_zipf(seed, nb, sample, cdf)
+    cols = ["int", "int", "long"]
+    proj = ["B$0", "B$1", "B$2", "P$2"]
+    buckets = 1 << max(1, (nb - 1).bit_length())
+    what = "|R|=%d build + %d Zipf probe tuples" % (nb, sample)
+    if R.available():
+        threads = R.default_threads()
+        rows, tb, tt = R.run_join(cols, cols, [0, 1], [0, 1], proj, b, p, threads=threads, fn="bytes",
+                                  buckets=buckets, tuplesperbucket=2)
+        return {"value": (nb + sample) / tt, "unit": "tuples/s", "cores": threads, "kind": "reference",
+                "sample": what + "; reference HashJoinOp, bytes hash, %d threads" % threads, "seconds": tt,
+                "out_tuples": int(rows.shape[0])}
+    d = O.join_desc(cols, cols, [0, 1], [0, 1], proj, fn="bytes", buckets=buckets)
+    t0 = time.perf_counter()
+    out, st = O.hashjoin(d, b, p)
+    dt = time.perf_counter() - t0
+    return {"value": (nb + sample) / dt, "unit": "tuples/s", "cores": 1, "kind": "port",
+            "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS) + sorted(ZIPF_WORKLOADS))
+    ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -80,8 +115,8 @@ def main():
     import torch
     import pjoin as P
 
-    seed, dom, nb, npr, nmiss = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
     if args.calibrate_copy:
         x = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
         y = torch.empty_like(x)
@@ -89,13 +124,25 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         del x, y
-    stream = torch.cuda.current_stream().cuda_stream
-    db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
-    dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
-    P.check(P.lib().pj_gen_pkfk(seed, dom, nb, npr, nmiss, C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()),
-                                1, C.c_void_p(stream)))
-    desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                       partitioner=args.partitioner, timing=True)
+    zipf = args.workload in ZIPF_WORKLOADS
+    if zipf:
+        seed, nb, npr = ZIPF_WORKLOADS[args.workload]
+        nmiss, dom = 0, None
+        cdf = P.zipf_cdf(nb, args.zipf)
+        db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
+        dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
+        P.check(P.lib().pj_gen_pairs_zipf(seed, nb, npr, cdf.ctypes.data_as(C.c_void_p), C.c_void_p(db.data_ptr()),
+                                          C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
+        desc = P.make_desc(["int", "int", "long"], ["int", "int", "long"], [0, 1], [0, 1],
+                           ["B$0", "B$1", "B$2", "P$2"], partitioner=args.partitioner, timing=True)
+    else:
+        seed, dom, nb, npr, nmiss = WORKLOADS[args.workload]
+        db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
+        dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
+        P.check(P.lib().pj_gen_pkfk(seed, dom, nb, npr, nmiss, C.c_void_p(db.data_ptr()),
+                                    C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
+        desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                           partitioner=args.partitioner, timing=True)
     j = P.Join(desc)
     cap = npr
     out = torch.empty(cap * j.out_width, dtype=torch.uint8, device=dev)
@@ -123,7 +170,10 @@ def main():
     s, _ = P.checksum_device(out.data_ptr(), n, j.out_width, stream)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
+    if zipf:
+        es, ec = O.expected_pairs_zipf(seed, nb, npr, cdf, threads=16)
+    else:
+        es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
     parity = (n == ec and s == es)
 
     w_in, w_out = 16, j.out_width
@@ -133,7 +183,9 @@ def main():
     job_gbs = b_alg / (ms * 1e-3) / 1e9
     t = j.timings()
     res = {
-        "metric": "join tuples/s (build+probe) + HBM GB/s, 16M x 256M int64; radix",
+        "metric": ("join tuples/s (build+probe) + HBM GB/s, 16M x 256M int64; " + args.partitioner) if not zipf
+        else "join tuples/s (build+probe), 8M x 128M (int32, int32) keys, Zipf z=%g; %s" % (args.zipf,
+                                                                                             args.partitioner),
         "value": (nb + npr) / (ms * 1e-3),
         "unit": "tuples/s",
         "n_gpus": 1,
@@ -145,7 +197,8 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": args.workload, "build_tuples": nb, "probe_tuples": npr, "probe_misses": nmiss,
+        "config": {"workload": args.workload + ("_z%g" % args.zipf if zipf else ""), "build_tuples": nb,
+                   "probe_tuples": npr, "probe_misses": nmiss,
                    "tuple_bytes": 16, "out_tuple_bytes": w_out, "out_tuples": n,
                    "partitioner": args.partitioner, "partitions": t.partitions,
                    "subpartitions": t.subpartitions, "work_items": t.work_items},
@@ -161,7 +214,8 @@ def main():
     del db, dp, out
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(seed, dom, nb, npr, nmiss)
+        res["cpu_baseline"] = cpu_baseline_zipf(seed, nb, npr, cdf) if zipf else cpu_baseline(seed, dom, nb, npr,
+                                                                                                  nmiss)
     print(json.dumps(res))
     if not parity:
         sys.exit(1)

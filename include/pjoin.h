@@ -181,6 +181,17 @@ int pj_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_pro
                 uint64_t n_miss, void* build_dst, void* probe_dst, int on_device,
                 void* stream);
 
+/* Zipf CDF over ranks 1..n with exponent z, normalised, in double precision
+ * (operators/generator_zipf.cpp:91-113). */
+int pj_zipf_cdf(uint64_t n, double z, double* cdf);
+/* Composite-key workload: build tuples (int32 a, int32 b, int64 payload) with
+ * unique (a, b) in [0, 2^31)^2; probe tuples of the same shape whose key is
+ * the key of a build tuple drawn Zipf(z) over ranks through cdf (a host
+ * array of n_build doubles from pj_zipf_cdf), rank -> build tuple by a seeded
+ * permutation (generator_zipf.cpp:85-89). */
+int pj_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,
+                      void* build_dst, void* probe_dst, int on_device, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

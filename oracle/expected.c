@@ -94,3 +94,85 @@ int or_gen_pkfk_probe_range(uint64_t seed, uint64_t domain, uint64_t n_build, ui
         pj_pkfk_probe(&p, j, &q[2 * (j - j0)], &q[2 * (j - j0) + 1]);
     return 0;
 }
+
+typedef struct {
+    const double* cdf;
+    uint64_t seed, n_build, b, e, sum, cnt;
+} pz_job;
+
+static void* pz_worker(void* arg)
+{
+    pz_job* j = (pz_job*)arg;
+    uint64_t s = 0;
+    for (uint64_t i = j->b; i < j->e; ++i) {
+        uint32_t a, b, ba, bb;
+        uint64_t pp, bp;
+        uint64_t idx = pj_pair_zipf_probe(j->cdf, j->n_build, j->seed, i, &a, &b, &pp);
+        pj_pair_build(j->seed, idx, &ba, &bb, &bp);
+        unsigned char t[24];
+        memcpy(t, &a, 4);
+        memcpy(t + 4, &b, 4);
+        memcpy(t + 8, &bp, 8);
+        memcpy(t + 16, &pp, 8);
+        s += or_tuple_mix(t, 24);
+    }
+    j->sum = s;
+    j->cnt = j->e - j->b;
+    return NULL;
+}
+
+/* Every Zipf probe matches exactly one build tuple (unique composite keys),
+ * so the output of projection [B$0, B$1, B$2, P$2] is one (a, b, bp, pp) per
+ * probe tuple. */
+int or_expected_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf, int threads,
+                           uint64_t* sum, uint64_t* count)
+{
+    if (threads < 1)
+        threads = 1;
+    if (threads > 64)
+        threads = 64;
+    pthread_t th[64];
+    pz_job jobs[64];
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].cdf = cdf;
+        jobs[t].seed = seed;
+        jobs[t].n_build = n_build;
+        jobs[t].b = n_probe * (uint64_t)t / (uint64_t)threads;
+        jobs[t].e = n_probe * (uint64_t)(t + 1) / (uint64_t)threads;
+        pthread_create(&th[t], NULL, pz_worker, &jobs[t]);
+    }
+    uint64_t s = 0, c = 0;
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(th[t], NULL);
+        s += jobs[t].sum;
+        c += jobs[t].cnt;
+    }
+    *sum = s;
+    *count = c;
+    return 0;
+}
+
+/* Host generation of the same relations, (a, b, payload) packed in 16 B. */
+int or_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf, void* build,
+                      void* probe)
+{
+    unsigned char* B = (unsigned char*)build;
+    unsigned char* P = (unsigned char*)probe;
+    for (uint64_t i = 0; i < n_build; ++i) {
+        uint32_t a, b;
+        uint64_t pay;
+        pj_pair_build(seed, i, &a, &b, &pay);
+        memcpy(B + 16 * i, &a, 4);
+        memcpy(B + 16 * i + 4, &b, 4);
+        memcpy(B + 16 * i + 8, &pay, 8);
+    }
+    for (uint64_t j = 0; j < n_probe; ++j) {
+        uint32_t a, b;
+        uint64_t pay;
+        pj_pair_zipf_probe(cdf, n_build, seed, j, &a, &b, &pay);
+        memcpy(P + 16 * j, &a, 4);
+        memcpy(P + 16 * j + 4, &b, 4);
+        memcpy(P + 16 * j + 8, &pay, 8);
+    }
+    return 0;
+}

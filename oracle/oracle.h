@@ -157,6 +157,12 @@ int or_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_pro
 /* Probe tuples [j0, j1) of the full probe relation of n_probe tuples. */
 int or_gen_pkfk_probe_range(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_probe,
                             uint64_t n_miss, uint64_t j0, uint64_t j1, void* probe);
+/* The composite-key Zipf workload: expected output count and checksum of
+ * projection [B$0, B$1, B$2, P$2], and host generation of the relations. */
+int or_expected_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,
+                           int threads, uint64_t* sum, uint64_t* count);
+int or_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,
+                      void* build, void* probe);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,10 @@ def lib():
                                            C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "or_gen_pkfk": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                       C.c_void_p, C.c_void_p]),
+            "or_expected_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_int,
+                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+            "or_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+                                            C.c_void_p]),
             "or_gen_pkfk_probe_range": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                                   C.c_uint64, C.c_uint64, C.c_void_p]),
         }
@@ -223,3 +227,26 @@ def sorted_rows(rows):
         return rows
     v = rows.view(np.dtype((np.void, rows.shape[1]))).ravel()
     return rows[np.argsort(v, kind="stable")]
+
+
+def zipf_cdf(n, z):
+    """The Zipf CDF as the reference's generator builds it
+    (operators/generator_zipf.cpp:91-113): running sum of (i+1)^-z in double,
+    then divided by the total. Computed here independently of the product."""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    cdf = np.cumsum(i ** (-z))
+    return cdf / cdf[-1]
+
+
+def gen_pairs_zipf(seed, n_build, n_probe, cdf):
+    b = np.zeros(n_build * 16, np.uint8)
+    p = np.zeros(n_probe * 16, np.uint8)
+    lib().or_gen_pairs_zipf(seed, n_build, n_probe, cdf.ctypes.data_as(C.c_void_p), _ptr(b), _ptr(p))
+    return b, p
+
+
+def expected_pairs_zipf(seed, n_build, n_probe, cdf, threads=8):
+    s, c = C.c_uint64(), C.c_uint64()
+    lib().or_expected_pairs_zipf(seed, n_build, n_probe, cdf.ctypes.data_as(C.c_void_p), threads, C.byref(s),
+                                 C.byref(c))
+    return s.value, c.value

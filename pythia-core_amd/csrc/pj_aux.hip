@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -82,6 +83,27 @@ __global__ void k_gen_pkfk(pj_pkfk_params p, uint64_t* __restrict__ build, uint6
             pj_pkfk_probe(&p, j, &k, &v);
             probe[2 * j] = k;
             probe[2 * j + 1] = v;
+        }
+    }
+}
+
+struct PairTuple {   // (int32 a, int32 b, int64 payload), 16 bytes packed
+    uint32_t a, b;
+    uint64_t pay;
+};
+
+__global__ void k_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* __restrict__ cdf,
+                                 PairTuple* __restrict__ build, PairTuple* __restrict__ probe)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_build + n_probe; i += stride) {
+        PairTuple t;
+        if (i < n_build) {
+            pj_pair_build(seed, i, &t.a, &t.b, &t.pay);
+            build[i] = t;
+        } else {
+            pj_pair_zipf_probe(cdf, n_build, seed, i - n_build, &t.a, &t.b, &t.pay);
+            probe[i - n_build] = t;
         }
     }
 }
@@ -242,6 +264,65 @@ int pj_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_pro
             pj_pkfk_probe(&p, i - n_build, &k, &v);
             q[2 * (i - n_build)] = k;
             q[2 * (i - n_build) + 1] = v;
+        }
+    });
+    return PJ_OK;
+}
+
+int pj_zipf_cdf(uint64_t n, double z, double* cdf)
+{
+    // generator_zipf.cpp:91-113: cdf[r] = sum_{i<=r} (i+1)^-z / H, summed
+    // in order in double precision
+    if (!cdf || n == 0)
+        return fail(PJ_ERR_INVALID, "null argument");
+    double h = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        h += std::pow((double)(i + 1), -z);
+        cdf[i] = h;
+    }
+    for (uint64_t i = 0; i < n; ++i)
+        cdf[i] /= h;
+    return PJ_OK;
+}
+
+int pj_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf, void* build_dst,
+                      void* probe_dst, int on_device, void* stream)
+{
+    if ((n_build && !build_dst) || (n_probe && (!probe_dst || !cdf || n_build == 0)))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (n_build >= (1ull << 62))
+        return fail(PJ_ERR_INVALID, "too many build tuples");
+    if (on_device) {
+        hipStream_t st = (hipStream_t)stream;
+        double* dcdf = nullptr;
+        if (n_probe) {
+            PJ_HIP(hipMalloc(&dcdf, n_build * sizeof(double)));
+            if (hipMemcpy(dcdf, cdf, n_build * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipFree(dcdf);
+                return fail(PJ_ERR_DEVICE, "cdf upload failed");
+            }
+        }
+        hipLaunchKernelGGL(k_gen_pairs_zipf, dim3(4096), dim3(256), 0, st, seed, n_build, n_probe, dcdf,
+                           (PairTuple*)build_dst, (PairTuple*)probe_dst);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(st);
+        if (dcdf)
+            (void)hipFree(dcdf);
+        if (e != hipSuccess)
+            return fail(PJ_ERR_DEVICE, "pair generator failed: %s", hipGetErrorString(e));
+        return PJ_OK;
+    }
+    PairTuple* b = (PairTuple*)build_dst;
+    PairTuple* q = (PairTuple*)probe_dst;
+    host_parallel(n_build + n_probe, [&](uint64_t i) {
+        PairTuple t;
+        if (i < n_build) {
+            pj_pair_build(seed, i, &t.a, &t.b, &t.pay);
+            b[i] = t;
+        } else {
+            pj_pair_zipf_probe(cdf, n_build, seed, i - n_build, &t.a, &t.b, &t.pay);
+            q[i - n_build] = t;
         }
     });
     return PJ_OK;

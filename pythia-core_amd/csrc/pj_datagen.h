@@ -152,3 +152,17 @@ PJ_HD void pj_pair_build(uint64_t seed, uint64_t i, uint32_t* a, uint32_t* b, ui
     *b = (uint32_t)(v >> 31);
     *pay = pj_rand64(seed, 1, i);
 }
+
+// Zipf probe relation over the composite-key build relation: probe tuple j
+// is (a, b, payload) of build tuple pj_zipf_build_index(rank) with rank drawn
+// from the CDF; its payload is rand(seed, 3, j). Returns the build index.
+PJ_HD uint64_t pj_pair_zipf_probe(const double* cdf, uint64_t n_build, uint64_t seed, uint64_t j, uint32_t* a,
+                                  uint32_t* b, uint64_t* pay)
+{
+    const uint64_t rank = pj_zipf_rank(cdf, n_build, seed, j);
+    const uint64_t idx = pj_zipf_build_index(rank, n_build, seed);
+    uint64_t bp;
+    pj_pair_build(seed, idx, a, b, &bp);
+    *pay = pj_rand64(seed, 3, j);
+    return idx;
+}

@@ -133,9 +133,12 @@ constexpr int kWcThreads = 1024;
 constexpr int kWcItems = 4;
 
 // Software write-combining scatter: every partition owns a B-record buffer
-// in LDS; a full buffer leaves as one aligned B * 16-byte chunk, the lanes of
-// a B-lane group writing one record each. Threads whose record finds its
-// buffer full retry after the flush.
+// in LDS and leaves HBM only in aligned chunks of B records. A round claims
+// buffer slots with one LDS atomic per record; a claim index s >= B ranks the
+// record among the partition's overflow. After the claims, a full buffer
+// leaves as one chunk (a B-lane group writes it), whole chunks of overflow
+// records are written straight after it, and the overflow remainder refills
+// the buffer. Three barriers a round, whatever the skew.
 template <bool IDENT, int B>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
                                                             SideLayout L, uint32_t bits, uint64_t chunk,
@@ -145,8 +148,8 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     extern __shared__ uint4 sm4[];
     const uint32_t F = 1u << bits;
     uint4* buf = sm4;                                           // F * B
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + F * B);  // F
-    uint32_t* goff = cnt + F;                                   // F
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + F * B);  // F: buffered records
+    uint32_t* goff = cnt + F;                                   // F: next chunk's record index
     for (uint32_t p = threadIdx.x; p < F; p += kWcThreads) {
         goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
         cnt[p] = 0;
@@ -165,48 +168,58 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     }
     for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * kWcItems) {
         uint4 r[kWcItems];
-        uint32_t p[kWcItems];
-        bool pend[kWcItems];
+        uint32_t p[kWcItems], s[kWcItems];
+        bool v[kWcItems];
 #pragma unroll
         for (int k = 0; k < kWcItems; ++k) {
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
-            pend[k] = i < end;
+            v[k] = i < end;
             r[k] = nxt[k];
             const uint64_t i2 = i + (uint64_t)kWcThreads * kWcItems;
             if (i2 < end)
                 nxt[k] = load_rec<IDENT>(in, i2, L);
         }
+        // claim
 #pragma unroll
         for (int k = 0; k < kWcItems; ++k)
-            if (pend[k])
+            if (v[k]) {
                 p[k] = radix_part(key_mix(rec_key(r[k])), bits);
-        for (;;) {
-            bool more = false;
+                s[k] = atomicAdd(&cnt[p[k]], 1u);
+                if (s[k] < (uint32_t)B)
+                    buf[p[k] * B + s[k]] = r[k];
+            }
+        __syncthreads();
+        // full buffers leave; whole chunks of overflow go straight out
+        for (uint32_t q = grp; q < F; q += ngrp)
+            if (cnt[q] >= (uint32_t)B)
+                out[goff[q] + gl] = buf[q * B + gl];
+        uint32_t keep[kWcItems];
 #pragma unroll
-            for (int k = 0; k < kWcItems; ++k)
-                if (pend[k]) {
-                    const uint32_t s = atomicAdd(&cnt[p[k]], 1u);
-                    if (s < (uint32_t)B) {
-                        buf[p[k] * B + s] = r[k];
-                        pend[k] = false;
-                    } else {
-                        more = true;
-                    }
-                }
-            const int again = __syncthreads_or(more);
-            for (uint32_t q = grp; q < F; q += ngrp)
-                if (cnt[q] >= (uint32_t)B)
-                    out[goff[q] + gl] = buf[q * B + gl];
-            __syncthreads();
-            for (uint32_t q = threadIdx.x; q < F; q += kWcThreads)
-                if (cnt[q] >= (uint32_t)B) {
-                    cnt[q] = 0;
-                    goff[q] += B;
-                }
-            __syncthreads();
-            if (!again)
-                break;
+        for (int k = 0; k < kWcItems; ++k) {
+            keep[k] = ~0u;
+            if (v[k] && s[k] >= (uint32_t)B) {
+                const uint32_t rank = s[k] - B;
+                const uint32_t direct = (cnt[p[k]] - B) / B * B;   // overflow records in whole chunks
+                if (rank < direct)
+                    out[goff[p[k]] + B + rank] = r[k];
+                else
+                    keep[k] = rank - direct;
+            }
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kWcItems; ++k)
+            if (keep[k] != ~0u)
+                buf[p[k] * B + keep[k]] = r[k];
+        for (uint32_t q = threadIdx.x; q < F; q += kWcThreads) {
+            const uint32_t c = cnt[q];
+            if (c >= (uint32_t)B) {
+                const uint32_t direct = (c - B) / B * B;
+                goff[q] += B + direct;
+                cnt[q] = c - B - direct;
+            }
+        }
+        __syncthreads();
     }
     // partial buffers leave padded to a whole chunk
     for (uint32_t q = grp; q < F; q += ngrp) {

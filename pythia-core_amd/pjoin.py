@@ -86,6 +86,9 @@ _SIGS = {
                                C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                C.POINTER(C.c_uint32)]),
     "pj_set_error": (None, [C.c_char_p]),
+    "pj_zipf_cdf": (C.c_int, [C.c_uint64, C.c_double, C.c_void_p]),
+    "pj_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int, C.c_void_p]),
 }
 
 # Entry points added by later components; bound when the library has them.
@@ -215,3 +218,10 @@ def checksum_device(d_ptr, n, width, stream=None):
     x = C.c_uint32()
     check(lib().pj_checksum_device(C.c_void_p(d_ptr), n, width, C.byref(s), C.byref(x), C.c_void_p(stream)))
     return s.value, x.value
+
+
+def zipf_cdf(n, z):
+    import numpy as np
+    cdf = np.zeros(n, np.float64)
+    check(lib().pj_zipf_cdf(n, z, cdf.ctypes.data_as(C.c_void_p)))
+    return cdf

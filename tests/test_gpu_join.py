@@ -294,3 +294,54 @@ def test_gpu_equals_reference_hashjoin():
     got, _ = run_join(P.make_desc(**HEADLINE), b.view(np.uint8), p.view(np.uint8), 200000, 1000000)
     assert got.shape == ref["rows"].shape
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(ref["rows"]))
+
+
+COMPOSITE = dict(build_cols=["int", "int", "long"], probe_cols=["int", "int", "long"], buildjattr=[0, 1],
+                 probejattr=[0, 1], projection=["B$0", "B$1", "B$2", "P$2"])
+
+
+def test_zipf_generator_device_equals_host():
+    _say("case: zipf pair generator device == host")
+    n, m = 5000, 60000
+    cdf = O.zipf_cdf(n, 1.5)
+    hb, hp = O.gen_pairs_zipf(11, n, m, cdf)
+    db = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    dp = torch.empty(m * 16, dtype=torch.uint8, device="cuda")
+    P.check(P.lib().pj_gen_pairs_zipf(11, n, m, cdf.ctypes.data_as(C.c_void_p), C.c_void_p(db.data_ptr()),
+                                      C.c_void_p(dp.data_ptr()), 1, None))
+    assert np.array_equal(db.cpu().numpy(), hb) and np.array_equal(dp.cpu().numpy(), hp)
+
+
+@pytest.mark.parametrize("z", [0.5, 1.0, 1.5])
+def test_zipf_composite_vs_oracle(z):
+    # heavy hitters: at z = 1.5 one key takes ~38% of the probe side
+    _say("case: zipf composite z=%.1f" % z)
+    n, m = 40000, 400000
+    cdf = O.zipf_cdf(n, z)
+    b, p = O.gen_pairs_zipf(12, n, m, cdf)
+    got, t = compare(COMPOSITE, b, p, n, m, timing=True)
+    assert got.shape[0] == m
+
+
+def _full_zipf(z, n, m, seed):
+    cdf = O.zipf_cdf(n, z)
+    db = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    dp = torch.empty(m * 16, dtype=torch.uint8, device="cuda")
+    P.check(P.lib().pj_gen_pairs_zipf(seed, n, m, cdf.ctypes.data_as(C.c_void_p), C.c_void_p(db.data_ptr()),
+                                      C.c_void_p(dp.data_ptr()), 1, None))
+    j = P.Join(P.make_desc(**COMPOSITE, timing=True))
+    out = torch.empty(m * 24, dtype=torch.uint8, device="cuda")
+    cnt = j.run_device(db.data_ptr(), n, dp.data_ptr(), m, out.data_ptr(), m)
+    s, _ = P.checksum_device(out.data_ptr(), cnt, 24)
+    j.close()
+    del db, dp, out
+    torch.cuda.empty_cache()
+    es, ec = O.expected_pairs_zipf(seed, n, m, cdf, threads=16)
+    return cnt, s, ec, es
+
+
+@pytest.mark.parametrize("z", [0.5, 1.0, 1.5])
+def test_full_8m_x_128m_zipf_composite(z):
+    _say("case: 8M x 128M zipf composite z=%.1f" % z)
+    cnt, s, ec, es = _full_zipf(z, 1 << 23, 1 << 27, 303)
+    assert (cnt, s) == (ec, es)

@@ -335,3 +335,39 @@ def test_expected_pkfk_matches_restated_hashjoin():
         exp_sum, exp_cnt = O.expected_pkfk(seed, dom, nb, npr, nmiss)
         assert out.shape[0] == exp_cnt == npr - nmiss
         assert O.checksum(out, 24)[0] == exp_sum
+
+
+# ------------------------------------------------ Zipf composite workload
+
+def test_zipf_cdf_product_equals_reference_formula():
+    # the product's CDF (pj_zipf_cdf) equals generator_zipf.cpp:91-113's
+    # running sum in double, computed independently here
+    import pjoin as P
+    for n, z in ((1000, 0.5), (100000, 1.0), (1 << 20, 1.5)):
+        assert np.array_equal(P.zipf_cdf(n, z), O.zipf_cdf(n, z))
+
+
+def test_zipf_sampling_follows_the_cdf():
+    # P(rank 0) = 1 / H_n(z): about 38% of probes hit the top key at z = 1.5
+    n = 1 << 16
+    cdf = O.zipf_cdf(n, 1.5)
+    b, p = O.gen_pairs_zipf(9, n, 200000, cdf)
+    bk = b.reshape(-1, 16)[:, :8].copy().view(np.uint64).ravel()
+    pk = p.reshape(-1, 16)[:, :8].copy().view(np.uint64).ravel()
+    assert len(np.unique(bk)) == n
+    vals, counts = np.unique(pk, return_counts=True)
+    top = counts.max() / pk.size
+    assert abs(top - cdf[0]) < 0.005, (top, cdf[0])
+    assert np.isin(pk, bk).all()
+
+
+def test_expected_pairs_zipf_matches_restated_hashjoin():
+    n, m = 20000, 100000
+    cdf = O.zipf_cdf(n, 1.0)
+    b, p = O.gen_pairs_zipf(5, n, m, cdf)
+    cols = ["int", "int", "long"]
+    d = O.join_desc(cols, cols, [0, 1], [0, 1], ["B$0", "B$1", "B$2", "P$2"], fn="bytes", buckets=1 << 14)
+    out, _ = O.hashjoin(d, b, p)
+    s, c = O.expected_pairs_zipf(5, n, m, cdf)
+    assert out.shape[0] == c == m
+    assert O.checksum(out, 24)[0] == s

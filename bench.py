@@ -27,12 +27,19 @@ JOIN_TRAFFIC_BYTES = {"uniform_16Mx256M": 11.645e9}
 WORKLOADS = {
     # name: (seed, key domain, |R|, |S|, misses)
     "uniform_16Mx256M": (202, 1 << 40, 1 << 24, 1 << 28, round(0.1 * (1 << 28))),
+    "uniform_1Mx16M": (101, 1 << 40, 1 << 20, 1 << 24, 0),
     "dense_1Mx16M": (101, 1 << 20, 1 << 20, 1 << 24, 0),
 }
 # Composite (int32 a, int32 b) keys, probe side Zipf(z) over build ranks.
 ZIPF_WORKLOADS = {
     # name: (seed, |R|, |S|)
     "zipf_composite_8Mx128M": (303, 1 << 23, 1 << 27),
+}
+# int64 build keys from 256 dense clusters (widths 2^10..2^24, weights
+# Zipf 1.2), probe keys Zipf(--zipf, default 1.0) over build ranks.
+CLUSTERED_WORKLOADS = {
+    # name: (seed, |R|, |S|)
+    "clustered_64Mx512M": (404, 1 << 26, 1 << 29),
 }
 
 
@@ -100,7 +107,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS) + sorted(ZIPF_WORKLOADS))
+    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS) + sorted(ZIPF_WORKLOADS) + sorted(CLUSTERED_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -125,7 +132,19 @@ def main():
         torch.cuda.synchronize()
         del x, y
     zipf = args.workload in ZIPF_WORKLOADS
-    if zipf:
+    clustered = args.workload in CLUSTERED_WORKLOADS
+    if clustered:
+        seed, nb, npr = CLUSTERED_WORKLOADS[args.workload]
+        nmiss, dom = 0, None
+        table = P.cluster_table(seed, nb)
+        cdf = P.zipf_cdf(nb, args.zipf)
+        db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
+        dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
+        P.check(P.lib().pj_gen_clustered(table.ctypes.data_as(C.c_void_p), npr, cdf.ctypes.data_as(C.c_void_p),
+                                         C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
+        desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                           partitioner=args.partitioner, timing=True)
+    elif zipf:
         seed, nb, npr = ZIPF_WORKLOADS[args.workload]
         nmiss, dom = 0, None
         cdf = P.zipf_cdf(nb, args.zipf)
@@ -170,7 +189,9 @@ def main():
     s, _ = P.checksum_device(out.data_ptr(), n, j.out_width, stream)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    if zipf:
+    if clustered:
+        es, ec = O.expected_clustered(table, npr, cdf, threads=16)
+    elif zipf:
         es, ec = O.expected_pairs_zipf(seed, nb, npr, cdf, threads=16)
     else:
         es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
@@ -183,7 +204,9 @@ def main():
     job_gbs = b_alg / (ms * 1e-3) / 1e9
     t = j.timings()
     res = {
-        "metric": ("join tuples/s (build+probe) + HBM GB/s, 16M x 256M int64; " + args.partitioner) if not zipf
+        "metric": ("join tuples/s (build+probe), 64M x 512M clustered int64 keys, Zipf z=%g fk; %s"
+                   % (args.zipf, args.partitioner)) if clustered else
+        ("join tuples/s (build+probe) + HBM GB/s, 16M x 256M int64; " + args.partitioner) if not zipf
         else "join tuples/s (build+probe), 8M x 128M (int32, int32) keys, Zipf z=%g; %s" % (args.zipf,
                                                                                              args.partitioner),
         "value": (nb + npr) / (ms * 1e-3),
@@ -197,7 +220,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
-        "config": {"workload": args.workload + ("_z%g" % args.zipf if zipf else ""), "build_tuples": nb,
+        "config": {"workload": args.workload + ("_z%g" % args.zipf if zipf or clustered else ""), "build_tuples": nb,
                    "probe_tuples": npr, "probe_misses": nmiss,
                    "tuple_bytes": 16, "out_tuple_bytes": w_out, "out_tuples": n,
                    "partitioner": args.partitioner, "partitions": t.partitions,
@@ -214,8 +237,11 @@ def main():
     del db, dp, out
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline_zipf(seed, nb, npr, cdf) if zipf else cpu_baseline(seed, dom, nb, npr,
-                                                                                                  nmiss)
+        if clustered:
+            res["cpu_baseline"] = None   # the reference table for 64M build tuples: see the uniform workload
+        else:
+            res["cpu_baseline"] = cpu_baseline_zipf(seed, nb, npr, cdf) if zipf else cpu_baseline(seed, dom, nb,
+                                                                                                  npr, nmiss)
     print(json.dumps(res))
     if not parity:
         sys.exit(1)

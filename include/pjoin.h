@@ -192,6 +192,18 @@ int pj_zipf_cdf(uint64_t n, double z, double* cdf);
 int pj_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,
                       void* build_dst, void* probe_dst, int on_device, void* stream);
 
+/* Clustered-key workload: nclusters dense clusters (centres uniform in
+ * [0, 2^62), widths 2^u for u uniform in [log2_wmin, log2_wmax], cluster
+ * weights Zipf(weight_z) capped by width) holding n_keys distinct int64
+ * keys; build tuples (key, payload) in a seeded order, probe tuples whose key
+ * is that of a build tuple drawn Zipf through cdf (n_keys doubles). */
+int pj_clusters_make(uint64_t seed, uint32_t nclusters, uint64_t n_keys, uint32_t log2_wmin,
+                     uint32_t log2_wmax, double weight_z, void* table_out);
+int pj_gen_clustered(const void* table, uint64_t n_probe, const double* cdf, void* build_dst,
+                     void* probe_dst, int on_device, void* stream);
+/* Bytes of the cluster table pj_clusters_make fills (at most 1024 clusters). */
+uint64_t pj_cluster_table_size(void);
+
 #ifdef __cplusplus
 }
 #endif

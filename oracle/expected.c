@@ -176,3 +176,119 @@ int or_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const d
     }
     return 0;
 }
+
+/* ---- clustered keys with Zipf foreign keys ------------------------------ */
+
+/* guide[g] = first rank r with cdf[r] > g / G, so the rank of u lies
+ * between guide[g - 1] and guide[g + 2] for g = floor(u * G). */
+static uint32_t* make_guide(const double* cdf, uint64_t n, uint64_t G)
+{
+    uint32_t* guide = (uint32_t*)malloc((G + 1) * sizeof(uint32_t));
+    uint64_t r = 0;
+    for (uint64_t g = 0; g <= G; ++g) {
+        const double t = (double)g / (double)G;
+        while (r < n - 1 && !(cdf[r] > t))
+            r++;
+        guide[g] = (uint32_t)r;
+    }
+    return guide;
+}
+
+/* The same rank as pj_zipf_rank (same uniforms, same redraws, first CDF
+ * entry > u), searched inside a guide-table bracket. */
+static uint64_t zipf_rank_guided(const double* cdf, uint64_t n, const uint32_t* guide, uint64_t G, uint64_t seed,
+                                 uint64_t j)
+{
+    const double last = cdf[n - 1];
+    for (uint64_t attempt = 0;; ++attempt) {
+        uint64_t r = pj_rand64(seed, 5 + attempt, j) >> 2;
+        double u = (double)r / 4611686018427387904.0;
+        if (u >= last)
+            continue;
+        uint64_t g = (uint64_t)(u * (double)G);
+        if (g > G)
+            g = G;
+        uint64_t lo = g >= 1 ? guide[g - 1] : 0, hi = g + 2 <= G ? guide[g + 2] : n - 1;
+        if (hi > n - 1)
+            hi = n - 1;
+        while (lo < hi) {
+            uint64_t mid = lo + (hi - lo) / 2;
+            if (cdf[mid] > u)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        return lo;
+    }
+}
+
+typedef struct {
+    const pj_cluster_table* t;
+    const double* cdf;
+    const uint32_t* guide;
+    uint64_t G, b, e, sum, cnt;
+} cl_job;
+
+static void* cl_worker(void* arg)
+{
+    cl_job* jb = (cl_job*)arg;
+    uint64_t s = 0;
+    for (uint64_t j = jb->b; j < jb->e; ++j) {
+        uint64_t key, pp, bk, bp;
+        uint64_t rank = zipf_rank_guided(jb->cdf, jb->t->n_keys, jb->guide, jb->G, jb->t->seed, j);
+        uint64_t idx = pj_clustered_probe_with_rank(jb->t, rank, j, &key, &pp);
+        pj_clustered_build(jb->t, idx, &bk, &bp);
+        uint64_t tup[3] = {key, bp, pp};
+        s += or_tuple_mix(tup, 24);
+    }
+    jb->sum = s;
+    jb->cnt = jb->e - jb->b;
+    return NULL;
+}
+
+/* Expected output of projection [B$0, B$1, P$1] over the clustered workload:
+ * one (key, bp, pp) per probe tuple (every probe key is a build key). */
+int or_expected_clustered(const void* table, uint64_t n_probe, const double* cdf, int threads, uint64_t* sum,
+                          uint64_t* count)
+{
+    const pj_cluster_table* t = (const pj_cluster_table*)table;
+    const uint64_t G = 1ull << 24;
+    uint32_t* guide = make_guide(cdf, t->n_keys, G);
+    if (threads < 1)
+        threads = 1;
+    if (threads > 64)
+        threads = 64;
+    pthread_t th[64];
+    cl_job jobs[64];
+    for (int k = 0; k < threads; ++k) {
+        jobs[k].t = t;
+        jobs[k].cdf = cdf;
+        jobs[k].guide = guide;
+        jobs[k].G = G;
+        jobs[k].b = n_probe * (uint64_t)k / (uint64_t)threads;
+        jobs[k].e = n_probe * (uint64_t)(k + 1) / (uint64_t)threads;
+        pthread_create(&th[k], NULL, cl_worker, &jobs[k]);
+    }
+    uint64_t s = 0, c = 0;
+    for (int k = 0; k < threads; ++k) {
+        pthread_join(th[k], NULL);
+        s += jobs[k].sum;
+        c += jobs[k].cnt;
+    }
+    free(guide);
+    *sum = s;
+    *count = c;
+    return 0;
+}
+
+/* Probe ranks through the guide table, for checking it against plain
+ * binary search. */
+int or_zipf_ranks_guided(const double* cdf, uint64_t n, uint64_t seed, uint64_t m, uint64_t* ranks)
+{
+    const uint64_t G = 1ull << 16;
+    uint32_t* guide = make_guide(cdf, n, G);
+    for (uint64_t j = 0; j < m; ++j)
+        ranks[j] = zipf_rank_guided(cdf, n, guide, G, seed, j);
+    free(guide);
+    return 0;
+}

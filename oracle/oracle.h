@@ -163,6 +163,11 @@ int or_expected_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, co
                            int threads, uint64_t* sum, uint64_t* count);
 int or_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,
                       void* build, void* probe);
+/* The clustered-key workload (table from pj_clusters_make): expected output
+ * of projection [B$0, B$1, P$1]; Zipf ranks through a guide table. */
+int or_expected_clustered(const void* table, uint64_t n_probe, const double* cdf, int threads,
+                          uint64_t* sum, uint64_t* count);
+int or_zipf_ranks_guided(const double* cdf, uint64_t n, uint64_t seed, uint64_t m, uint64_t* ranks);
 
 #ifdef __cplusplus
 }

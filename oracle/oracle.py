@@ -83,6 +83,9 @@ def lib():
                                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
             "or_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
+            "or_expected_clustered": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int,
+                                                C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+            "or_zipf_ranks_guided": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
             "or_gen_pkfk_probe_range": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                                   C.c_uint64, C.c_uint64, C.c_void_p]),
         }
@@ -249,4 +252,11 @@ def expected_pairs_zipf(seed, n_build, n_probe, cdf, threads=8):
     s, c = C.c_uint64(), C.c_uint64()
     lib().or_expected_pairs_zipf(seed, n_build, n_probe, cdf.ctypes.data_as(C.c_void_p), threads, C.byref(s),
                                  C.byref(c))
+    return s.value, c.value
+
+
+def expected_clustered(table, n_probe, cdf, threads=8):
+    s, c = C.c_uint64(), C.c_uint64()
+    lib().or_expected_clustered(table.ctypes.data_as(C.c_void_p), n_probe, cdf.ctypes.data_as(C.c_void_p), threads,
+                                C.byref(s), C.byref(c))
     return s.value, c.value

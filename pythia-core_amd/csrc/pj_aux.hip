@@ -108,6 +108,31 @@ __global__ void k_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_pro
     }
 }
 
+__global__ void k_gen_clustered(const pj_cluster_table* __restrict__ t, uint64_t n_probe,
+                                const double* __restrict__ cdf, uint64_t* __restrict__ build,
+                                uint64_t* __restrict__ probe)
+{
+    __shared__ pj_cluster_table st;   // 24.6 KiB: the table, once per block
+    for (uint32_t w = threadIdx.x; w < sizeof(pj_cluster_table) / 8; w += blockDim.x)
+        reinterpret_cast<uint64_t*>(&st)[w] = reinterpret_cast<const uint64_t*>(t)[w];
+    __syncthreads();
+    const uint64_t nb = st.n_keys;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb + n_probe; i += stride) {
+        uint64_t k, v;
+        if (i < nb) {
+            pj_clustered_build(&st, i, &k, &v);
+            build[2 * i] = k;
+            build[2 * i + 1] = v;
+        } else {
+            const uint64_t j = i - nb;
+            pj_clustered_probe_with_rank(&st, pj_zipf_rank(cdf, nb, st.seed, j), j, &k, &v);
+            probe[2 * j] = k;
+            probe[2 * j + 1] = v;
+        }
+    }
+}
+
 template <class F>
 void host_parallel(uint64_t n, F f)
 {
@@ -323,6 +348,116 @@ int pj_gen_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const d
         } else {
             pj_pair_zipf_probe(cdf, n_build, seed, i - n_build, &t.a, &t.b, &t.pay);
             q[i - n_build] = t;
+        }
+    });
+    return PJ_OK;
+}
+
+uint64_t pj_cluster_table_size(void) { return sizeof(pj_cluster_table); }
+
+int pj_clusters_make(uint64_t seed, uint32_t nclusters, uint64_t n_keys, uint32_t log2_wmin, uint32_t log2_wmax,
+                     double weight_z, void* table_out)
+{
+    pj_cluster_table* t = (pj_cluster_table*)table_out;
+    if (!t || nclusters == 0 || nclusters > PJ_MAX_CLUSTERS || log2_wmin > log2_wmax || log2_wmax > 40)
+        return fail(PJ_ERR_INVALID, "bad cluster parameters");
+    memset(t, 0, sizeof(*t));
+    t->nclusters = nclusters;
+    t->seed = seed;
+    t->n_keys = n_keys;
+    std::vector<uint64_t> centre(nclusters), width(nclusters), count(nclusters);
+    std::vector<double> weight(nclusters);
+    double wsum = 0;
+    for (uint32_t c = 0; c < nclusters; ++c) {
+        centre[c] = pj_rand64(seed, 10, c) >> 2;   // [0, 2^62)
+        const double u = (double)(pj_rand64(seed, 11, c) >> 11) / 9007199254740992.0;
+        width[c] = (uint64_t)std::floor(std::pow(2.0, log2_wmin + u * (log2_wmax - log2_wmin)));
+        weight[c] = std::pow((double)(c + 1), -weight_z);
+        wsum += weight[c];
+    }
+    uint64_t placed = 0;
+    for (uint32_t c = 0; c < nclusters; ++c) {
+        count[c] = std::min<uint64_t>(width[c], (uint64_t)std::llround((double)n_keys * weight[c] / wsum));
+        placed += count[c];
+    }
+    if (placed > n_keys) {   // rounding: trim from the lightest clusters
+        for (uint32_t c = nclusters; c-- > 0 && placed > n_keys;) {
+            const uint64_t d = std::min<uint64_t>(count[c], placed - n_keys);
+            count[c] -= d;
+            placed -= d;
+        }
+    }
+    for (uint32_t c = 0; c < nclusters && placed < n_keys; ++c) {   // remainder to clusters with room
+        const uint64_t d = std::min<uint64_t>(width[c] - count[c], n_keys - placed);
+        count[c] += d;
+        placed += d;
+    }
+    if (placed != n_keys)
+        return fail(PJ_ERR_INVALID, "clusters hold only %llu of %llu keys", (unsigned long long)placed,
+                    (unsigned long long)n_keys);
+    std::vector<uint32_t> order(nclusters);
+    for (uint32_t c = 0; c < nclusters; ++c)
+        order[c] = c;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return centre[a] < centre[b]; });
+    uint64_t prev_end = 0, g = 0;
+    for (uint32_t k = 0; k < nclusters; ++k) {
+        const uint32_t c = order[k];
+        uint64_t ctr = std::max(centre[c], prev_end);   // keep clusters disjoint
+        t->centre[k] = ctr;
+        t->width[k] = width[c];
+        t->start[k] = g;
+        g += count[c];
+        prev_end = ctr + width[c];
+    }
+    t->start[nclusters] = g;
+    return PJ_OK;
+}
+
+int pj_gen_clustered(const void* table, uint64_t n_probe, const double* cdf, void* build_dst, void* probe_dst,
+                     int on_device, void* stream)
+{
+    const pj_cluster_table* t = (const pj_cluster_table*)table;
+    if (!t || (t->n_keys && !build_dst) || (n_probe && (!probe_dst || !cdf || t->n_keys == 0)))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (on_device) {
+        hipStream_t st = (hipStream_t)stream;
+        void* dt = nullptr;
+        double* dcdf = nullptr;
+        PJ_HIP(hipMalloc(&dt, sizeof(*t)));
+        hipError_t e = hipMemcpy(dt, t, sizeof(*t), hipMemcpyHostToDevice);
+        if (e == hipSuccess && n_probe) {
+            e = hipMalloc(&dcdf, t->n_keys * sizeof(double));
+            if (e == hipSuccess)
+                e = hipMemcpy(dcdf, cdf, t->n_keys * sizeof(double), hipMemcpyHostToDevice);
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_gen_clustered, dim3(4096), dim3(256), 0, st, (const pj_cluster_table*)dt, n_probe,
+                               dcdf, (uint64_t*)build_dst, (uint64_t*)probe_dst);
+            e = hipGetLastError();
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(st);
+        }
+        (void)hipFree(dt);
+        if (dcdf)
+            (void)hipFree(dcdf);
+        if (e != hipSuccess)
+            return fail(PJ_ERR_DEVICE, "clustered generator failed: %s", hipGetErrorString(e));
+        return PJ_OK;
+    }
+    uint64_t* b = (uint64_t*)build_dst;
+    uint64_t* q = (uint64_t*)probe_dst;
+    const uint64_t nb = t->n_keys;
+    host_parallel(nb + n_probe, [&](uint64_t i) {
+        uint64_t k, v;
+        if (i < nb) {
+            pj_clustered_build(t, i, &k, &v);
+            b[2 * i] = k;
+            b[2 * i + 1] = v;
+        } else {
+            const uint64_t j = i - nb;
+            pj_clustered_probe_with_rank(t, pj_zipf_rank(cdf, nb, t->seed, j), j, &k, &v);
+            q[2 * j] = k;
+            q[2 * j + 1] = v;
         }
     });
     return PJ_OK;

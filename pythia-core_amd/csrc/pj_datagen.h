@@ -166,3 +166,50 @@ PJ_HD uint64_t pj_pair_zipf_probe(const double* cdf, uint64_t n_build, uint64_t 
     *pay = pj_rand64(seed, 3, j);
     return idx;
 }
+
+// ---------------------------------------------------------------------------
+// Clustered build keys: a table of dense clusters [centre, centre + width),
+// sorted by centre and disjoint; cluster c holds count_c distinct keys
+// centre + perm_width(j), j < count_c. Global key index g lies in the cluster
+// whose [start_c, start_{c+1}) contains it. Build tuple i has global index
+// perm_n(i); probe tuples draw a build index through the Zipf CDF.
+#define PJ_MAX_CLUSTERS 1024
+typedef struct {
+    uint32_t nclusters;
+    uint64_t seed;
+    uint64_t n_keys;
+    uint64_t centre[PJ_MAX_CLUSTERS];
+    uint64_t width[PJ_MAX_CLUSTERS];
+    uint64_t start[PJ_MAX_CLUSTERS + 1];
+} pj_cluster_table;
+
+PJ_HD uint64_t pj_cluster_key(const pj_cluster_table* t, uint64_t g)
+{
+    uint32_t lo = 0, hi = t->nclusters;   // last c with start[c] <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (t->start[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t j = g - t->start[lo];
+    return t->centre[lo] + pj_perm(j, t->width[lo], t->seed ^ (0xc1u + (uint64_t)lo));
+}
+
+PJ_HD void pj_clustered_build(const pj_cluster_table* t, uint64_t i, uint64_t* key, uint64_t* pay)
+{
+    *key = pj_cluster_key(t, pj_perm(i, t->n_keys, t->seed ^ 0x9e11ull));
+    *pay = pj_rand64(t->seed, 1, i);
+}
+
+// Build index of the rank-th most frequent probe key (a seeded shuffle).
+PJ_HD uint64_t pj_clustered_probe_with_rank(const pj_cluster_table* t, uint64_t rank, uint64_t j, uint64_t* key,
+                                            uint64_t* pay)
+{
+    const uint64_t idx = pj_zipf_build_index(rank, t->n_keys, t->seed);
+    uint64_t bp;
+    pj_clustered_build(t, idx, key, &bp);
+    *pay = pj_rand64(t->seed, 3, j);
+    return idx;
+}

@@ -87,6 +87,11 @@ _SIGS = {
                                C.POINTER(C.c_uint32)]),
     "pj_set_error": (None, [C.c_char_p]),
     "pj_zipf_cdf": (C.c_int, [C.c_uint64, C.c_double, C.c_void_p]),
+    "pj_clusters_make": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double,
+                                   C.c_void_p]),
+    "pj_gen_clustered": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                   C.c_void_p]),
+    "pj_cluster_table_size": (C.c_uint64, []),
     "pj_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),
 }
@@ -225,3 +230,12 @@ def zipf_cdf(n, z):
     cdf = np.zeros(n, np.float64)
     check(lib().pj_zipf_cdf(n, z, cdf.ctypes.data_as(C.c_void_p)))
     return cdf
+
+
+def cluster_table(seed, n_keys, nclusters=256, log2_wmin=10, log2_wmax=24, weight_z=1.2):
+    """The clustered-key workload's cluster table (an opaque byte buffer)."""
+    import numpy as np
+    t = np.zeros(lib().pj_cluster_table_size(), np.uint8)
+    check(lib().pj_clusters_make(seed, nclusters, n_keys, log2_wmin, log2_wmax, weight_z,
+                                 t.ctypes.data_as(C.c_void_p)))
+    return t

@@ -345,3 +345,47 @@ def test_full_8m_x_128m_zipf_composite(z):
     _say("case: 8M x 128M zipf composite z=%.1f" % z)
     cnt, s, ec, es = _full_zipf(z, 1 << 23, 1 << 27, 303)
     assert (cnt, s) == (ec, es)
+
+
+def _clustered(seed, n, m, z=1.0, **kw):
+    t = P.cluster_table(seed, n, **kw)
+    cdf = O.zipf_cdf(n, z)
+    db = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+    dp = torch.empty(m * 16, dtype=torch.uint8, device="cuda")
+    P.check(P.lib().pj_gen_clustered(t.ctypes.data_as(C.c_void_p), m, cdf.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()), 1, None))
+    return t, cdf, db, dp
+
+
+_LEARNED_PENDING = pytest.mark.skip(reason="learned partitioner not built yet")
+_PARTS = ["radix", pytest.param("learned", marks=_LEARNED_PENDING)]
+
+
+@pytest.mark.parametrize("partitioner", _PARTS)
+def test_clustered_vs_oracle(partitioner):
+    _say("case: clustered keys, %s" % partitioner)
+    n, m = 300000, 1200000
+    t, cdf, db, dp = _clustered(45, n, m, nclusters=64, log2_wmin=10, log2_wmax=17)
+    b = db.cpu().numpy()
+    p = dp.cpu().numpy()
+    got, tm = compare(HEADLINE, b, p, n, m, partitioner=partitioner, timing=True)
+    assert got.shape[0] == m
+
+
+@pytest.mark.parametrize("partitioner", _PARTS)
+def test_full_64m_x_512m_clustered(partitioner):
+    _say("case: 64M x 512M clustered, %s" % partitioner)
+    n, m = 1 << 26, 1 << 29
+    t, cdf, db, dp = _clustered(404, n, m)
+    j = P.Join(P.make_desc(**HEADLINE, partitioner=partitioner, timing=True))
+    out = torch.empty(m * 24, dtype=torch.uint8, device="cuda")
+    cnt = j.run_device(db.data_ptr(), n, dp.data_ptr(), m, out.data_ptr(), m)
+    s, _ = P.checksum_device(out.data_ptr(), cnt, 24)
+    used = j.timings().used_learned
+    j.close()
+    del db, dp, out
+    torch.cuda.empty_cache()
+    es, ec = O.expected_clustered(t, m, cdf, threads=16)
+    assert (cnt, s) == (ec, es)
+    if partitioner == "learned":
+        assert used == 1

@@ -371,3 +371,57 @@ def test_expected_pairs_zipf_matches_restated_hashjoin():
     s, c = O.expected_pairs_zipf(5, n, m, cdf)
     assert out.shape[0] == c == m
     assert O.checksum(out, 24)[0] == s
+
+
+# ------------------------------------------------- clustered-key workload
+
+def _ranks_binary_search(cdf, seed, m):
+    # pj_zipf_rank restated with numpy: same uniforms (splitmix64 streams),
+    # first CDF entry > u; u >= the last entry never occurs here
+    import pjoin as P  # noqa: F401
+    out = []
+    M = (1 << 64) - 1
+
+    def fin(x):
+        x ^= x >> 30
+        x = (x * 0xBF58476D1CE4E5B9) & M
+        x ^= x >> 27
+        x = (x * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+    for j in range(m):
+        a = 0
+        while True:
+            r = fin((fin(seed ^ (((5 + a) * 0xD1B54A32D192ED03) & M)) + j * 0x9E3779B97F4A7C15) & M) >> 2
+            u = r / float(1 << 62)
+            if u < cdf[-1]:
+                break
+            a += 1
+        out.append(int(np.searchsorted(cdf, u, side="right")))
+    return out
+
+
+def test_guided_zipf_rank_equals_binary_search():
+    import ctypes as C
+    cdf = O.zipf_cdf(50000, 1.0)
+    ranks = np.zeros(3000, np.uint64)
+    O.lib().or_zipf_ranks_guided(cdf.ctypes.data_as(C.c_void_p), cdf.size, 17, 3000, ranks.ctypes.data)
+    assert ranks.tolist() == _ranks_binary_search(cdf, 17, 3000)
+
+
+def test_cluster_table_and_expected_clustered():
+    import ctypes as C
+    import pjoin as P
+    n, m = 200000, 300000
+    t = P.cluster_table(41, n, nclusters=64, log2_wmin=10, log2_wmax=16)
+    cdf = O.zipf_cdf(n, 1.0)
+    b = np.zeros(2 * n, np.uint64)
+    p = np.zeros(2 * m, np.uint64)
+    P.check(P.lib().pj_gen_clustered(t.ctypes.data_as(C.c_void_p), m, cdf.ctypes.data_as(C.c_void_p),
+                                     b.ctypes.data_as(C.c_void_p), p.ctypes.data_as(C.c_void_p), 0, None))
+    keys = b[0::2]
+    assert len(np.unique(keys)) == n and keys.max() < (1 << 63)
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"], buckets=1 << 18)
+    out, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    s, c = O.expected_clustered(t, m, cdf)
+    assert out.shape[0] == c == m
+    assert O.checksum(out, 24)[0] == s

@@ -172,7 +172,8 @@ def main():
     for _ in range(args.warmup):
         n = step()
     torch.cuda.synchronize()
-    phases = {"build_partition_ms": 0.0, "probe_partition_ms": 0.0, "join_ms": 0.0, "total_ms": 0.0}
+    phases = {"model_ms": 0.0, "build_partition_ms": 0.0, "probe_partition_ms": 0.0, "join_ms": 0.0,
+              "total_ms": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         n = step()
@@ -224,7 +225,8 @@ def main():
                    "probe_tuples": npr, "probe_misses": nmiss,
                    "tuple_bytes": 16, "out_tuple_bytes": w_out, "out_tuples": n,
                    "partitioner": args.partitioner, "partitions": t.partitions,
-                   "subpartitions": t.subpartitions, "work_items": t.work_items},
+                   "subpartitions": t.subpartitions, "work_items": t.work_items,
+                   "used_learned": t.used_learned, "spilled_items": t.spilled_items},
         "hbm_gbs_algorithmic": job_gbs,
         "hbm_frac_of_peak": job_gbs / HBM_PEAK_GBS,
         "phases_ms": phases,

@@ -204,6 +204,62 @@ int pj_gen_clustered(const void* table, uint64_t n_probe, const double* cdf, voi
 /* Bytes of the cluster table pj_clusters_make fills (at most 1024 clusters). */
 uint64_t pj_cluster_table_size(void);
 
+/* ---------------------------------------------------------------- learned
+ * The learned partitioner's model, a two-stage piecewise-linear CDF over
+ * ordered keys (the key with its sign bit flipped, so unsigned order is the
+ * signed order of the reference's IntLongEqual keys). No reference
+ * counterpart: it replaces the hash bits that choose a partition in
+ * HashJoinOp's TupleHasher (hash.h:109-197) and PartitionOp's ranges
+ * (partition.cpp:166-200) with ranges fitted to the key distribution.
+ *
+ * Root: the sorted sample is cut into segments wherever two neighbouring
+ * sample keys are more than 2^32 apart; segment s owns leaves
+ * seg_leaf0[s] .. seg_leaf0[s+1]-1 of equal key width seg_lw[s] starting at
+ * seg_lo[s]. Leaf j: rank(key) ~ leaf_start[j] + a[j] + b[j] * x with
+ * x = (key - leaf_lo(j)) / seg_lw in [0, 1), fitted by least squares over
+ * the samples of leaves j-window .. j+window of its segment, then refined by
+ * gradient steps on that loss plus lambda * (jump at each inner leaf edge)^2. */
+#define PJ_MODEL_LEAVES 4096
+#define PJ_MODEL_MAX_SEGS 1024
+typedef struct {
+    uint32_t nleaves;     /* leaves in use */
+    uint32_t nseg;        /* root segments */
+    uint64_t n;           /* sample keys fitted */
+    uint64_t n_keys;      /* keys of the relation sampled */
+    uint32_t window;      /* least-squares window, leaves either side */
+    uint32_t iters;       /* gradient steps run */
+    float lambda;         /* weight of the continuity term */
+    float loss_lsq;       /* loss after the least-squares fit */
+    float loss;           /* loss after the gradient steps */
+    uint32_t reserved;
+    uint64_t seg_lo[PJ_MODEL_MAX_SEGS];
+    uint64_t seg_hi[PJ_MODEL_MAX_SEGS];
+    uint64_t seg_lw[PJ_MODEL_MAX_SEGS];
+    uint32_t seg_leaf0[PJ_MODEL_MAX_SEGS + 1];
+    uint16_t leaf_seg[PJ_MODEL_LEAVES];
+    uint32_t leaf_start[PJ_MODEL_LEAVES + 1];
+    float a[PJ_MODEL_LEAVES];
+    float b[PJ_MODEL_LEAVES];
+} pj_model;
+
+uint64_t pj_model_size(void);
+/* Sorts n 64-bit unsigned keys in device memory, in place (LSD radix sort). */
+int pj_sort_u64(uint64_t* d_keys, uint64_t n, void* stream);
+/* Fits the model to n ordered sample keys in device memory (any order; they
+ * are sorted into a copy) drawn from a relation of n_keys keys, running
+ * iters gradient steps after the least-squares fit. */
+int pj_model_fit(const uint64_t* d_keys, uint64_t n, uint64_t n_keys, uint32_t iters, pj_model* model,
+                 void* stream);
+/* Loss and gradient at the model's parameters over its (sorted) sample:
+ * grad[j] = dL/da[j], grad[nleaves + j] = dL/db[j]. The forward and backward
+ * kernels of one training step. */
+int pj_model_grad(const pj_model* model, const uint64_t* d_sorted, uint64_t n, float* grad, double* loss,
+                  void* stream);
+/* Range boundaries (ordered keys) the model places for build partitions of
+ * at most max_coarse probe partitions; *nbounds = partitions - 1. */
+int pj_model_bounds(const pj_model* model, uint32_t max_coarse, uint64_t* bounds, uint32_t cap,
+                    uint32_t* nbounds);
+
 #ifdef __cplusplus
 }
 #endif

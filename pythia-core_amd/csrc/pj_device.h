@@ -1,0 +1,98 @@
+// pj_device.h -- device helpers shared by the partition, join and model
+// kernels: the canonical record, streaming loads and stores, and the two
+// partition functions (radix bits of a key mix, or ordered-key ranges).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pj_internal.h"
+
+namespace pj {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Streaming (non-temporal) 16-byte load and 8-byte store.
+__device__ __forceinline__ uint4 ld_stream(const uint4* p)
+{
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_stream2(uint32_t* p, uint32_t a, uint32_t b)
+{
+    u32x2 v = {a, b};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(p));
+}
+
+__device__ __forceinline__ uint64_t rec_key(uint4 r) { return (uint64_t)r.x | ((uint64_t)r.y << 32); }
+
+// The key as an unsigned value whose order is the signed order of the
+// normalised 64-bit key: range partitions are ranges of this value.
+__host__ __device__ __forceinline__ uint64_t ord_key(uint64_t k) { return k ^ 0x8000000000000000ull; }
+
+// Key mix used for radix partition, sub-partition and LDS bucket bits.
+__device__ __forceinline__ uint64_t key_mix(uint64_t k) { return pj_fin64(k); }
+
+__device__ __forceinline__ uint32_t radix_part(uint64_t h, uint32_t bits)
+{
+    return bits ? (uint32_t)(h >> (64 - bits)) : 0u;
+}
+
+template <bool IDENT>
+__device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint64_t i, const SideLayout& L)
+{
+    if (IDENT) {
+        return ld_stream(reinterpret_cast<const uint4*>(in) + i);
+    } else {
+        const uint32_t* t = in + i * L.tw;
+        uint4 r;
+        r.x = t[L.k0];
+        r.y = L.k1 >= 0 ? t[L.k1] : (uint32_t)((int32_t)r.x >> 31);
+        r.z = L.p0 >= 0 ? t[L.p0] : 0u;
+        r.w = L.p1 >= 0 ? t[L.p1] : 0u;
+        return r;
+    }
+}
+
+// Range partition of ordered key u: the number of the F - 1 ascending
+// boundaries that are <= u, by a fixed-step search over the boundary array
+// padded with ~0 to p2 - 1 entries (p2 a power of two >= F). Every lane runs
+// the same steps, so a wave never diverges; the clamp maps u = ~0 (which
+// also passes the padding) to the last partition.
+__device__ __forceinline__ uint32_t range_part(const uint64_t* bnd, uint32_t F, uint32_t p2, uint64_t u)
+{
+    uint32_t pos = 0;
+    for (uint32_t step = p2 >> 1; step; step >>= 1)
+        if (bnd[pos + step - 1] <= u)
+            pos += step;
+    return pos < F - 1 ? pos : F - 1;
+}
+
+// Partition of a key: RANGE selects the ordered-key ranges in lds_bnd,
+// otherwise the top pf.bits bits of the key mix.
+template <bool RANGE>
+__device__ __forceinline__ uint32_t part_of(uint64_t key, const PartFn& pf, const uint64_t* lds_bnd)
+{
+    if (RANGE)
+        return range_part(lds_bnd, pf.F, pf.p2, ord_key(key));
+    return radix_part(key_mix(key), pf.bits);
+}
+
+// Copies the range boundaries into LDS, padded with ~0 to p2 - 1 entries (a
+// no-op for radix partitions); the caller synchronises before the first
+// lookup.
+template <bool RANGE>
+__device__ __forceinline__ void load_bounds(uint64_t* lds_bnd, const PartFn& pf)
+{
+    if (RANGE)
+        for (uint32_t i = threadIdx.x; i + 1 < pf.p2; i += blockDim.x)
+            lds_bnd[i] = i + 1 < pf.F ? pf.bnd[i] : ~0ull;
+}
+
+// LDS bytes a kernel reserves for range boundaries.
+__host__ __device__ __forceinline__ uint32_t bounds_lds_bytes(const PartFn& pf, bool range)
+{
+    return range && pf.p2 > 1 ? (pf.p2 - 1) * 8u : 0u;
+}
+
+}  // namespace pj

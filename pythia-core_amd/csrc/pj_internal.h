@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/pjoin.h"
 #include "pj_datagen.h"
@@ -63,11 +64,76 @@ constexpr uint32_t kTableBuckets = 4096;   // bucket directory entries
 constexpr uint32_t kLdsBytes = 163840;     // gfx950 LDS per CU
 constexpr uint32_t kScatterThreads = 512;
 
+// LDS table formats of range-partitioned build partitions.
+enum : uint32_t {
+    kFmtCodes = 0,   // 32-bit key codes (key - base) in hash buckets, any multiplicity
+    kFmtDense = 1,   // the partition holds every key of [base, base + r_cnt) once:
+                     // payloads indexed by key - base, no keys stored
+};
+constexpr uint32_t kCodeCap = 12928;    // records per code table (12 B each + bucket directory)
+constexpr uint32_t kDenseCap = 20096;   // records per dense table (8 B each + presence bitmap)
+
 struct Item {           // one join work item
     uint32_t r_off, r_cnt;   // build records [r_off, r_off + r_cnt) of R_part
     uint32_t s_off, s_cnt;   // probe records of S_part
     uint32_t part;           // build partition id; records with other ids are skipped
-    uint32_t unused;
+    uint32_t fmt;            // range partitions: kFmtCodes or kFmtDense
+    uint64_t lo, hi;         // range partitions: ordered-key range [lo, hi] of the build partition
+    uint64_t base;           // range partitions: smallest ordered build key of the partition
 };
+
+// A partition function: radix (the top `bits` bits of the key mix, F =
+// 2^bits) or ranges of the ordered key (F - 1 ascending boundaries in bnd,
+// a device array; partition p holds bnd[p-1] <= key < bnd[p]; p2 is the
+// power of two >= F the lookup steps over).
+struct PartFn {
+    uint32_t F;
+    uint32_t bits;
+    const uint64_t* bnd;
+    uint32_t p2;
+};
+
+inline PartFn radix_fn(uint32_t bits) { return PartFn{1u << bits, bits, nullptr, 0}; }
+inline PartFn range_fn(uint32_t F, const uint64_t* bnd)
+{
+    uint32_t p2 = 1;
+    while (p2 < F)
+        p2 <<= 1;
+    return PartFn{F, 0, bnd, p2};
+}
+
+// ---------------------------------------------------------------- host
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+    ~DevBuf() { release(); }
+};
+
+// The range partitioning chosen by the learned model for one build side.
+struct RangePlan {
+    bool ok = false;                 // false: the model declined (radix is used)
+    std::string why;                 // reason when declined
+    uint32_t nfine = 0, ncoarse = 0;
+    std::vector<uint64_t> fine;      // nfine - 1 ordered-key boundaries (build partitions)
+    std::vector<uint64_t> coarse;    // ncoarse - 1 boundaries (probe partitions), a subset of fine
+    std::vector<uint32_t> coarse_of; // nfine: probe partition of each build partition
+    uint32_t nseg = 0, nleaves = 0;
+    float loss_lsq = 0.f, loss = 0.f;
+};
+
+// Device state of the learned partitioner (pj_learned.hip).
+struct LearnedCtx {
+    DevBuf sample, tmp, hist, bsum, model, params, moments, resid, bounds, segs, loss;
+    std::vector<uint8_t> host_model;   // pj_model
+    ~LearnedCtx() {}
+};
+
+// Trains the CDF model on a sample of the build keys and derives the range
+// partitioning (pj_learned.hip). Returns PJ_OK with plan->ok false when the
+// key distribution does not suit range partitions.
+int learned_plan(LearnedCtx& ctx, const void* d_build, uint64_t nb, const SideLayout& L, uint32_t max_coarse,
+                 hipStream_t st, RangePlan* plan);
 
 }  // namespace pj

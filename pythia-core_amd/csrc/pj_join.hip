@@ -1,4 +1,4 @@
-// pj_join.hip -- the radix-partitioned hash join for gfx950.
+// pj_join.hip -- the partitioned hash join for gfx950.
 //
 // What it replaces in the reference (paths relative to its tree):
 //   HashJoinOp::scanStart / buildFromPage  operators/join.cpp:548-637, 838-884
@@ -6,11 +6,12 @@
 //   HashTable allocate / iterator          util/hashtable.cpp:254-297, util/hashtable.h:126-171
 //   ConjunctionEqualsEvaluator on keys     conjunctionevaluator.h:65-73, rawcompfns.cpp:48-180
 // The reference shares one chained hash table among T threads. Here both
-// relations are radix-partitioned on the top bits of a 64-bit mix of the key
-// (histogram, exclusive scan, LDS-sorted scatter), every build partition is
-// loaded into an LDS-resident bucket table, and the probe partition streams
-// past it; a whole workgroup reserves output space with one atomic per tile
-// and lanes write adjacent output tuples.
+// relations are partitioned (histogram, exclusive scan, LDS-buffered
+// scatter), either on the top bits of a 64-bit mix of the key (radix) or on
+// ranges of the key chosen by the learned CDF model (pj_learned.hip). Every
+// build partition is loaded into an LDS-resident table and the probe
+// partition streams past it; a whole workgroup reserves output space with one
+// atomic per round and lanes write adjacent output tuples.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,55 +22,11 @@
 #include <mutex>
 #include <vector>
 
+#include "pj_device.h"
 #include "pj_internal.h"
 #include "pj_scan.h"
 
 namespace pj {
-
-// ======================================================================
-// device helpers
-// ======================================================================
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-// Streaming (non-temporal) 16-byte load and 8-byte store.
-__device__ __forceinline__ uint4 ld_stream(const uint4* p)
-{
-    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ void st_stream2(uint32_t* p, uint32_t a, uint32_t b)
-{
-    u32x2 v = {a, b};
-    __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(p));
-}
-
-__device__ __forceinline__ uint64_t rec_key(uint4 r) { return (uint64_t)r.x | ((uint64_t)r.y << 32); }
-
-// Key mix used for partition, sub-partition and LDS bucket bits.
-__device__ __forceinline__ uint64_t key_mix(uint64_t k) { return pj_fin64(k); }
-
-__device__ __forceinline__ uint32_t radix_part(uint64_t h, uint32_t bits)
-{
-    return bits ? (uint32_t)(h >> (64 - bits)) : 0u;
-}
-
-template <bool IDENT>
-__device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint64_t i, const SideLayout& L)
-{
-    if (IDENT) {
-        return ld_stream(reinterpret_cast<const uint4*>(in) + i);
-    } else {
-        const uint32_t* t = in + i * L.tw;
-        uint4 r;
-        r.x = t[L.k0];
-        r.y = L.k1 >= 0 ? t[L.k1] : (uint32_t)((int32_t)r.x >> 31);
-        r.z = L.p0 >= 0 ? t[L.p0] : 0u;
-        r.w = L.p1 >= 0 ? t[L.p1] : 0u;
-        return r;
-    }
-}
 
 // ======================================================================
 // partition pass: histogram, scan, scatter
@@ -78,23 +35,39 @@ __device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint6
 // A partition's records from one block form a segment padded to a whole
 // number of B-record chunks, so the write-combining scatter below only ever
 // writes whole, aligned chunks. Padding records carry a key of another
-// partition; the join drops any record whose partition bits are not the
-// work item's.
+// partition; the join drops any record outside the work item's partition.
 
 constexpr int kHistThreads = 512;
 constexpr int kHistItems = 8;
 
 // hist[p * nblk + blk] = tuples of block blk's chunk in partition p, rounded
-// up to a multiple of pad.
-template <bool IDENT>
+// up to a multiple of pad. With pmin/pmax (range partitions of a build side)
+// it also lowers pmin[p] / raises pmax[p] to the partition's ordered keys;
+// with pid (range partitions) it stores every tuple's partition, so the
+// scatter need not search the boundaries again.
+template <bool IDENT, bool RANGE>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ in, uint64_t n,
-                                                        SideLayout L, uint32_t bits, uint64_t chunk,
-                                                        uint32_t nblk, uint32_t pad, uint32_t* __restrict__ hist)
+                                                        SideLayout L, PartFn pf, uint64_t chunk,
+                                                        uint32_t nblk, uint32_t pad, uint32_t* __restrict__ hist,
+                                                        unsigned long long* __restrict__ pmin,
+                                                        unsigned long long* __restrict__ pmax,
+                                                        uint16_t* __restrict__ pid)
 {
-    extern __shared__ uint32_t cnt[];
-    const uint32_t F = 1u << bits;
-    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+    extern __shared__ uint64_t smh[];
+    const uint32_t F = pf.F;
+    const bool mm = RANGE && pmin != nullptr;
+    uint64_t* bnd = smh;
+    unsigned long long* lmin = reinterpret_cast<unsigned long long*>(bnd + (RANGE ? pf.p2 - 1 : 0));
+    unsigned long long* lmax = lmin + (mm ? F : 0);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(lmax + (mm ? F : 0));
+    load_bounds<RANGE>(bnd, pf);
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads) {
         cnt[p] = 0;
+        if (mm) {
+            lmin[p] = ~0ull;
+            lmax[p] = 0ull;
+        }
+    }
     __syncthreads();
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
@@ -109,13 +82,27 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
             uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
-            if (i < end)
-                atomicAdd(&cnt[radix_part(key_mix(rec_key(r[k])), bits)], 1u);
+            if (i < end) {
+                const uint64_t key = rec_key(r[k]);
+                const uint32_t p = part_of<RANGE>(key, pf, bnd);
+                atomicAdd(&cnt[p], 1u);
+                if (RANGE && pid)
+                    pid[i] = (uint16_t)p;
+                if (mm) {
+                    atomicMin(&lmin[p], (unsigned long long)ord_key(key));
+                    atomicMax(&lmax[p], (unsigned long long)ord_key(key));
+                }
+            }
         }
     }
     __syncthreads();
-    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads) {
         hist[(uint64_t)p * nblk + blockIdx.x] = (cnt[p] + pad - 1) / pad * pad;
+        if (mm && cnt[p]) {
+            atomicMin(&pmin[p], lmin[p]);
+            atomicMax(&pmax[p], lmax[p]);
+        }
+    }
 }
 
 // part_start[p] = first record of partition p; part_start[F] = total.
@@ -139,17 +126,20 @@ constexpr int kWcItems = 4;
 // leaves as one chunk (a B-lane group writes it), whole chunks of overflow
 // records are written straight after it, and the overflow remainder refills
 // the buffer. Three barriers a round, whatever the skew.
-template <bool IDENT, int B>
+template <bool IDENT, int B, bool RANGE>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
-                                                            SideLayout L, uint32_t bits, uint64_t chunk,
+                                                            SideLayout L, PartFn pf, uint64_t chunk,
                                                             uint32_t nblk, const uint32_t* __restrict__ offs,
-                                                            uint4* __restrict__ out, uint4 pad_other, uint4 pad_zero)
+                                                            uint4* __restrict__ out, uint4 pad_other, uint4 pad_zero,
+                                                            const uint16_t* __restrict__ pid)
 {
     extern __shared__ uint4 sm4[];
-    const uint32_t F = 1u << bits;
-    uint4* buf = sm4;                                           // F * B
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + F * B);  // F: buffered records
-    uint32_t* goff = cnt + F;                                   // F: next chunk's record index
+    const uint32_t F = pf.F;
+    uint4* buf = sm4;                                                          // F * B
+    uint64_t* bnd = reinterpret_cast<uint64_t*>(buf + F * B);                  // F - 1 (range)
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));    // F: buffered records
+    uint32_t* goff = cnt + F;                                                  // F: next chunk's record index
+    load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < F; p += kWcThreads) {
         goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
         cnt[p] = 0;
@@ -159,12 +149,17 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
+    const bool have_pid = RANGE && pid != nullptr;
     uint4 nxt[kWcItems];   // the next round's records, loaded one round ahead
+    uint32_t nxp[kWcItems] = {};
 #pragma unroll
     for (int k = 0; k < kWcItems; ++k) {
         const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
-        if (i < end)
+        if (i < end) {
             nxt[k] = load_rec<IDENT>(in, i, L);
+            if (have_pid)
+                nxp[k] = pid[i];
+        }
     }
     for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * kWcItems) {
         uint4 r[kWcItems];
@@ -175,15 +170,20 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
             v[k] = i < end;
             r[k] = nxt[k];
+            p[k] = nxp[k];
             const uint64_t i2 = i + (uint64_t)kWcThreads * kWcItems;
-            if (i2 < end)
+            if (i2 < end) {
                 nxt[k] = load_rec<IDENT>(in, i2, L);
+                if (have_pid)
+                    nxp[k] = pid[i2];
+            }
         }
         // claim
 #pragma unroll
         for (int k = 0; k < kWcItems; ++k)
             if (v[k]) {
-                p[k] = radix_part(key_mix(rec_key(r[k])), bits);
+                if (!have_pid)
+                    p[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
                 s[k] = atomicAdd(&cnt[p[k]], 1u);
                 if (s[k] < (uint32_t)B)
                     buf[p[k] * B + s[k]] = r[k];
@@ -234,18 +234,20 @@ constexpr int kScatterItems = 16;
 // Scatter with an LDS-sorted tile, for fan-outs too wide for one LDS buffer
 // per partition: each tile is counting-sorted by partition in LDS, then
 // written so that lanes of a wave store consecutive records of a partition.
-template <bool IDENT>
+template <bool IDENT, bool RANGE>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
-                                                              SideLayout L, uint32_t bits, uint64_t chunk,
+                                                              SideLayout L, PartFn pf, uint64_t chunk,
                                                               uint32_t nblk, const uint32_t* __restrict__ offs,
                                                               uint4* __restrict__ out, uint32_t ts)
 {
     extern __shared__ uint4 sm4[];
-    const uint32_t F = 1u << bits;
+    const uint32_t F = pf.F;
     uint4* tile = sm4;
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(tile + ts);  // F + 1
-    uint32_t* goff = cnt + F + 1;                            // F
-    uint32_t* wsum = goff + F;                               // 9
+    uint64_t* bnd = reinterpret_cast<uint64_t*>(tile + ts);                   // F - 1 (range)
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));   // F + 1
+    uint32_t* goff = cnt + F + 1;                                             // F
+    uint32_t* wsum = goff + F;                                                // 9
+    load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
         goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
             if (i < tn) {
-                part[k] = radix_part(key_mix(rec_key(r[k])), bits);
+                part[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
                 rank[k] = atomicAdd(&cnt[part[k]], 1u);
             }
         }
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < tn; i += kScatterThreads) {
             uint4 v = tile[i];
-            uint32_t p = radix_part(key_mix(rec_key(v)), bits);
+            uint32_t p = part_of<RANGE>(rec_key(v), pf, bnd);
             out[goff[p] + (i - cnt[p])] = v;
         }
         __syncthreads();
@@ -293,10 +295,11 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
 }
 
 // ======================================================================
-// join: LDS-resident bucket table per build partition, streaming probe
+// join: LDS-resident table per build partition, streaming probe
 // ======================================================================
 
 constexpr int kProbeItems = 8;
+constexpr uint32_t kMiscWords = 64;   // LDS words ahead of the table: item, flags, wave bases, wave sums
 
 __device__ __forceinline__ uint32_t pick_word(uint64_t bk, uint2 bp, const uint4& s, uint32_t src)
 {
@@ -328,201 +331,360 @@ __device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t r
             o[w] = pick_word(bk, bp, s, f.src[w]);
 }
 
-// Work items come from a shared counter. Per item the block builds the LDS
-// table of its build records: a counting sort by bucket, keys and payloads
-// in separate arrays, bucket starts as 16-bit offsets. The probe runs in
-// rounds of 8192 records: every wave probes a 512-record slice, the block
-// reserves the round's output rows with one atomic, and the lanes of each
-// wave write adjacent output rows.
+// Bucket of a 32-bit key code (Fibonacci hashing onto 12 bits).
+__device__ __forceinline__ uint32_t code_bucket(uint32_t c) { return (c * 0x9E3779B1u) >> 20; }
+
+// Table views. lookup() counts a probe record's matches and returns the
+// slot of the first; match() tests a later slot of the same bucket against
+// the probe's match key mkey(); bkey() is the build key of a matching slot.
+struct RadixTab {
+    const uint64_t* tkey;
+    const uint2* tpay;
+    const uint16_t* dir;
+    uint32_t bits, bkt_shift, part;
+    __device__ __forceinline__ void lookup(const uint4& s, uint32_t& nm, uint32_t& first) const
+    {
+        const uint64_t key = rec_key(s);
+        const uint64_t h = key_mix(key);
+        if (radix_part(h, bits) == part) {
+            const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
+            const uint32_t lo = b ? dir[b - 1] : 0u, hi = dir[b];
+            for (uint32_t j = lo; j < hi; ++j)
+                if (tkey[j] == key) {
+                    if (nm == 0)
+                        first = j;
+                    nm++;
+                }
+        }
+    }
+    __device__ __forceinline__ uint64_t mkey(const uint4& s) const { return rec_key(s); }
+    __device__ __forceinline__ bool match(uint32_t j, uint64_t mk) const { return tkey[j] == mk; }
+    __device__ __forceinline__ uint64_t bkey(uint32_t j, const uint4&) const { return tkey[j]; }
+    __device__ __forceinline__ uint2 bpay(uint32_t j) const { return tpay[j]; }
+};
+
+struct CodeTab {
+    const uint32_t* tcode;
+    const uint2* tpay;
+    const uint16_t* dir;
+    uint64_t lo, hi, base;
+    __device__ __forceinline__ void lookup(const uint4& s, uint32_t& nm, uint32_t& first) const
+    {
+        const uint64_t u = ord_key(rec_key(s));
+        const uint64_t d = u - base;
+        if (u >= lo && u <= hi && (d >> 32) == 0) {
+            const uint32_t c = (uint32_t)d;
+            const uint32_t b = code_bucket(c);
+            const uint32_t blo = b ? dir[b - 1] : 0u, bhi = dir[b];
+            for (uint32_t j = blo; j < bhi; ++j)
+                if (tcode[j] == c) {
+                    if (nm == 0)
+                        first = j;
+                    nm++;
+                }
+        }
+    }
+    __device__ __forceinline__ uint64_t mkey(const uint4& s) const { return (uint32_t)(ord_key(rec_key(s)) - base); }
+    __device__ __forceinline__ bool match(uint32_t j, uint64_t mk) const { return tcode[j] == (uint32_t)mk; }
+    __device__ __forceinline__ uint64_t bkey(uint32_t, const uint4& s) const { return rec_key(s); }
+    __device__ __forceinline__ uint2 bpay(uint32_t j) const { return tpay[j]; }
+};
+
+struct DenseTab {
+    const uint2* tpay;
+    uint64_t lo, hi, base;
+    uint32_t cnt;
+    __device__ __forceinline__ void lookup(const uint4& s, uint32_t& nm, uint32_t& first) const
+    {
+        const uint64_t u = ord_key(rec_key(s));
+        const uint64_t d = u - base;
+        if (u >= lo && u <= hi && d < cnt) {
+            first = (uint32_t)d;
+            nm = 1;
+        }
+    }
+    __device__ __forceinline__ uint64_t mkey(const uint4&) const { return 0; }
+    __device__ __forceinline__ bool match(uint32_t, uint64_t) const { return true; }
+    __device__ __forceinline__ uint64_t bkey(uint32_t, const uint4& s) const { return rec_key(s); }
+    __device__ __forceinline__ uint2 bpay(uint32_t j) const { return tpay[j]; }
+};
+
+// Streams the probe records [0, s_cnt) of Sp past a built table. The probe
+// runs in rounds of 8192 records: every wave probes a 512-record slice, the
+// block reserves the round's output rows with one atomic, and the lanes of
+// each wave write adjacent output rows. Loads are software-pipelined: a
+// record's slot is refilled with the next round's record as soon as its
+// lookup is done.
+template <class T>
+__device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restrict__ Sp, uint32_t s_cnt,
+                                             uint32_t* __restrict__ out, uint64_t out_cap,
+                                             unsigned long long* __restrict__ out_cursor, const OutFormat& fmt,
+                                             bool kbp, uint32_t* wsum, uint32_t* wbase_lds)
+{
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    constexpr uint32_t kSlice = 64 * kProbeItems;
+    constexpr uint32_t kRound = (kJoinThreads / 64) * kSlice;
+    uint4 nxt[kProbeItems];
+#pragma unroll
+    for (int k = 0; k < kProbeItems; ++k) {
+        const uint32_t i = wave * kSlice + k * 64 + lane;
+        if (i < s_cnt)
+            nxt[k] = ld_stream(Sp + i);
+    }
+    for (uint32_t r0 = 0; r0 < s_cnt; r0 += kRound) {
+        const uint32_t t0 = r0 + wave * kSlice;
+        uint4 s[kProbeItems];
+        uint32_t nm[kProbeItems], first[kProbeItems];
+        bool multi = false;
+#pragma unroll
+        for (int k = 0; k < kProbeItems; ++k) {
+            const uint32_t i = t0 + k * 64 + lane;
+            s[k] = nxt[k];
+            nm[k] = 0;
+            first[k] = 0;
+            if (i < s_cnt)
+                tab.lookup(s[k], nm[k], first[k]);
+            multi |= nm[k] > 1;
+            if (i + kRound < s_cnt)
+                nxt[k] = ld_stream(Sp + i + kRound);
+        }
+        const bool wmulti = __any(multi);
+        uint64_t m[kProbeItems];
+        uint32_t mine = 0, pre = 0, wtotal = 0;
+        if (!wmulti) {
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                m[k] = __ballot(nm[k] != 0);
+                wtotal += (uint32_t)__popcll(m[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k)
+                mine += nm[k];
+            uint32_t x = mine;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (lane >= (uint32_t)d)
+                    x += y;
+            }
+            pre = x - mine;
+            wtotal = __shfl(x, 63);
+        }
+        if (lane == 0)
+            wsum[wave] = wtotal;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t v = lane < kJoinThreads / 64 ? wsum[lane] : 0u;
+            uint32_t x = v;
+#pragma unroll
+            for (int d = 1; d < (int)(kJoinThreads / 64); d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, 64);
+                if (lane >= (uint32_t)d)
+                    x += y;
+            }
+            const uint32_t btotal = __shfl(x, kJoinThreads / 64 - 1);
+            unsigned long long base = 0;
+            if (lane == 0 && btotal)
+                base = atomicAdd(out_cursor, (unsigned long long)btotal);
+            base = __shfl(base, 0);
+            if (lane < kJoinThreads / 64) {
+                const uint64_t wb = base + (x - v);
+                wbase_lds[2 * lane] = (uint32_t)wb;
+                wbase_lds[2 * lane + 1] = (uint32_t)(wb >> 32);
+            }
+        }
+        __syncthreads();
+        if (!wtotal)
+            continue;
+        const uint64_t wbase = (uint64_t)wbase_lds[2 * wave] | ((uint64_t)wbase_lds[2 * wave + 1] << 32);
+        if (wmulti) {
+            // many-to-many: a lane writes all matches of its probes
+            uint64_t row = wbase + pre;
+#pragma unroll
+            for (int k = 0; k < kProbeItems; ++k) {
+                const uint64_t mk = tab.mkey(s[k]);
+                uint32_t left = nm[k];
+                for (uint32_t j = first[k]; left; ++j)
+                    if (tab.match(j, mk)) {
+                        if (row < out_cap)
+                            write_row(out, row, fmt, false, tab.bkey(j, s[k]), tab.bpay(j), s[k]);
+                        row++;
+                        left--;
+                    }
+            }
+            continue;
+        }
+        // at most one match a probe: lanes of a wave write adjacent rows
+        uint64_t row0 = wbase;
+#pragma unroll
+        for (int k = 0; k < kProbeItems; ++k) {
+            if (nm[k]) {
+                const uint64_t row = row0 + (uint64_t)__popcll(m[k] & below);
+                if (row < out_cap)
+                    write_row(out, row, fmt, kbp, tab.bkey(first[k], s[k]), tab.bpay(first[k]), s[k]);
+            }
+            row0 += (uint64_t)__popcll(m[k]);
+        }
+    }
+}
+
+// Two-pass counting sort of build records into an LDS bucket table. Pass 1
+// counts per bucket, a scan turns counts into bucket starts, pass 2 places
+// each record at its bucket's cursor; afterwards dir entry b holds the end of
+// bucket b. Two 16-bit entries share a 32-bit word. bucket_of(r) returns the
+// bucket of record r, or ~0 to skip it; put(j, r) stores r at slot j.
+template <class BucketOf, class Put>
+__device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint32_t r_off, uint32_t r_cnt,
+                                              uint32_t* dirw, uint32_t* wsum, BucketOf bucket_of, Put put)
+{
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t w = tid; w <= kTableBuckets / 2; w += kJoinThreads)
+        dirw[w] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < r_cnt; i += kJoinThreads) {
+        const uint32_t b = bucket_of(R[r_off + i]);
+        if (b != ~0u)
+            atomicAdd(&dirw[b >> 1], 1u << ((b & 1u) * 16u));
+    }
+    __syncthreads();
+    {
+        static_assert(kTableBuckets == 4 * kJoinThreads, "scan layout");
+        const uint32_t w0 = dirw[2 * tid], w1 = dirw[2 * tid + 1];
+        const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16, c2 = w1 & 0xFFFFu, c3 = w1 >> 16;
+        uint32_t total;
+        const uint32_t pre = block_excl_scan<kJoinThreads>(c0 + c1 + c2 + c3, wsum, &total);
+        dirw[2 * tid] = pre | ((pre + c0) << 16);
+        dirw[2 * tid + 1] = (pre + c0 + c1) | ((pre + c0 + c1 + c2) << 16);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < r_cnt; i += kJoinThreads) {
+        const uint4 r = R[r_off + i];
+        const uint32_t b = bucket_of(r);
+        if (b != ~0u) {
+            const uint32_t sh = (b & 1u) * 16u;
+            const uint32_t j = (atomicAdd(&dirw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
+            put(j, r);
+        }
+    }
+    __syncthreads();
+}
+
+// Work items come from a shared counter; per item the block builds the LDS
+// table of its build records and streams the probe records past it.
+// RANGE = false: radix partitions, tables of 64-bit keys. RANGE = true:
+// range partitions, whose tables are dense (payload slot = key - base) or
+// hold 32-bit key codes; a dense item whose keys turn out not to be distinct
+// is rebuilt as a code table, or, when it is too large for one, counted in
+// *fail_ctr (the caller then runs the join on radix partitions).
+template <bool RANGE>
 __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
                                                         const Item* __restrict__ items, uint32_t nitems,
                                                         uint32_t* __restrict__ item_ctr,
                                                         uint32_t* __restrict__ out, uint64_t out_cap,
                                                         unsigned long long* __restrict__ out_cursor,
-                                                        OutFormat fmt, uint32_t bitsR, uint32_t bkt_shift)
+                                                        OutFormat fmt, uint32_t bitsR, uint32_t bkt_shift,
+                                                        uint32_t* __restrict__ fail_ctr)
 {
     extern __shared__ uint64_t lds64[];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint64_t* tkey = lds64;                                           // kTableCap
-    uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);         // kTableCap
-    uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);   // kTableBuckets / 2 + 1
-    const uint16_t* dir = reinterpret_cast<const uint16_t*>(dirw);    // bucket ends
-    uint32_t* wsum = dirw + kTableBuckets / 2 + 1;                    // 17
-    uint32_t* misc = wsum + 17;                                       // [0] item, [2..33] wave bases
+    const uint32_t tid = threadIdx.x;
+    uint32_t* misc = reinterpret_cast<uint32_t*>(lds64);   // [0] item, [1] flag, [2..33] wave bases
+    uint32_t* wsum = misc + 40;                            // 17
+    uint64_t* table = lds64 + kMiscWords / 2;
     const bool kbp = fmt.words == 6 && fmt.src[0] == SRC_BK0 && fmt.src[1] == SRC_BK1 && fmt.src[2] == SRC_BP0 &&
                      fmt.src[3] == SRC_BP1 && fmt.src[4] == SRC_PP0 && fmt.src[5] == SRC_PP1;
-    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
 
     for (;;) {
-        if (tid == 0)
+        if (tid == 0) {
             misc[0] = atomicAdd(item_ctr, 1u);
+            misc[1] = 0;
+        }
         __syncthreads();
         const uint32_t it = misc[0];
         if (it >= nitems)
             break;
         const Item I = items[it];
-
-        // ---- build: counting sort of the item's build records by bucket.
-        // Pass 1 counts, a scan turns counts into bucket starts, pass 2
-        // places each record at its bucket's cursor; afterwards entry b holds
-        // the end of bucket b. Two 16-bit entries share a 32-bit word.
-        for (uint32_t w = tid; w <= kTableBuckets / 2; w += kJoinThreads)
-            dirw[w] = 0;
-        __syncthreads();
-        for (uint32_t i = tid; i < I.r_cnt; i += kJoinThreads) {
-            const uint64_t h = key_mix(rec_key(R[I.r_off + i]));
-            if (radix_part(h, bitsR) == I.part) {   // drops padding records
-                const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
-                atomicAdd(&dirw[b >> 1], 1u << ((b & 1u) * 16u));
-            }
-        }
-        __syncthreads();
-        {
-            static_assert(kTableBuckets == 4 * kJoinThreads, "scan layout");
-            const uint32_t w0 = dirw[2 * tid], w1 = dirw[2 * tid + 1];
-            const uint32_t c0 = w0 & 0xFFFFu, c1 = w0 >> 16, c2 = w1 & 0xFFFFu, c3 = w1 >> 16;
-            uint32_t total;
-            const uint32_t pre = block_excl_scan<kJoinThreads>(c0 + c1 + c2 + c3, wsum, &total);
-            dirw[2 * tid] = pre | ((pre + c0) << 16);
-            dirw[2 * tid + 1] = (pre + c0 + c1) | ((pre + c0 + c1 + c2) << 16);
-        }
-        __syncthreads();
-        for (uint32_t i = tid; i < I.r_cnt; i += kJoinThreads) {
-            const uint4 r = R[I.r_off + i];
-            const uint64_t h = key_mix(rec_key(r));
-            if (radix_part(h, bitsR) == I.part) {
-                const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
-                const uint32_t sh = (b & 1u) * 16u;
-                const uint32_t j = (atomicAdd(&dirw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
-                tkey[j] = rec_key(r);
-                tpay[j] = make_uint2(r.z, r.w);
-            }
-        }
-        __syncthreads();
-
-        // ---- probe, software-pipelined: a record's slot is refilled with
-        // the next round's record as soon as its lookup is done
         const uint4* Sp = S + I.s_off;
-        constexpr uint32_t kSlice = 64 * kProbeItems;
-        constexpr uint32_t kRound = (kJoinThreads / 64) * kSlice;
-        uint4 nxt[kProbeItems];
-#pragma unroll
-        for (int k = 0; k < kProbeItems; ++k) {
-            const uint32_t i = wave * kSlice + k * 64 + lane;
-            if (i < I.s_cnt)
-                nxt[k] = ld_stream(Sp + i);
-        }
-        for (uint32_t r0 = 0; r0 < I.s_cnt; r0 += kRound) {
-            const uint32_t t0 = r0 + wave * kSlice;
-            uint4 s[kProbeItems];
-            uint32_t nm[kProbeItems], first[kProbeItems];
-            bool multi = false;
-#pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
-                const uint32_t i = t0 + k * 64 + lane;
-                s[k] = nxt[k];
-                nm[k] = 0;
-                first[k] = 0;
-                if (i < I.s_cnt) {
-                    const uint64_t key = rec_key(s[k]);
-                    const uint64_t h = key_mix(key);
-                    if (radix_part(h, bitsR) == I.part) {
-                        const uint32_t b = (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1);
-                        const uint32_t lo = b ? dir[b - 1] : 0u, hi = dir[b];
-                        for (uint32_t j = lo; j < hi; ++j)
-                            if (tkey[j] == key) {
-                                if (nm[k] == 0)
-                                    first[k] = j;
-                                nm[k]++;
-                            }
+
+        if (!RANGE) {
+            uint64_t* tkey = table;                                            // kTableCap
+            uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);          // kTableCap
+            uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);    // kTableBuckets / 2 + 1
+            build_buckets(
+                R, I.r_off, I.r_cnt, dirw, wsum,
+                [&](const uint4& r) -> uint32_t {
+                    const uint64_t h = key_mix(rec_key(r));
+                    return radix_part(h, bitsR) == I.part ? (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1) : ~0u;
+                },
+                [&](uint32_t j, const uint4& r) {
+                    tkey[j] = rec_key(r);
+                    tpay[j] = make_uint2(r.z, r.w);
+                });
+            const RadixTab tab{tkey, tpay, reinterpret_cast<const uint16_t*>(dirw), bitsR, bkt_shift, I.part};
+            probe_rounds(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
+        } else {
+            uint2* tpay = reinterpret_cast<uint2*>(table);
+            bool dense = I.fmt == kFmtDense;
+            if (dense) {
+                uint32_t* bits = reinterpret_cast<uint32_t*>(tpay + kDenseCap);   // presence bitmap
+                for (uint32_t w = tid; w < (I.r_cnt + 31) / 32; w += kJoinThreads)
+                    bits[w] = 0;
+                __syncthreads();
+                for (uint32_t i = tid; i < I.r_cnt; i += kJoinThreads) {
+                    const uint4 r = R[I.r_off + i];
+                    const uint64_t d = ord_key(rec_key(r)) - I.base;   // < r_cnt when the plan is right
+                    if (d >= I.r_cnt) {
+                        misc[1] = 2;
+                        continue;
                     }
+                    tpay[d] = make_uint2(r.z, r.w);
+                    const uint32_t bit = 1u << (d & 31u);
+                    if (atomicOr(&bits[d >> 5], bit) & bit)
+                        misc[1] = 1;   // a repeated key: the partition is not dense
                 }
-                multi |= nm[k] > 1;
-                if (i + kRound < I.s_cnt)
-                    nxt[k] = ld_stream(Sp + i + kRound);
+                __syncthreads();
+                if (misc[1]) {
+                    if (misc[1] == 2 || I.r_cnt > kCodeCap) {
+                        if (tid == 0)
+                            atomicAdd(fail_ctr, 1u);
+                        __syncthreads();
+                        continue;
+                    }
+                    dense = false;
+                }
             }
-            const bool wmulti = __any(multi);
-            uint64_t m[kProbeItems];
-            uint32_t mine = 0, pre = 0, wtotal = 0;
-            if (!wmulti) {
-#pragma unroll
-                for (int k = 0; k < kProbeItems; ++k) {
-                    m[k] = __ballot(nm[k] != 0);
-                    wtotal += (uint32_t)__popcll(m[k]);
-                }
+            if (dense) {
+                const DenseTab tab{tpay, I.lo, I.hi, I.base, I.r_cnt};
+                probe_rounds(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
             } else {
-#pragma unroll
-                for (int k = 0; k < kProbeItems; ++k)
-                    mine += nm[k];
-                uint32_t x = mine;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const uint32_t y = __shfl_up(x, d, 64);
-                    if (lane >= (uint32_t)d)
-                        x += y;
-                }
-                pre = x - mine;
-                wtotal = __shfl(x, 63);
-            }
-            if (lane == 0)
-                wsum[wave] = wtotal;
-            __syncthreads();
-            if (wave == 0) {
-                const uint32_t v = lane < kJoinThreads / 64 ? wsum[lane] : 0u;
-                uint32_t x = v;
-#pragma unroll
-                for (int d = 1; d < (int)(kJoinThreads / 64); d <<= 1) {
-                    const uint32_t y = __shfl_up(x, d, 64);
-                    if (lane >= (uint32_t)d)
-                        x += y;
-                }
-                const uint32_t btotal = __shfl(x, kJoinThreads / 64 - 1);
-                unsigned long long base = 0;
-                if (lane == 0 && btotal)
-                    base = atomicAdd(out_cursor, (unsigned long long)btotal);
-                base = __shfl(base, 0);
-                if (lane < kJoinThreads / 64) {
-                    const uint64_t wb = base + (x - v);
-                    misc[2 + 2 * lane] = (uint32_t)wb;
-                    misc[3 + 2 * lane] = (uint32_t)(wb >> 32);
-                }
-            }
-            __syncthreads();
-            if (!wtotal)
-                continue;
-            const uint64_t wbase = (uint64_t)misc[2 + 2 * wave] | ((uint64_t)misc[3 + 2 * wave] << 32);
-            if (wmulti) {
-                // many-to-many: a lane writes all matches of its probes
-                uint64_t row = wbase + pre;
-#pragma unroll
-                for (int k = 0; k < kProbeItems; ++k) {
-                    const uint64_t key = rec_key(s[k]);
-                    uint32_t left = nm[k];
-                    for (uint32_t j = first[k]; left; ++j)
-                        if (tkey[j] == key) {
-                            if (row < out_cap)
-                                write_row(out, row, fmt, false, tkey[j], tpay[j], s[k]);
-                            row++;
-                            left--;
-                        }
-                }
-                continue;
-            }
-            // at most one match a probe: lanes of a wave write adjacent rows
-            uint64_t row0 = wbase;
-#pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
-                if (nm[k]) {
-                    const uint64_t row = row0 + (uint64_t)__popcll(m[k] & below);
-                    if (row < out_cap)
-                        write_row(out, row, fmt, kbp, tkey[first[k]], tpay[first[k]], s[k]);
-                }
-                row0 += (uint64_t)__popcll(m[k]);
+                uint32_t* tcode = reinterpret_cast<uint32_t*>(tpay + kCodeCap);   // kCodeCap
+                uint32_t* dirw = tcode + kCodeCap;                                 // kTableBuckets / 2 + 1
+                const uint64_t base = I.base;
+                build_buckets(
+                    R, I.r_off, I.r_cnt, dirw, wsum,
+                    [&](const uint4& r) -> uint32_t { return code_bucket((uint32_t)(ord_key(rec_key(r)) - base)); },
+                    [&](uint32_t j, const uint4& r) {
+                        tcode[j] = (uint32_t)(ord_key(rec_key(r)) - base);
+                        tpay[j] = make_uint2(r.z, r.w);
+                    });
+                const CodeTab tab{tcode, tpay, reinterpret_cast<const uint16_t*>(dirw), I.lo, I.hi, I.base};
+                probe_rounds(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
             }
         }
         __syncthreads();
     }
 }
+
+// LDS bytes of the join kernel.
+constexpr size_t join_lds_bytes(bool range)
+{
+    return kMiscWords * 4 +
+           (range ? std::max<size_t>((size_t)kDenseCap * 8 + (kDenseCap + 31) / 32 * 4,
+                                     (size_t)kCodeCap * 12 + (kTableBuckets / 2 + 1) * 4)
+                  : (size_t)kTableCap * 16 + (kTableBuckets / 2 + 1) * 4);
+}
+static_assert(join_lds_bytes(false) <= kLdsBytes && join_lds_bytes(true) <= kLdsBytes, "join LDS budget");
 
 // ======================================================================
 // host side
@@ -697,6 +859,32 @@ int derive_layout(const pj_join_desc* d, Layout* L)
     return PJ_OK;
 }
 
+int DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap)
+        return PJ_OK;
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(PJ_ERR_DEVICE, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    cap = want;
+    return PJ_OK;
+}
+
+void DevBuf::release()
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
 }  // namespace pj
 
 // ======================================================================
@@ -707,38 +895,9 @@ using namespace pj;
 
 namespace {
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t bytes)
-    {
-        if (bytes <= cap)
-            return PJ_OK;
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        size_t want = bytes < 256 ? 256 : bytes;
-        hipError_t e = hipMalloc(&p, want);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(PJ_ERR_DEVICE, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
-        }
-        cap = want;
-        return PJ_OK;
-    }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
-
 // One relation partitioned on the device.
 struct Partitioned {
-    uint32_t bits = 0;
+    uint32_t F = 1;
     uint64_t n = 0;
     std::vector<uint32_t> start;   // F + 1 partition starts (host copy)
 };
@@ -761,6 +920,15 @@ struct pj_join_s {
     DevBuf rpart, spart, hist, bsum, starts, items, counters;
     Partitioned R, S;
     uint32_t bR = 0, bS = 0;  // radix bits of R and of S (bR >= bS)
+    // range partitions chosen by the learned model
+    bool range = false;
+    RangePlan plan;
+    LearnedCtx lctx;
+    DevBuf fine_bnd, coarse_bnd, pminmax, pids;
+    std::vector<uint64_t> pmin, pmax;   // per build partition, ordered keys
+    const void* raw_build = nullptr;    // device build tuples of the current run (radix re-run)
+    uint64_t raw_nbuild = 0;
+    float model_ms = 0.f;
     pj_timings last;
     hipEvent_t ev[8];
     bool ev_ok = false;
@@ -769,6 +937,7 @@ struct pj_join_s {
 namespace {
 
 constexpr uint32_t kMaxBits = 14;
+constexpr uint32_t kMaxCoarse = 2048;   // probe partitions of a range plan (write-combining fan-out)
 
 uint32_t choose_build_bits(uint64_t nr, int32_t forced)
 {
@@ -782,26 +951,25 @@ uint32_t choose_build_bits(uint64_t nr, int32_t forced)
     return kMaxBits;
 }
 
-uint32_t scatter_tile(uint32_t bits)
+uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes)
 {
-    const uint32_t F = 1u << bits;
-    const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4;
+    const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4 - bnd_bytes;
     int64_t ts = avail / 16;
     ts = (ts / kScatterThreads) * kScatterThreads;
     ts = std::min<int64_t>(ts, (int64_t)kScatterThreads * kScatterItems);
     return ts > 0 ? (uint32_t)ts : 0;
 }
 
-// Records per write-combining chunk for a fan-out of 2^bits: the largest
-// power of two (at most 16) whose buffers fit in 128 KiB of LDS; 1 selects
-// the tile-sorted scatter (no padding).
-uint32_t wc_chunk(uint32_t bits)
+// Records per write-combining chunk for a fan-out of F: the largest power of
+// two (at most 16) whose buffers fit in 128 KiB of LDS, and next to the
+// range boundaries in the LDS that is left; 1 selects the tile-sorted
+// scatter (no padding).
+uint32_t wc_chunk(uint32_t F, uint32_t bnd_bytes)
 {
-    if (bits == 0)
+    if (F <= 1)
         return 1;
-    const uint32_t F = 1u << bits;
     uint32_t B = 16;
-    while (B > 1 && (uint64_t)F * B * 16 > 131072)
+    while (B > 1 && ((uint64_t)F * B * 16 > 131072 || (uint64_t)F * B * 16 + 8ull * F + bnd_bytes > kLdsBytes))
         B >>= 1;
     return B < 2 ? 1 : B;
 }
@@ -825,26 +993,82 @@ uint64_t pad_key_nonzero_part()
             return k;
 }
 
-template <int B>
-int launch_wc(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, uint32_t bits, uint64_t chunk,
-              uint32_t nblk, const uint32_t* offs, uint4* out, hipStream_t st)
+template <int B, bool RANGE>
+int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+                uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
 {
-    const uint32_t F = 1u << bits;
-    const size_t lds = (size_t)F * B * 16 + 2 * F * 4;
-    const uint64_t pk = pad_key_nonzero_part();
-    const uint4 pad_other = make_uint4(0, 0, 0, 0);
-    const uint4 pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
-    if (ident) {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL((k_scatter_wc<true, B>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, bits, chunk,
-                           nblk, offs, out, pad_other, pad_zero);
+    const uint32_t F = pf.F;
+    const size_t lds = (size_t)F * B * 16 + 2 * F * 4 + bounds_lds_bytes(pf, RANGE);
+    uint4 pad_other, pad_zero;
+    if (RANGE) {
+        // ordered key 0 lies in partition 0, ordered key ~0 in the last one
+        pad_other = make_uint4(0, 0x80000000u, 0, 0);
+        pad_zero = make_uint4(0xFFFFFFFFu, 0x7FFFFFFFu, 0, 0);
     } else {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)lds));
-        hipLaunchKernelGGL((k_scatter_wc<false, B>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, bits, chunk,
-                           nblk, offs, out, pad_other, pad_zero);
+        const uint64_t pk = pad_key_nonzero_part();
+        pad_other = make_uint4(0, 0, 0, 0);
+        pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
     }
+    if (ident) {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B, RANGE>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_scatter_wc<true, B, RANGE>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, pf, chunk,
+                           nblk, offs, out, pad_other, pad_zero, pid);
+    } else {
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B, RANGE>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_scatter_wc<false, B, RANGE>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, pf,
+                           chunk, nblk, offs, out, pad_other, pad_zero, pid);
+    }
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
+}
+
+template <int B>
+int launch_wc(bool ident, bool range, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
+              uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
+{
+    return range ? launch_wc_t<B, true>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
+                 : launch_wc_t<B, false>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+}
+
+template <bool RANGE>
+int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+                uint32_t nblk, uint32_t pad, uint32_t* hist, unsigned long long* pmin, unsigned long long* pmax,
+                uint16_t* pid, hipStream_t st)
+{
+    const size_t lds = (size_t)pf.F * 4 + bounds_lds_bytes(pf, RANGE) + (pmin ? (size_t)pf.F * 16 : 0);
+    if (lds > kLdsBytes)
+        return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the histogram's LDS budget", pf.F);
+    const void* fn = ident ? (const void*)k_hist<true, RANGE> : (const void*)k_hist<false, RANGE>;
+    PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (ident)
+        hipLaunchKernelGGL((k_hist<true, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk, nblk,
+                           pad, hist, pmin, pmax, pid);
+    else
+        hipLaunchKernelGGL((k_hist<false, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk,
+                           nblk, pad, hist, pmin, pmax, pid);
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
+}
+
+template <bool RANGE>
+int launch_tile_scatter(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
+                        uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, hipStream_t st)
+{
+    const uint32_t bb = bounds_lds_bytes(pf, RANGE);
+    const uint32_t ts = scatter_tile(pf.F, bb);
+    if (ts == 0)
+        return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the scatter's LDS budget", pf.F);
+    const size_t lds = (size_t)ts * 16 + bb + (2 * pf.F + 1) * 4 + 16 * 4;
+    const void* fn = ident ? (const void*)k_scatter<true, RANGE> : (const void*)k_scatter<false, RANGE>;
+    PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (ident)
+        hipLaunchKernelGGL((k_scatter<true, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf, chunk,
+                           nblk, offs, out, ts);
+    else
+        hipLaunchKernelGGL((k_scatter<false, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf,
+                           chunk, nblk, offs, out, ts);
     PJ_HIP(hipGetLastError());
     return PJ_OK;
 }
@@ -860,19 +1084,24 @@ uint32_t part_blocks(uint64_t n)
 
 // Partitions raw tuples of one side into dst (canonical records, each
 // block's share of a partition padded to whole write-combining chunks).
-int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits,
-                   bool write_combine, DevBuf& dst, Partitioned& P, hipStream_t st)
+// With range = true the partition function is pf's boundaries; pminmax (2F
+// ordered keys, build side only) then receives each partition's key range,
+// and a write-combining scatter takes each tuple's partition from the
+// histogram pass through h->pids instead of searching again.
+int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
+                   bool write_combine, DevBuf& dst, Partitioned& P, unsigned long long* pminmax, hipStream_t st)
 {
     int rc;
-    P.bits = bits;
+    const uint32_t F = pf.F;
+    P.F = F;
     P.n = n;
-    const uint32_t F = 1u << bits;
     P.start.assign(F + 1, 0);
     if (n == 0)
         return PJ_OK;
     if (n >= (1ull << 31))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
-    const uint32_t B = write_combine ? wc_chunk(bits) : 1;
+    const uint32_t bb = bounds_lds_bytes(pf, range);
+    const uint32_t B = write_combine ? wc_chunk(F, bb) : 1;
     const uint32_t nblk = B > 1 ? part_blocks(n) : hist_blocks(n);
     const uint64_t chunk = (n + nblk - 1) / nblk;
     const uint64_t m = (uint64_t)F * nblk;
@@ -887,13 +1116,23 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
     uint32_t* hist = (uint32_t*)h->hist.p;
     uint32_t* bsum = (uint32_t*)h->bsum.p;
-    if (ident)
-        hipLaunchKernelGGL(k_hist<true>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, B,
-                           hist);
-    else
-        hipLaunchKernelGGL(k_hist<false>, dim3(nblk), dim3(kHistThreads), F * 4, st, in, n, L, bits, chunk, nblk, B,
-                           hist);
-    PJ_HIP(hipGetLastError());
+    unsigned long long* pmin = pminmax;
+    unsigned long long* pmax = pminmax ? pminmax + F : nullptr;
+    if (pminmax) {
+        // pmin starts at ~0 (all bytes 0xFF), pmax at 0
+        PJ_HIP(hipMemsetAsync(pmin, 0xFF, (size_t)F * 8, st));
+        PJ_HIP(hipMemsetAsync(pmax, 0, (size_t)F * 8, st));
+    }
+    uint16_t* pid = nullptr;
+    if (range && B > 1 && F <= 65536) {
+        if ((rc = h->pids.ensure(n * 2)))
+            return rc;
+        pid = (uint16_t*)h->pids.p;
+    }
+    rc = range ? launch_hist<true>(ident, in, n, L, pf, chunk, nblk, B, hist, pmin, pmax, pid, st)
+               : launch_hist<false>(ident, in, n, L, pf, chunk, nblk, B, hist, nullptr, nullptr, nullptr, st);
+    if (rc)
+        return rc;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
     hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)bsum);
@@ -902,28 +1141,13 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     PJ_HIP(hipGetLastError());
     uint4* out = (uint4*)dst.p;
     switch (B) {
-    case 16: rc = launch_wc<16>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
-    case 8: rc = launch_wc<8>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
-    case 4: rc = launch_wc<4>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
-    case 2: rc = launch_wc<2>(ident, in, n, L, bits, chunk, nblk, hist, out, st); break;
-    default: {
-        const uint32_t ts = scatter_tile(bits);
-        if (ts == 0)
-            return fail(PJ_ERR_UNSUPPORTED, "radix bits %u exceed the scatter's LDS budget", bits);
-        const size_t lds = (size_t)ts * 16 + (2 * F + 1) * 4 + 16 * 4;
-        if (ident) {
-            PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
-            hipLaunchKernelGGL(k_scatter<true>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk,
-                               nblk, hist, out, ts);
-        } else {
-            PJ_HIP(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds));
-            hipLaunchKernelGGL(k_scatter<false>, dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, bits, chunk,
-                               nblk, hist, out, ts);
-        }
-        PJ_HIP(hipGetLastError());
-    }
+    case 16: rc = launch_wc<16>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 8: rc = launch_wc<8>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 4: rc = launch_wc<4>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 2: rc = launch_wc<2>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    default:
+        rc = range ? launch_tile_scatter<true>(ident, in, n, L, pf, chunk, nblk, hist, out, st)
+                   : launch_tile_scatter<false>(ident, in, n, L, pf, chunk, nblk, hist, out, st);
     }
     if (rc)
         return rc;
@@ -945,18 +1169,22 @@ int device_count_cus()
     return cus;
 }
 
-// Builds the work list: for every build partition (a sub-partition of a
-// probe partition when R uses more radix bits than S), its records (split
+uint64_t probe_chunk(uint64_t ns)
+{
+    const uint64_t slices = (uint64_t)device_count_cus() * 16;
+    return std::max<uint64_t>((ns + slices - 1) / slices, 32768);
+}
+
+// Builds the radix work list: for every build partition (a sub-partition of
+// a probe partition when R uses more radix bits than S), its records (split
 // by position when they exceed one LDS table) times chunks of the probe
 // partition's records.
-void plan_items(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
+void plan_items_radix(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
 {
     items.clear();
     *spilled = 0;
     const uint32_t FR = 1u << h->bR, shift = h->bR - h->bS;
-    const uint64_t ns = h->S.start.back();
-    const uint64_t slices = (uint64_t)device_count_cus() * 16;
-    const uint64_t chunk = std::max<uint64_t>((ns + slices - 1) / slices, 32768);
+    const uint64_t chunk = probe_chunk(h->S.start.back());
     for (uint32_t pr = 0; pr < FR; ++pr) {
         const uint32_t ps = pr >> shift;
         const uint32_t s0 = h->S.start[ps], s1 = h->S.start[ps + 1];
@@ -970,19 +1198,83 @@ void plan_items(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
             const uint32_t ro = r0 + rs * kTableCap;
             const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
             for (uint64_t so = s0; so < s1; so += chunk) {
-                Item it;
+                Item it = {};
                 it.r_off = ro;
                 it.r_cnt = rc;
                 it.s_off = (uint32_t)so;
                 it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
                 it.part = pr;
-                it.unused = 0;
                 items.push_back(it);
             }
         }
     }
     std::stable_sort(items.begin(), items.end(),
                      [](const Item& a, const Item& b) { return a.s_cnt > b.s_cnt; });
+}
+
+// Table format of range build partition f: dense when it holds exactly
+// the keys base..base+cnt-1 (checked for repeats in the kernel), code
+// tables (split into chunks when larger than one) when its keys span less
+// than 2^32; otherwise the plan cannot be used.
+bool range_format(const pj_join_s* h, uint32_t f, uint32_t cnt, uint32_t* fmt)
+{
+    if (cnt == 0) {
+        *fmt = kFmtCodes;
+        return true;
+    }
+    const uint64_t lo = h->pmin[f], hi = h->pmax[f];
+    if (hi - lo + 1 == (uint64_t)cnt && cnt <= kDenseCap) {
+        *fmt = kFmtDense;
+        return true;
+    }
+    *fmt = kFmtCodes;
+    return hi - lo < (1ull << 32);
+}
+
+// Range work list: build partition f (inside probe partition coarse_of[f])
+// times chunks of that probe partition.
+int plan_items_range(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
+{
+    items.clear();
+    *spilled = 0;
+    const RangePlan& pl = h->plan;
+    const uint64_t chunk = probe_chunk(h->S.start.back());
+    for (uint32_t f = 0; f < pl.nfine; ++f) {
+        const uint32_t q = pl.coarse_of[f];
+        const uint32_t s0 = h->S.start[q], s1 = h->S.start[q + 1];
+        const uint32_t r0 = h->R.start[f], r1 = h->R.start[f + 1];
+        if (s1 == s0 || r1 == r0)
+            continue;
+        uint32_t fmt;
+        if (!range_format(h, f, r1 - r0, &fmt))
+            return fail(PJ_ERR_UNSUPPORTED, "range partition %u spans 2^32 keys or more", f);
+        const uint32_t cap = fmt == kFmtDense ? kDenseCap : kCodeCap;
+        const uint32_t nsplit = (r1 - r0 + cap - 1) / cap;
+        if (nsplit > 1)
+            *spilled += nsplit;
+        const uint64_t lo = f ? pl.fine[f - 1] : 0ull;
+        const uint64_t hi = f + 1 < pl.nfine ? pl.fine[f] - 1 : ~0ull;
+        for (uint32_t rs = 0; rs < nsplit; ++rs) {
+            const uint32_t ro = r0 + rs * cap;
+            const uint32_t rc = std::min<uint32_t>(cap, r1 - ro);
+            for (uint64_t so = s0; so < s1; so += chunk) {
+                Item it = {};
+                it.r_off = ro;
+                it.r_cnt = rc;
+                it.s_off = (uint32_t)so;
+                it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
+                it.part = f;
+                it.fmt = fmt;
+                it.lo = lo;
+                it.hi = hi;
+                it.base = h->pmin[f];
+                items.push_back(it);
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Item& a, const Item& b) { return a.s_cnt > b.s_cnt; });
+    return PJ_OK;
 }
 
 hipStream_t as_stream(void* s) { return (hipStream_t)s; }
@@ -1001,23 +1293,110 @@ float elapsed(pj_join_s* h, int a, int b)
     return ms;
 }
 
-int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st)
+int prepare_radix(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st)
 {
+    h->range = false;
     h->bR = choose_build_bits(nb, h->desc.radix_bits);
     h->bS = std::min<uint32_t>(h->bR, 11);
+    const PartFn pf = radix_fn(h->bR);
+    return partition_side(h, d_build, nb, h->lay.build, pf, false, false, h->rpart, h->R, nullptr, st);
+}
+
+// Range partitions of the build side from the learned model. Leaves
+// h->range false (and the radix plan to the caller) when the model declines
+// or a partition cannot be held by either range table format.
+int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st)
+{
+    h->range = false;
+    int rc;
+    hipEvent_t m0 = h->ev[6], m1 = h->ev[7];
+    if (h->ev_ok)
+        (void)hipEventRecord(m0, st);
+    if ((rc = learned_plan(h->lctx, d_build, nb, h->lay.build, kMaxCoarse, st, &h->plan)))
+        return rc;
+    if (h->ev_ok) {
+        (void)hipEventRecord(m1, st);
+        (void)hipEventSynchronize(m1);
+        (void)hipEventElapsedTime(&h->model_ms, m0, m1);
+    }
+    if (!h->plan.ok)
+        return PJ_OK;
+    const RangePlan& pl = h->plan;
+    if ((rc = h->fine_bnd.ensure(std::max<size_t>(pl.fine.size(), 1) * 8)) ||
+        (rc = h->coarse_bnd.ensure(std::max<size_t>(pl.coarse.size(), 1) * 8)) ||
+        (rc = h->pminmax.ensure((size_t)pl.nfine * 16)))
+        return rc;
+    if (!pl.fine.empty())
+        PJ_HIP(hipMemcpyAsync(h->fine_bnd.p, pl.fine.data(), pl.fine.size() * 8, hipMemcpyHostToDevice, st));
+    if (!pl.coarse.empty())
+        PJ_HIP(hipMemcpyAsync(h->coarse_bnd.p, pl.coarse.data(), pl.coarse.size() * 8, hipMemcpyHostToDevice, st));
+    const PartFn pf = range_fn(pl.nfine, (const uint64_t*)h->fine_bnd.p);
+    if ((rc = partition_side(h, d_build, nb, h->lay.build, pf, true, false, h->rpart, h->R,
+                             (unsigned long long*)h->pminmax.p, st)))
+        return rc;
+    h->pmin.resize(pl.nfine);
+    h->pmax.resize(pl.nfine);
+    PJ_HIP(hipMemcpyAsync(h->pmin.data(), h->pminmax.p, (size_t)pl.nfine * 8, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipMemcpyAsync(h->pmax.data(), (char*)h->pminmax.p + (size_t)pl.nfine * 8, (size_t)pl.nfine * 8,
+                          hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipStreamSynchronize(st));
+    for (uint32_t f = 0; f < pl.nfine; ++f) {
+        uint32_t fmt;
+        if (!range_format(h, f, h->R.start[f + 1] - h->R.start[f], &fmt)) {
+            h->plan.ok = false;
+            h->plan.why = "a build partition spans 2^32 keys or more";
+            return PJ_OK;
+        }
+    }
+    h->range = true;
+    return PJ_OK;
+}
+
+// Whether the learned partitioner should be tried: always when asked for,
+// and under "auto" when radix would have to re-read probe partitions (more
+// build than probe partitions).
+bool want_learned(const pj_join_s* h, uint64_t nb)
+{
+    if (h->desc.partitioner == PJ_PART_LEARNED)
+        return true;
+    if (h->desc.partitioner != PJ_PART_AUTO || h->desc.radix_bits >= 0)
+        return false;
+    return choose_build_bits(nb, -1) > 11;
+}
+
+int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st)
+{
+    h->model_ms = 0.f;
+    h->raw_build = d_build;
+    h->raw_nbuild = nb;
+    h->plan = RangePlan();
+    int rc;
     record(h, 0, st);
-    int rc = partition_side(h, d_build, nb, h->lay.build, h->bR, false, h->rpart, h->R, st);
+    if (want_learned(h, nb) && nb > 0) {
+        if ((rc = prepare_range(h, d_build, nb, st)))
+            return rc;
+    }
+    if (!h->range)
+        rc = prepare_radix(h, d_build, nb, st);
     record(h, 1, st);
     return rc;
 }
 
-int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
-                 hipStream_t st)
+int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
+               uint32_t* fails, hipStream_t st)
 {
     int rc;
     *out_n = 0;
+    *fails = 0;
     record(h, 2, st);
-    if ((rc = partition_side(h, d_probe, np, h->lay.probe, h->bS, true, h->spart, h->S, st)))
+    if (h->range) {
+        const PartFn pf = range_fn(h->plan.ncoarse, (const uint64_t*)h->coarse_bnd.p);
+        rc = partition_side(h, d_probe, np, h->lay.probe, pf, true, true, h->spart, h->S, nullptr, st);
+    } else {
+        const PartFn pf = radix_fn(h->bS);
+        rc = partition_side(h, d_probe, np, h->lay.probe, pf, false, true, h->spart, h->S, nullptr, st);
+    }
+    if (rc)
         return rc;
     record(h, 3, st);
     PJ_HIP(hipStreamSynchronize(st));  // partition starts are on the host now
@@ -1027,7 +1406,12 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
     }
     std::vector<Item> items;
     uint64_t spilled = 0;
-    plan_items(h, items, &spilled);
+    if (h->range) {
+        if ((rc = plan_items_range(h, items, &spilled)))
+            return rc;
+    } else {
+        plan_items_radix(h, items, &spilled);
+    }
     const uint32_t nitems = (uint32_t)items.size();
     if ((rc = h->items.ensure(std::max<size_t>(nitems, 1) * sizeof(Item))) || (rc = h->counters.ensure(64)))
         return rc;
@@ -1036,35 +1420,65 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
     PJ_HIP(hipMemsetAsync(h->counters.p, 0, 64, st));
     uint32_t* item_ctr = (uint32_t*)h->counters.p;
     unsigned long long* cursor = (unsigned long long*)((char*)h->counters.p + 8);
-    const size_t lds = (size_t)kTableCap * 16 + (kTableBuckets / 2 + 1) * 4 + 17 * 4 + 34 * 4;
-    PJ_HIP(hipFuncSetAttribute((const void*)k_join, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    uint32_t* fail_ctr = (uint32_t*)((char*)h->counters.p + 16);
+    const size_t lds = join_lds_bytes(h->range);
+    const void* kfn = h->range ? (const void*)k_join<true> : (const void*)k_join<false>;
+    PJ_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const uint32_t bkt_shift = 64 - h->bR - 12;
     record(h, 4, st);
     if (nitems) {
         const uint32_t grid = std::min<uint32_t>(nitems, (uint32_t)device_count_cus());
-        hipLaunchKernelGGL(k_join, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr, (uint32_t*)d_out,
-                           out_cap, cursor, h->lay.out, h->bR, bkt_shift);
+        if (h->range)
+            hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                               (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr,
+                               (uint32_t*)d_out, out_cap, cursor, h->lay.out, 0u, 0u, fail_ctr);
+        else
+            hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                               (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr,
+                               (uint32_t*)d_out, out_cap, cursor, h->lay.out, h->bR, bkt_shift, fail_ctr);
         PJ_HIP(hipGetLastError());
     }
     record(h, 5, st);
-    unsigned long long produced = 0;
-    PJ_HIP(hipMemcpyAsync(&produced, cursor, 8, hipMemcpyDeviceToHost, st));
+    unsigned long long produced[3] = {0, 0, 0};
+    PJ_HIP(hipMemcpyAsync(produced, h->counters.p, 24, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
-    *out_n = produced;
+    *out_n = produced[1];
+    *fails = (uint32_t)produced[2];
     pj_timings& t = h->last;
-    t.build_partition_ms = elapsed(h, 0, 1);
+    t.build_partition_ms = elapsed(h, 0, 1) - h->model_ms;
     t.probe_partition_ms = elapsed(h, 2, 3);
     t.join_ms = elapsed(h, 4, 5);
-    t.model_ms = 0.f;
+    t.model_ms = h->model_ms;
     t.total_ms = elapsed(h, 0, 5);
-    t.partitions = 1u << h->bS;
-    t.subpartitions = 1u << (h->bR - h->bS);
+    t.partitions = h->range ? h->plan.ncoarse : 1u << h->bS;
+    t.subpartitions = h->range ? (h->plan.nfine + h->plan.ncoarse - 1) / h->plan.ncoarse : 1u << (h->bR - h->bS);
     t.work_items = nitems;
-    t.used_learned = 0;
+    t.used_learned = h->range ? 1u : 0u;
     t.spilled_items = spilled;
-    if (produced > out_cap)
-        return fail(PJ_ERR_CAPACITY, "join produced %llu tuples, output holds %llu", produced,
+    return PJ_OK;
+}
+
+// The probe phase. When a range plan's kernel met a partition it could not
+// hold (repeated keys in a partition too large for a code table), the build
+// side is partitioned again with radix bits and the probe phase repeats.
+int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
+                 hipStream_t st)
+{
+    uint32_t fails = 0;
+    int rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
+    if (rc == PJ_OK && fails) {
+        if (!h->raw_build)
+            return fail(PJ_ERR_STATE, "range plan failed and the build tuples are gone");
+        h->plan.ok = false;
+        h->plan.why = "repeated keys in a dense-looking partition";
+        if ((rc = prepare_radix(h, h->raw_build, h->raw_nbuild, st)))
+            return rc;
+        rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
+    }
+    if (rc)
+        return rc;
+    if (*out_n > out_cap)
+        return fail(PJ_ERR_CAPACITY, "join produced %llu tuples, output holds %llu", (unsigned long long)*out_n,
                     (unsigned long long)out_cap);
     return PJ_OK;
 }

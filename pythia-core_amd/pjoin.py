@@ -46,6 +46,21 @@ class Timings(C.Structure):
                 ("spilled_items", C.c_uint64)]
 
 
+PJ_MODEL_LEAVES = 4096
+PJ_MODEL_MAX_SEGS = 1024
+
+
+class Model(C.Structure):
+    """pj_model: the learned partitioner's two-stage piecewise-linear CDF."""
+    _fields_ = [("nleaves", C.c_uint32), ("nseg", C.c_uint32), ("n", C.c_uint64), ("n_keys", C.c_uint64),
+                ("window", C.c_uint32), ("iters", C.c_uint32), ("lam", C.c_float), ("loss_lsq", C.c_float),
+                ("loss", C.c_float), ("reserved", C.c_uint32),
+                ("seg_lo", C.c_uint64 * PJ_MODEL_MAX_SEGS), ("seg_hi", C.c_uint64 * PJ_MODEL_MAX_SEGS),
+                ("seg_lw", C.c_uint64 * PJ_MODEL_MAX_SEGS), ("seg_leaf0", C.c_uint32 * (PJ_MODEL_MAX_SEGS + 1)),
+                ("leaf_seg", C.c_uint16 * PJ_MODEL_LEAVES), ("leaf_start", C.c_uint32 * (PJ_MODEL_LEAVES + 1)),
+                ("a", C.c_float * PJ_MODEL_LEAVES), ("b", C.c_float * PJ_MODEL_LEAVES)]
+
+
 class Cursor(C.Structure):
     _fields_ = [("impl", C.c_void_p), ("pending", C.c_uint64)]
 
@@ -94,6 +109,13 @@ _SIGS = {
     "pj_cluster_table_size": (C.c_uint64, []),
     "pj_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),
+    "pj_model_size": (C.c_uint64, []),
+    "pj_sort_u64": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pj_model_fit": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(Model), C.c_void_p]),
+    "pj_model_grad": (C.c_int, [C.POINTER(Model), C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double),
+                                C.c_void_p]),
+    "pj_model_bounds": (C.c_int, [C.POINTER(Model), C.c_uint32, C.c_void_p, C.c_uint32,
+                                  C.POINTER(C.c_uint32)]),
 }
 
 # Entry points added by later components; bound when the library has them.
@@ -230,6 +252,37 @@ def zipf_cdf(n, z):
     cdf = np.zeros(n, np.float64)
     check(lib().pj_zipf_cdf(n, z, cdf.ctypes.data_as(C.c_void_p)))
     return cdf
+
+
+def model_fit(d_keys, n, n_keys, iters, stream=None):
+    """Fits the learned partitioner's model to n ordered keys in device memory."""
+    m = Model()
+    check(lib().pj_model_fit(C.c_void_p(d_keys), n, n_keys, iters, C.byref(m), C.c_void_p(stream)))
+    return m
+
+
+def model_grad(model, d_sorted, n, stream=None):
+    """(loss, gradient) of the training loss at the model's parameters; the
+    gradient is [dL/da[0..L), dL/db[0..L)]."""
+    import numpy as np
+    g = np.zeros(2 * model.nleaves, np.float32)
+    loss = C.c_double()
+    check(lib().pj_model_grad(C.byref(model), C.c_void_p(d_sorted), n, g.ctypes.data_as(C.c_void_p),
+                              C.byref(loss), C.c_void_p(stream)))
+    return loss.value, g
+
+
+def model_bounds(model, max_coarse=2048):
+    import numpy as np
+    cap = 2 * max_coarse * 4 + PJ_MODEL_MAX_SEGS
+    b = np.zeros(cap, np.uint64)
+    nb = C.c_uint32()
+    check(lib().pj_model_bounds(C.byref(model), max_coarse, b.ctypes.data_as(C.c_void_p), cap, C.byref(nb)))
+    return b[: nb.value].copy()
+
+
+def sort_u64(d_keys, n, stream=None):
+    check(lib().pj_sort_u64(C.c_void_p(d_keys), n, C.c_void_p(stream)))
 
 
 def cluster_table(seed, n_keys, nclusters=256, log2_wmin=10, log2_wmax=24, weight_z=1.2):

@@ -357,8 +357,7 @@ def _clustered(seed, n, m, z=1.0, **kw):
     return t, cdf, db, dp
 
 
-_LEARNED_PENDING = pytest.mark.skip(reason="learned partitioner not built yet")
-_PARTS = ["radix", pytest.param("learned", marks=_LEARNED_PENDING)]
+_PARTS = ["radix", "learned"]
 
 
 @pytest.mark.parametrize("partitioner", _PARTS)

@@ -76,15 +76,24 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
             uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
+            r[k] = make_uint4(0, 0, 0, 0);
             if (i < end)
                 r[k] = load_rec<IDENT>(in, i, L);
+        }
+        uint32_t part[kHistItems];
+        if (RANGE) {   // all records' searches advance together
+            uint64_t u[kHistItems];
+#pragma unroll
+            for (int k = 0; k < kHistItems; ++k)
+                u[k] = ord_key(rec_key(r[k]));   // records past the end search harmlessly
+            range_part_multi<kHistItems>(bnd, F, pf.p2, u, part);
         }
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
             uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
             if (i < end) {
                 const uint64_t key = rec_key(r[k]);
-                const uint32_t p = part_of<RANGE>(key, pf, bnd);
+                const uint32_t p = RANGE ? part[k] : part_of<false>(key, pf, bnd);
                 atomicAdd(&cnt[p], 1u);
                 if (RANGE && pid)
                     pid[i] = (uint16_t)p;
@@ -234,20 +243,25 @@ constexpr int kScatterItems = 16;
 // Scatter with an LDS-sorted tile, for fan-outs too wide for one LDS buffer
 // per partition: each tile is counting-sorted by partition in LDS, then
 // written so that lanes of a wave store consecutive records of a partition.
+// Range partitions keep each tile record's partition in LDS (tpart) and take
+// it from the histogram pass (pid) when given.
 template <bool IDENT, bool RANGE>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
                                                               SideLayout L, PartFn pf, uint64_t chunk,
                                                               uint32_t nblk, const uint32_t* __restrict__ offs,
-                                                              uint4* __restrict__ out, uint32_t ts)
+                                                              uint4* __restrict__ out, uint32_t ts,
+                                                              const uint16_t* __restrict__ pid)
 {
     extern __shared__ uint4 sm4[];
     const uint32_t F = pf.F;
     uint4* tile = sm4;
-    uint64_t* bnd = reinterpret_cast<uint64_t*>(tile + ts);                   // F - 1 (range)
+    uint64_t* bnd = reinterpret_cast<uint64_t*>(tile + ts);                       // p2 - 1 (range)
     uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));   // F + 1
-    uint32_t* goff = cnt + F + 1;                                             // F
-    uint32_t* wsum = goff + F;                                                // 9
-    load_bounds<RANGE>(bnd, pf);
+    uint32_t* goff = cnt + F + 1;                                                 // F
+    uint32_t* wsum = goff + F;                                                    // 16
+    uint16_t* tpart = reinterpret_cast<uint16_t*>(wsum + 16);                     // ts (range)
+    if (!(RANGE && pid))
+        load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
         goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
@@ -262,14 +276,18 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
 #pragma unroll
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
-            if (i < tn)
+            if (i < tn) {
                 r[k] = load_rec<IDENT>(in, t0 + i, L);
+                if (RANGE && pid)
+                    part[k] = pid[t0 + i];
+            }
         }
 #pragma unroll
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
             if (i < tn) {
-                part[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
+                if (!(RANGE && pid))
+                    part[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
                 rank[k] = atomicAdd(&cnt[part[k]], 1u);
             }
         }
@@ -278,13 +296,17 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
 #pragma unroll
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
-            if (i < tn)
-                tile[cnt[part[k]] + rank[k]] = r[k];
+            if (i < tn) {
+                const uint32_t slot = cnt[part[k]] + rank[k];
+                tile[slot] = r[k];
+                if (RANGE)
+                    tpart[slot] = (uint16_t)part[k];
+            }
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < tn; i += kScatterThreads) {
             uint4 v = tile[i];
-            uint32_t p = part_of<RANGE>(rec_key(v), pf, bnd);
+            const uint32_t p = RANGE ? (uint32_t)tpart[i] : part_of<false>(rec_key(v), pf, bnd);
             out[goff[p] + (i - cnt[p])] = v;
         }
         __syncthreads();
@@ -951,10 +973,10 @@ uint32_t choose_build_bits(uint64_t nr, int32_t forced)
     return kMaxBits;
 }
 
-uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes)
+uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes, bool range)
 {
     const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4 - bnd_bytes;
-    int64_t ts = avail / 16;
+    int64_t ts = avail / (range ? 18 : 16);
     ts = (ts / kScatterThreads) * kScatterThreads;
     ts = std::min<int64_t>(ts, (int64_t)kScatterThreads * kScatterItems);
     return ts > 0 ? (uint32_t)ts : 0;
@@ -1054,21 +1076,22 @@ int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
 
 template <bool RANGE>
 int launch_tile_scatter(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
-                        uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, hipStream_t st)
+                        uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
+                        hipStream_t st)
 {
     const uint32_t bb = bounds_lds_bytes(pf, RANGE);
-    const uint32_t ts = scatter_tile(pf.F, bb);
+    const uint32_t ts = scatter_tile(pf.F, bb, RANGE);
     if (ts == 0)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the scatter's LDS budget", pf.F);
-    const size_t lds = (size_t)ts * 16 + bb + (2 * pf.F + 1) * 4 + 16 * 4;
+    const size_t lds = (size_t)ts * (RANGE ? 18 : 16) + bb + (2 * pf.F + 1) * 4 + 16 * 4;
     const void* fn = ident ? (const void*)k_scatter<true, RANGE> : (const void*)k_scatter<false, RANGE>;
     PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (ident)
         hipLaunchKernelGGL((k_scatter<true, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf, chunk,
-                           nblk, offs, out, ts);
+                           nblk, offs, out, ts, pid);
     else
         hipLaunchKernelGGL((k_scatter<false, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf,
-                           chunk, nblk, offs, out, ts);
+                           chunk, nblk, offs, out, ts, pid);
     PJ_HIP(hipGetLastError());
     return PJ_OK;
 }
@@ -1124,7 +1147,7 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
         PJ_HIP(hipMemsetAsync(pmax, 0, (size_t)F * 8, st));
     }
     uint16_t* pid = nullptr;
-    if (range && B > 1 && F <= 65536) {
+    if (range && F <= 65536) {
         if ((rc = h->pids.ensure(n * 2)))
             return rc;
         pid = (uint16_t*)h->pids.p;
@@ -1146,8 +1169,8 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     case 4: rc = launch_wc<4>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
     case 2: rc = launch_wc<2>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
     default:
-        rc = range ? launch_tile_scatter<true>(ident, in, n, L, pf, chunk, nblk, hist, out, st)
-                   : launch_tile_scatter<false>(ident, in, n, L, pf, chunk, nblk, hist, out, st);
+        rc = range ? launch_tile_scatter<true>(ident, in, n, L, pf, chunk, nblk, hist, out, pid, st)
+                   : launch_tile_scatter<false>(ident, in, n, L, pf, chunk, nblk, hist, out, nullptr, st);
     }
     if (rc)
         return rc;

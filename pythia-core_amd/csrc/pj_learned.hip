@@ -41,7 +41,7 @@ constexpr uint64_t kGap = 1ull << 32;      // segment cut
 constexpr uint32_t kWindow = 16;           // least-squares window, leaves either side
 constexpr uint64_t kLeafSamples = 128;     // samples per leaf the root aims for (up to 4096 leaves)
 constexpr float kLambda = 64.f;            // continuity weight (in samples)
-constexpr uint32_t kSteps = 16;            // training steps in a join
+constexpr uint32_t kSteps = 8;             // training steps in a join
 constexpr float kEta = 0.8f;               // step size of the Newton steps
 constexpr double kFill = 0.85;             // planned fill of an LDS table
 constexpr double kFillMax = 0.92;          // highest fill accepted to save a probe re-read
@@ -240,17 +240,20 @@ struct Moments {   // centred moments of (x, y) over a set of samples
 };
 
 // Per leaf, in its own frame (x in [0,1), y = rank - leaf_start): N, means,
-// centred sums of squares and products (two passes for accuracy).
+// centred sums of squares and products (two passes for accuracy). Also
+// stores every sample's x (own frame) in xs for the training passes.
 __global__ __launch_bounds__(kFitThreads) void k_leaf_moments(const pj_model* __restrict__ M,
                                                                const uint64_t* __restrict__ keys,
-                                                               Moments* __restrict__ mom)
+                                                               Moments* __restrict__ mom, float* __restrict__ xs)
 {
     __shared__ float red[kFitThreads / 64];
     const uint32_t j = blockIdx.x;
     const uint32_t s0 = M->leaf_start[j], s1 = M->leaf_start[j + 1];
     float sx = 0.f, sy = 0.f;
     for (uint32_t i = s0 + threadIdx.x; i < s1; i += kFitThreads) {
-        sx += leaf_x(M, j, keys[i]);
+        const float x = leaf_x(M, j, keys[i]);
+        xs[i] = x;
+        sx += x;
         sy += (float)(i - s0);
     }
     const float n = (float)(s1 - s0);
@@ -259,7 +262,7 @@ __global__ __launch_bounds__(kFitThreads) void k_leaf_moments(const pj_model* __
     const float mx = n > 0.f ? sx / n : 0.f, my = n > 0.f ? sy / n : 0.f;
     float cxx = 0.f, cxy = 0.f;
     for (uint32_t i = s0 + threadIdx.x; i < s1; i += kFitThreads) {
-        const float dx = leaf_x(M, j, keys[i]) - mx, dy = (float)(i - s0) - my;
+        const float dx = xs[i] - mx, dy = (float)(i - s0) - my;
         cxx += dx * dx;
         cxy += dx * dy;
     }
@@ -330,9 +333,10 @@ struct Resid {   // sums of r, r x and r^2 over leaf j's window
     float r, rx, rr;
 };
 
-// Forward: residuals of leaf j's line over its window, reduced per leaf.
+// Forward: residuals of leaf j's line over its window, reduced per leaf
+// (xs: every sample's x in its own leaf's frame).
 __global__ __launch_bounds__(kFitThreads) void k_gd_forward(const pj_model* __restrict__ M,
-                                                             const uint64_t* __restrict__ keys,
+                                                             const float* __restrict__ xs,
                                                              const float2* __restrict__ ab, Resid* __restrict__ res)
 {
     __shared__ float red[kFitThreads / 64];
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(kFitThreads) void k_gd_forward(const pj_model* __re
         const float shift = (float)((int32_t)m - (int32_t)j);
         const uint32_t s0 = M->leaf_start[m], s1 = M->leaf_start[m + 1];
         for (uint32_t i = s0 + threadIdx.x; i < s1; i += kFitThreads) {
-            const float x = leaf_x(M, m, keys[i]) + shift;
+            const float x = xs[i] + shift;
             const float y = (float)((int64_t)i - (int64_t)sj);
             const float r = p.x + p.y * x - y;
             r1 += r;
@@ -434,6 +438,7 @@ struct FitBufs {
     uint64_t* seg_lo;
     uint64_t* seg_hi;
     uint32_t* seg_start;
+    float* xs;          // n: each sample's x in its own leaf
 };
 
 int scan_u32(uint32_t* a, uint64_t m, uint32_t* bsum, hipStream_t st)
@@ -480,7 +485,7 @@ int alloc_fit(LearnedCtx& c, uint64_t n, FitBufs* f)
 {
     int rc;
     const uint64_t sw = sort_scratch_words(n);
-    if ((rc = c.sample.ensure(n * 8 + 64)) || (rc = c.tmp.ensure(n * 8 + 64)) || (rc = c.hist.ensure(sw * 4)) ||
+    if ((rc = c.sample.ensure(n * 8 + 64)) || (rc = c.tmp.ensure(n * 12 + 64)) || (rc = c.hist.ensure(sw * 4)) ||
         (rc = c.bsum.ensure((scan_blocks(sw) + 2) * 4)) || (rc = c.model.ensure(sizeof(pj_model))) ||
         (rc = c.params.ensure(2 * kLeaves * sizeof(float2))) ||
         (rc = c.moments.ensure(kLeaves * (sizeof(Moments) + sizeof(LeafStat)))) ||
@@ -489,6 +494,7 @@ int alloc_fit(LearnedCtx& c, uint64_t n, FitBufs* f)
         return rc;
     f->sorted = (uint64_t*)c.sample.p;
     f->alt = (uint64_t*)c.tmp.p;
+    f->xs = (float*)(f->alt + n);   // after the sort's ping-pong half
     f->scan = (uint32_t*)c.hist.p;
     f->bsum = (uint32_t*)c.bsum.p;
     f->dm = (pj_model*)c.model.p;
@@ -583,7 +589,7 @@ int build_root(const FitBufs& f, uint64_t n, uint64_t n_keys, pj_model* M, bool*
 // stepped parameters to ab[cur ^ 1].
 int train_step(const FitBufs& f, uint32_t L, int cur, float eta, float* grad, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gd_forward, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.sorted, f.ab[cur], f.res);
+    hipLaunchKernelGGL(k_gd_forward, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.xs, f.ab[cur], f.res);
     hipLaunchKernelGGL(k_gd_backward, dim3(grid_for(L, 256)), dim3(256), 0, st, f.dm, f.ab[cur], f.res, f.stat,
                        kLambda, eta, f.ab[cur ^ 1], grad, f.loss);
     PJ_HIP(hipGetLastError());
@@ -601,7 +607,7 @@ int fit_model(const FitBufs& f, uint64_t n, uint64_t n_keys, uint32_t steps, pj_
     if ((rc = build_root(f, n, n_keys, M, ok, st)) || !*ok)
         return rc;
     const uint32_t L = M->nleaves;
-    hipLaunchKernelGGL(k_leaf_moments, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.sorted, f.mom);
+    hipLaunchKernelGGL(k_leaf_moments, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.sorted, f.mom, f.xs);
     hipLaunchKernelGGL(k_leaf_fit, dim3(grid_for(L, 256)), dim3(256), 0, st, f.dm, f.mom, f.ab[0], f.stat);
     PJ_HIP(hipGetLastError());
     double losses[2] = {0.0, 0.0};
@@ -825,6 +831,7 @@ int pj_model_grad(const pj_model* model, const uint64_t* d_sorted, uint64_t n, f
         ab[j] = make_float2(model->a[j], model->b[j]);
     PJ_HIP(hipMemcpyAsync(f.ab[0], ab.data(), L * sizeof(float2), hipMemcpyHostToDevice, st));
     PJ_HIP(hipMemsetAsync(f.loss, 0, 8, st));
+    hipLaunchKernelGGL(k_leaf_moments, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.sorted, f.mom, f.xs);
     if ((rc = train_step(f, L, 0, 0.f, f.grad, st)))
         return rc;
     PJ_HIP(hipMemcpyAsync(grad, f.grad, 2 * L * sizeof(float), hipMemcpyDeviceToHost, st));

@@ -260,6 +260,57 @@ int pj_model_grad(const pj_model* model, const uint64_t* d_sorted, uint64_t n, f
 int pj_model_bounds(const pj_model* model, uint32_t max_coarse, uint64_t* bounds, uint32_t cap,
                     uint32_t* nbounds);
 
+/* ---------------------------------------------------------------- batch
+ * Many independent joins of one descriptor, run as one captured pipeline
+ * (one hipGraph launch: batched partition passes, a device-side work list
+ * and one persistent join kernel). No reference counterpart: the reference
+ * runs one HashJoinOp per plan (query.cpp:301-334); each join of a batch
+ * gives the multiset of rows that operator gives for its two inputs. */
+typedef struct pj_batch_s* pj_batch_t;
+typedef struct {
+    const void* build;   /* device tuples of the build side */
+    uint64_t nbuild;
+    const void* probe;   /* device tuples of the probe side */
+    uint64_t nprobe;
+    void* out;           /* device rows, 16-byte aligned */
+    uint64_t out_cap;    /* rows out holds */
+} pj_batch_join;
+/* Per size class (floor(log2(build tuples))): radix bits 4..12, passes 1 or
+ * 2, and the LDS open-addressing tables' load factor 0.5..0.9. */
+typedef struct {
+    uint8_t bits;
+    uint8_t passes;
+    uint16_t reserved;
+    float load_factor;
+} pj_batch_cfg;
+
+int pj_batch_create(const pj_join_desc* desc, pj_batch_t* out);
+int pj_batch_destroy(pj_batch_t h);
+int pj_batch_set_config(pj_batch_t h, uint32_t size_class, const pj_batch_cfg* cfg);
+int pj_batch_get_config(pj_batch_t h, uint32_t size_class, pj_batch_cfg* cfg);
+/* Times the candidate configurations of every size class present on up to
+ * `trial` joins of that class and keeps the fastest (pj_batch_prepare must
+ * follow). */
+int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uint32_t trial, void* stream);
+/* Plans the batch and captures its pipeline into a graph. */
+int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, void* stream);
+/* Runs the prepared batch on stream: one graph launch (use_graph != 0) or
+ * the same kernels launched one by one. */
+int pj_batch_run(pj_batch_t h, int use_graph, void* stream);
+/* After a run: rows each join produced, and (done_us, optional) the time
+ * from the batch's start until each join's last work item finished, from
+ * the device clock. PJ_ERR_CAPACITY when a join's output overflowed. */
+int pj_batch_results(pj_batch_t h, uint64_t* out_counts, double* done_us, void* stream);
+
+/* Batch workload: join j of a batch of njoins has build tuples (int64 key,
+ * int32 payload) with unique keys (a seeded permutation of a uniform
+ * 2^40 sample) and 4x as many probe tuples of the same shape whose key is a
+ * uniformly drawn build key; build sizes are log-uniform in [nmin, nmax].
+ * pj_batch_sizes fills the sizes; pj_gen_batch_join writes join j's tuples. */
+int pj_batch_sizes(uint64_t seed, uint32_t njoins, uint64_t nmin, uint64_t nmax, uint64_t* nbuild);
+int pj_gen_batch_join(uint64_t seed, uint32_t j, uint64_t nbuild, uint64_t nprobe, void* build_dst,
+                      void* probe_dst, int on_device, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

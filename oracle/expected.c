@@ -70,6 +70,59 @@ int or_expected_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t 
     return 0;
 }
 
+/* Join j of the batch workload: rows (key, build payload32, probe
+ * payload32), 16 bytes. */
+typedef struct {
+    pj_pkfk_params p;
+    uint64_t b, e, sum, cnt;
+} batch_job;
+
+static void* batch_worker(void* arg)
+{
+    batch_job* j = (batch_job*)arg;
+    uint64_t s = 0, c = 0;
+    for (uint64_t i = j->b; i < j->e; ++i) {
+        uint64_t k, pp, bk, bp;
+        uint64_t idx = pj_pkfk_probe(&j->p, i, &k, &pp);
+        if (idx == ~0ull)
+            continue;
+        pj_pkfk_build(&j->p, idx, &bk, &bp);
+        uint64_t t[2] = {k, (bp & 0xFFFFFFFFull) | (pp << 32)};
+        s += or_tuple_mix(t, 16);
+        c++;
+    }
+    j->sum = s;
+    j->cnt = c;
+    return NULL;
+}
+
+int or_expected_batch_join(uint64_t seed, uint32_t jidx, uint64_t n_build, uint64_t n_probe, int threads,
+                           uint64_t* sum, uint64_t* count)
+{
+    pj_pkfk_params p = {pj_batch_seed(seed, jidx), 1ull << 40, n_build, n_probe, 0};
+    if (threads < 1)
+        threads = 1;
+    if (threads > 64)
+        threads = 64;
+    pthread_t th[64];
+    batch_job jobs[64];
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].p = p;
+        jobs[t].b = n_probe * (uint64_t)t / (uint64_t)threads;
+        jobs[t].e = n_probe * (uint64_t)(t + 1) / (uint64_t)threads;
+        pthread_create(&th[t], NULL, batch_worker, &jobs[t]);
+    }
+    uint64_t s = 0, c = 0;
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(th[t], NULL);
+        s += jobs[t].sum;
+        c += jobs[t].cnt;
+    }
+    *sum = s;
+    *count = c;
+    return 0;
+}
+
 /* Host generation of the same relations (for the CPU-side join runs). */
 int or_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_probe, uint64_t n_miss,
                 void* build, void* probe)

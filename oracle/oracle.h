@@ -157,6 +157,11 @@ int or_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_pro
 /* Probe tuples [j0, j1) of the full probe relation of n_probe tuples. */
 int or_gen_pkfk_probe_range(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_probe,
                             uint64_t n_miss, uint64_t j0, uint64_t j1, void* probe);
+/* Join jidx of the batch workload (seeded as pj_gen_batch_join): expected
+ * row count and checksum of rows (key, build payload32, probe payload32). */
+int or_expected_batch_join(uint64_t seed, uint32_t jidx, uint64_t n_build, uint64_t n_probe, int threads,
+                           uint64_t* sum, uint64_t* count);
+
 /* The composite-key Zipf workload: expected output count and checksum of
  * projection [B$0, B$1, B$2, P$2], and host generation of the relations. */
 int or_expected_pairs_zipf(uint64_t seed, uint64_t n_build, uint64_t n_probe, const double* cdf,

@@ -88,6 +88,8 @@ def lib():
             "or_zipf_ranks_guided": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
             "or_gen_pkfk_probe_range": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                                   C.c_uint64, C.c_uint64, C.c_void_p]),
+            "or_expected_batch_join": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
+                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -205,6 +207,13 @@ def checksum(rows, width):
 def expected_pkfk(seed, domain, n_build, n_probe, n_miss, threads=8):
     s, c = C.c_uint64(), C.c_uint64()
     lib().or_expected_pkfk(seed, domain, n_build, n_probe, n_miss, threads, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def expected_batch_join(seed, j, n_build, n_probe, threads=8):
+    """(checksum, count) of join j of the batch workload."""
+    s, c = C.c_uint64(), C.c_uint64()
+    lib().or_expected_batch_join(seed, j, n_build, n_probe, threads, C.byref(s), C.byref(c))
     return s.value, c.value
 
 

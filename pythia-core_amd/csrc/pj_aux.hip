@@ -87,6 +87,26 @@ __global__ void k_gen_pkfk(pj_pkfk_params p, uint64_t* __restrict__ build, uint6
     }
 }
 
+// Batch join tuples: (int64 key, int32 payload), 12 bytes packed.
+__global__ void k_gen_batch(pj_pkfk_params p, uint32_t* __restrict__ build, uint32_t* __restrict__ probe)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n_build + p.n_probe; i += stride) {
+        uint64_t k, v;
+        uint32_t* t;
+        if (i < p.n_build) {
+            pj_pkfk_build(&p, i, &k, &v);
+            t = build + 3 * i;
+        } else {
+            pj_pkfk_probe(&p, i - p.n_build, &k, &v);
+            t = probe + 3 * (i - p.n_build);
+        }
+        t[0] = (uint32_t)k;
+        t[1] = (uint32_t)(k >> 32);
+        t[2] = (uint32_t)v;
+    }
+}
+
 struct PairTuple {   // (int32 a, int32 b, int64 payload), 16 bytes packed
     uint32_t a, b;
     uint64_t pay;
@@ -290,6 +310,54 @@ int pj_gen_pkfk(uint64_t seed, uint64_t domain, uint64_t n_build, uint64_t n_pro
             q[2 * (i - n_build)] = k;
             q[2 * (i - n_build) + 1] = v;
         }
+    });
+    return PJ_OK;
+}
+
+int pj_batch_sizes(uint64_t seed, uint32_t njoins, uint64_t nmin, uint64_t nmax, uint64_t* nbuild)
+{
+    if (!nbuild || nmin == 0 || nmax < nmin)
+        return fail(PJ_ERR_INVALID, "need 0 < nmin <= nmax and an output array");
+    const double a = std::log((double)nmin), b = std::log((double)nmax);
+    for (uint32_t j = 0; j < njoins; ++j) {
+        const double u = (double)(pj_rand64(seed, 7, j) >> 11) * (1.0 / 9007199254740992.0);   // [0, 1)
+        uint64_t n = (uint64_t)std::floor(std::exp(a + u * (b - a)));
+        nbuild[j] = std::min(nmax, std::max(nmin, n));
+    }
+    return PJ_OK;
+}
+
+int pj_gen_batch_join(uint64_t seed, uint32_t j, uint64_t nbuild, uint64_t nprobe, void* build_dst, void* probe_dst,
+                      int on_device, void* stream)
+{
+    if ((nbuild && !build_dst) || (nprobe && !probe_dst) || (nprobe && !nbuild))
+        return fail(PJ_ERR_INVALID, "null argument or probe tuples without build tuples");
+    if (nbuild > (1ull << 40))
+        return fail(PJ_ERR_INVALID, "at most 2^40 build tuples");
+    pj_pkfk_params p = {pj_batch_seed(seed, j), 1ull << 40, nbuild, nprobe, 0};
+    if (on_device) {
+        hipStream_t st = (hipStream_t)stream;
+        if (nbuild + nprobe) {
+            hipLaunchKernelGGL(k_gen_batch, dim3(1024), dim3(256), 0, st, p, (uint32_t*)build_dst, (uint32_t*)probe_dst);
+            PJ_HIP(hipGetLastError());
+        }
+        return PJ_OK;
+    }
+    uint32_t* b = (uint32_t*)build_dst;
+    uint32_t* q = (uint32_t*)probe_dst;
+    host_parallel(nbuild + nprobe, [&](uint64_t i) {
+        uint64_t k, v;
+        uint32_t* t;
+        if (i < nbuild) {
+            pj_pkfk_build(&p, i, &k, &v);
+            t = b + 3 * i;
+        } else {
+            pj_pkfk_probe(&p, i - nbuild, &k, &v);
+            t = q + 3 * (i - nbuild);
+        }
+        t[0] = (uint32_t)k;
+        t[1] = (uint32_t)(k >> 32);
+        t[2] = (uint32_t)v;
     });
     return PJ_OK;
 }

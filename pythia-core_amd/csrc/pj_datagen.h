@@ -112,6 +112,14 @@ PJ_HD uint64_t pj_pkfk_probe(const pj_pkfk_params* p, uint64_t j, uint64_t* key,
     return idx;
 }
 
+// Batch workload: join j of a batch is the PK-FK workload above with seed
+// pj_batch_seed(seed, j), domain 2^40, no misses, and 32-bit payloads (the
+// low words of the 64-bit ones).
+PJ_HD uint64_t pj_batch_seed(uint64_t seed, uint32_t j)
+{
+    return pj_fin64(seed ^ (0xba7c4ull + (uint64_t)j * 0x9e3779b97f4a7c15ull));
+}
+
 // ---------------------------------------------------------------------------
 // Zipf foreign keys over build ranks (generator_zipf semantics,
 // operators/generator_zipf.cpp:73-152): u is a 62-bit uniform / 2^62,

@@ -61,6 +61,15 @@ class Model(C.Structure):
                 ("a", C.c_float * PJ_MODEL_LEAVES), ("b", C.c_float * PJ_MODEL_LEAVES)]
 
 
+class BatchJoin(C.Structure):
+    _fields_ = [("build", C.c_void_p), ("nbuild", C.c_uint64), ("probe", C.c_void_p), ("nprobe", C.c_uint64),
+                ("out", C.c_void_p), ("out_cap", C.c_uint64)]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [("bits", C.c_uint8), ("passes", C.c_uint8), ("reserved", C.c_uint16), ("load_factor", C.c_float)]
+
+
 class Cursor(C.Structure):
     _fields_ = [("impl", C.c_void_p), ("pending", C.c_uint64)]
 
@@ -108,6 +117,17 @@ _SIGS = {
                                    C.c_void_p]),
     "pj_cluster_table_size": (C.c_uint64, []),
     "pj_gen_pairs_zipf": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_int, C.c_void_p]),
+    "pj_batch_create": (C.c_int, [C.POINTER(JoinDesc), C.POINTER(C.c_void_p)]),
+    "pj_batch_destroy": (C.c_int, [C.c_void_p]),
+    "pj_batch_set_config": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchCfg)]),
+    "pj_batch_get_config": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchCfg)]),
+    "pj_batch_tune": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchJoin), C.c_uint32, C.c_void_p]),
+    "pj_batch_prepare": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchJoin), C.c_void_p]),
+    "pj_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pj_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pj_batch_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "pj_gen_batch_join": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),
     "pj_model_size": (C.c_uint64, []),
     "pj_sort_u64": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
@@ -216,6 +236,70 @@ class Join:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Owns a pj_batch_t: many joins of one descriptor in one captured pipeline."""
+
+    def __init__(self, desc):
+        self.h = C.c_void_p()
+        check(lib().pj_batch_create(C.byref(desc), C.byref(self.h)))
+        self.n = 0
+
+    @staticmethod
+    def _joins(joins):
+        arr = (BatchJoin * len(joins))()
+        for i, (b, nb, p, np_, o, cap) in enumerate(joins):
+            arr[i] = BatchJoin(b, nb, p, np_, o, cap)
+        return arr
+
+    def tune(self, joins, trial=4, stream=None):
+        arr = self._joins(joins)
+        check(lib().pj_batch_tune(self.h, len(joins), arr, trial, C.c_void_p(stream)))
+
+    def config(self, size_class):
+        c = BatchCfg()
+        check(lib().pj_batch_get_config(self.h, size_class, C.byref(c)))
+        return c.bits, c.passes, c.load_factor
+
+    def set_config(self, size_class, bits, passes, load_factor):
+        c = BatchCfg(bits, passes, 0, load_factor)
+        check(lib().pj_batch_set_config(self.h, size_class, C.byref(c)))
+
+    def prepare(self, joins, stream=None):
+        """joins: list of (d_build, nbuild, d_probe, nprobe, d_out, out_cap)."""
+        arr = self._joins(joins)
+        check(lib().pj_batch_prepare(self.h, len(joins), arr, C.c_void_p(stream)))
+        self.n = len(joins)
+
+    def run(self, graph=True, stream=None):
+        check(lib().pj_batch_run(self.h, 1 if graph else 0, C.c_void_p(stream)))
+
+    def results(self, stream=None):
+        import numpy as np
+        cnt = np.zeros(self.n, np.uint64)
+        done = np.zeros(self.n, np.float64)
+        check(lib().pj_batch_results(self.h, cnt.ctypes.data_as(C.c_void_p), done.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(stream)))
+        return cnt, done
+
+    def close(self):
+        if self.h:
+            lib().pj_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_sizes(seed, njoins, nmin, nmax):
+    import numpy as np
+    n = np.zeros(njoins, np.uint64)
+    check(lib().pj_batch_sizes(seed, njoins, nmin, nmax, n.ctypes.data_as(C.c_void_p)))
+    return n
 
 
 def run_plan(plan, sources, out_cap_bytes=1 << 20):

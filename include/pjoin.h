@@ -110,8 +110,8 @@ int pj_join_destroy(pj_join_t h);
 /* One whole join on device-resident relations: the compute of
  * HashJoinOp::scanStart (build, join.cpp:548-637) followed by every
  * HashJoinOp::getNext (probe, join.cpp:681-757). d_build/d_probe hold packed
- * build/probe tuples in HBM; d_out receives up to out_cap output tuples.
- * *out_n gets the number of output tuples; if it exceeds out_cap the call
+ * build/probe tuples in HBM; d_out (16-byte aligned) receives up to out_cap
+ * output tuples. *out_n gets the number of output tuples; if it exceeds out_cap the call
  * returns PJ_ERR_CAPACITY and the buffer holds an unspecified subset.
  * stream: a hipStream_t, or NULL for the default stream. Synchronous. */
 int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild,

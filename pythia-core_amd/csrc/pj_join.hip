@@ -1422,6 +1422,8 @@ int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild, const 
 {
     if (!h || !out_n || (nbuild && !d_build) || (nprobe && !d_probe) || (out_cap && !d_out))
         return fail(PJ_ERR_INVALID, "null argument");
+    if (out_cap && ((uintptr_t)d_out % 16))
+        return fail(PJ_ERR_INVALID, "the output buffer must be 16-byte aligned");
     std::lock_guard<std::mutex> g(h->mu);
     hipStream_t st = as_stream(stream);
     int rc = prepare_build(h, d_build, nbuild, st);

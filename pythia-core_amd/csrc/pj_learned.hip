@@ -47,7 +47,7 @@ constexpr double kFill = 0.85;             // planned fill of an LDS table
 constexpr double kFillMax = 0.92;          // highest fill accepted to save a probe re-read
 constexpr double kDenseDensity = 0.97;     // keys per key value above which a segment counts as dense
 
-constexpr uint32_t kMaxFine = 5120;        // build partitions the range kernels' LDS can hold
+constexpr uint32_t kMaxFine = 4096;        // build partitions the range kernels' LDS can hold
 
 constexpr uint32_t kSortThreads = 1024;
 constexpr uint32_t kSortChunk = 4096;

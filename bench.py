@@ -41,6 +41,125 @@ CLUSTERED_WORKLOADS = {
     # name: (seed, |R|, |S|)
     "clustered_64Mx512M": (404, 1 << 26, 1 << 29),
 }
+# Many independent (int64 key, int32 payload) joins: build sizes log-uniform
+# in [nmin, nmax], probe = 4 x build with a uniform fk.
+BATCH_WORKLOADS = {
+    # name: (seed, joins, nmin, nmax)
+    "batch_512": (505, 512, 1 << 10, 1 << 22),
+}
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_batch(args):
+    """The batch workload: every join through the single-join path in turn
+    (one launch sequence per join), radix bits tuned per size class. One step
+    is the whole batch; value is aggregate build + probe tuples/s."""
+    import numpy as np
+    import torch
+    import pjoin as P
+
+    seed, nj, nmin, nmax = BATCH_WORKLOADS[args.workload]
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    sizes = [int(x) for x in P.batch_sizes(seed, nj, nmin, nmax)]
+
+    def up(n):
+        return (n + 255) // 256 * 256
+
+    offs, tot = [], 0
+    for nb in sizes:
+        ob = tot
+        op = ob + up(nb * 12)
+        oo = op + up(4 * nb * 12)
+        tot = oo + up(4 * nb * 16)
+        offs.append((ob, op, oo))
+    arena = torch.empty(tot, dtype=torch.uint8, device=dev)
+    base = arena.data_ptr()
+    joins = []
+    for j, (nb, (ob, op, oo)) in enumerate(zip(sizes, offs)):
+        P.check(P.lib().pj_gen_batch_join(seed, j, nb, 4 * nb, C.c_void_p(base + ob), C.c_void_p(base + op), 1,
+                                          C.c_void_p(stream)))
+        joins.append((base + ob, nb, base + op, 4 * nb, base + oo, 4 * nb))
+    torch.cuda.synchronize()
+    _say("batch: %d joins generated, %.2f GB resident" % (nj, tot / 1e9))
+    desc = P.make_desc(["long", "int"], ["long", "int"], [0], [0], ["B$0", "B$1", "P$1"])
+    bt = P.Batch(desc)
+    bt.prepare(joins)
+
+    def timed(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bt.run(P.Batch.SEQUENTIAL, stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    for _ in range(args.warmup):
+        bt.run(P.Batch.SEQUENTIAL, stream)
+    ms_default = timed(max(1, args.steps // 2))
+    _, done_default = bt.results()
+    _say("batch: default bits %.2f ms/step" % ms_default)
+    t0 = time.perf_counter()
+    bt.tune(joins, trial=4, stream=stream)
+    tune_s = time.perf_counter() - t0
+    bt.prepare(joins)
+    for _ in range(args.warmup):
+        bt.run(P.Batch.SEQUENTIAL, stream)
+    ms = timed(args.steps)
+    cnt, done = bt.results()
+    _say("batch: tuned bits %.2f ms/step (tuning %.1f s)" % (ms, tune_s))
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    bad = 0
+    for j, (b, nb, p, npr, o, cap) in enumerate(joins):
+        es, ec = O.expected_batch_join(seed, j, nb, npr, threads=16)
+        s_, _ = P.checksum_device(o, int(cnt[j]), 16, stream)
+        bad += (int(cnt[j]), s_) != (ec, es)
+        if j % 128 == 127:
+            _say("batch: parity checked for %d joins" % (j + 1))
+    lat = np.diff(np.concatenate([[0.0], done]))
+    lat_d = np.diff(np.concatenate([[0.0], done_default]))
+    tuples = sum(5 * nb for nb in sizes)
+    out_rows = int(cnt.sum())
+    b_alg = sum(nb * 12 + 4 * nb * 12 for nb in sizes) + out_rows * 16
+    achieved = b_alg / (ms * 1e-3) / 1e9
+    tuned = {}
+    for c in range(32):
+        if any(int(np.log2(nb)) == c for nb in sizes):
+            tuned[c] = bt.config(c)[0]
+    res = {
+        "metric": "aggregate join tuples/s (build+probe), 512 int64-key joins, build 1K-4M log-uniform, probe 4x; "
+                  "one launch sequence per join, tuned radix bits",
+        "value": tuples / (ms * 1e-3),
+        "unit": "tuples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": args.workload, "joins": nj, "build_min": nmin, "build_max": nmax,
+                   "build_tuples": sum(sizes), "probe_tuples": 4 * sum(sizes), "tuple_bytes": 12,
+                   "out_tuple_bytes": 16, "out_tuples": out_rows, "mode": "sequential",
+                   "tuned_bits_by_size_class": tuned, "pipeline": "switched off"},
+        "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max()), "note": "per join, one launch sequence per join, tuned bits"},
+        "default_bits": {"ms_per_step": ms_default, "value": tuples / (ms_default * 1e-3),
+                         "latency_p50_us": float(np.percentile(lat_d, 50)),
+                         "latency_p99_us": float(np.percentile(lat_d, 99))},
+        "tune_seconds": tune_s,
+        "parity": {"joins": nj, "mismatched_joins": bad},
+        "roofline": {"bound": "hbm", "kernel": "whole batch (all kernels of every join)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    del arena
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_batch(seed, sizes)
+    print(json.dumps(res))
+    if bad:
+        sys.exit(1)
 
 
 def cpu_baseline(seed, dom, nb, npr, nmiss):
@@ -102,12 +221,50 @@ def cpu_baseline_zipf(seed, nb, npr, cdf):
             "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
 
 
+def cpu_baseline_batch(seed, sizes):
+    """The reference HashJoinOp (oracle/_ref), else the oracle restatement,
+    on every 8th join of the batch (a sample of the same size mix), one join
+    after the other, each with a modulo hash over next_pow2(|R|) buckets."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import reference_run as R
+    import pjoin as P
+    cols = ["long", "int"]
+    proj = ["B$0", "B$1", "P$1"]
+    total, secs, used, cores, kind = 0, 0.0, 0, 1, "port"
+    for j, nb in enumerate(sizes):
+        if j % 8:
+            continue
+        b = np.zeros(nb * 12, np.uint8)
+        p = np.zeros(4 * nb * 12, np.uint8)
+        P.check(P.lib().pj_gen_batch_join(seed, j, nb, 4 * nb, b.ctypes.data_as(C.c_void_p),
+                                          p.ctypes.data_as(C.c_void_p), 0, None))
+        buckets = 1 << max(1, (nb - 1).bit_length())
+        if R.available():
+            cores, kind = R.default_threads(), "reference"
+            _, _, tt = R.run_join(cols, cols, [0], [0], proj, b, p, threads=cores, buckets=buckets)
+        else:
+            d = O.join_desc(cols, cols, [0], [0], proj, buckets=buckets)
+            t0 = time.perf_counter()
+            O.hashjoin(d, b, p)
+            tt = time.perf_counter() - t0
+        secs += tt
+        total += 5 * nb
+        used += 1
+    return {"value": total / secs, "unit": "tuples/s", "cores": cores, "kind": kind,
+            "sample": "every 8th join of the batch, %d joins (%d tuples), one after the other; %s" % (
+                used, total, "reference HashJoinOp built -O3 from its sources, %d threads" % cores
+                if kind == "reference" else "restated HashJoinOp, one thread"), "seconds": secs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS) + sorted(ZIPF_WORKLOADS) + sorted(CLUSTERED_WORKLOADS))
+    ap.add_argument("--workload", default="uniform_16Mx256M", choices=sorted(WORKLOADS) + sorted(ZIPF_WORKLOADS) + sorted(CLUSTERED_WORKLOADS) +
+                    sorted(BATCH_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -118,6 +275,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
               "written up, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload in BATCH_WORKLOADS:
+        return bench_batch(args)
 
     import torch
     import pjoin as P

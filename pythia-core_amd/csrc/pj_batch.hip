@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -434,6 +435,12 @@ struct pj_batch_s {
     hipGraphExec_t exec = nullptr;
     bool prepared = false;
     bool enabled = false;
+    // sequential mode: the joins run one by one through pj_join_run_device,
+    // one join handle per radix-bits value
+    std::map<uint32_t, pj_join_t> seq;
+    bool last_sequential = false;
+    std::vector<uint64_t> seq_counts;
+    std::vector<double> seq_done_us, seq_run_us;
 };
 
 namespace {
@@ -675,6 +682,73 @@ std::vector<pj_batch_cfg> cfgs_for(pj_batch_s* h, const std::vector<pj_batch_joi
     return c;
 }
 
+int seq_handle(pj_batch_s* h, uint32_t bits, pj_join_t* out)
+{
+    auto it = h->seq.find(bits);
+    if (it != h->seq.end()) {
+        *out = it->second;
+        return PJ_OK;
+    }
+    pj_join_desc d = h->desc;
+    d.partitioner = PJ_PART_RADIX;
+    d.radix_bits = (int32_t)bits;
+    d.flags = 0;
+    int rc = pj_join_create(&d, out);
+    if (rc)
+        return rc;
+    h->seq[bits] = *out;
+    return PJ_OK;
+}
+
+// Radix bits of a sequential-mode join: its size class's tuned bits, else
+// the bits the single-join path picks by itself.
+uint32_t seq_bits(const pj_batch_s* h, uint64_t nb)
+{
+    const uint32_t c = size_class(nb);
+    return h->tuned[c] ? h->cfg[c].bits : choose_build_bits(nb, -1);
+}
+
+// Runs joins one by one, each a whole pj_join_run_device (partition both
+// sides, join) with `bits` radix bits, timing each with HIP events.
+int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+                   hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
+                   std::vector<double>& run_us)
+{
+    const size_t nj = joins.size();
+    counts.assign(nj, 0);
+    done_us.assign(nj, 0.0);
+    run_us.assign(nj, 0.0);
+    hipEvent_t e0, a, b;
+    PJ_HIP(hipEventCreate(&e0));
+    PJ_HIP(hipEventCreate(&a));
+    PJ_HIP(hipEventCreate(&b));
+    int rc = PJ_OK;
+    (void)hipEventRecord(e0, st);
+    for (size_t i = 0; i < nj && rc == PJ_OK; ++i) {
+        const pj_batch_join& q = joins[i];
+        pj_join_t jh;
+        if ((rc = seq_handle(h, bits[i], &jh)))
+            break;
+        (void)hipEventRecord(a, st);
+        uint64_t n = 0;
+        rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, st);
+        if (rc == PJ_ERR_CAPACITY)   // reported per join by the results
+            rc = PJ_OK;
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f, t = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventElapsedTime(&t, e0, b);
+        counts[i] = n;
+        run_us[i] = ms * 1e3;
+        done_us[i] = t * 1e3;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -705,6 +779,8 @@ int pj_batch_destroy(pj_batch_t h)
 {
     if (!h)
         return PJ_OK;
+    for (auto& kv : h->seq)
+        pj_join_destroy(kv.second);
     if (h->exec)
         (void)hipGraphExecDestroy(h->exec);
     if (h->graph)
@@ -743,26 +819,43 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
     std::lock_guard<std::mutex> g(h->mu);
     hipStream_t st = (hipStream_t)stream;
     std::vector<pj_batch_join> js(joins, joins + njoins);
+    h->last_sequential = false;
+    h->seq_counts.clear();
+    if (!h->enabled) {   // only the sequential mode runs: keep the list
+        h->joins = js;
+        h->prepared = true;
+        return PJ_OK;
+    }
     int rc = prepare(h, js, cfgs_for(h, js), st);
     if (rc)
         return rc;
     return capture(h, st);
 }
 
-int pj_batch_run(pj_batch_t h, int use_graph, void* stream)
+int pj_batch_run(pj_batch_t h, int mode, void* stream)
 {
-    if (!h)
-        return fail(PJ_ERR_INVALID, "null argument");
-    // The first device run of this pipeline (graph replay of a 40-join
-    // batch) stopped making progress and its cause has not been found; the
-    // path stays off until it is.
-    if (!h->enabled)
-        return fail(PJ_ERR_UNSUPPORTED, "batched joins are disabled: their first device run hung, cause not found");
+    if (!h || mode < PJ_BATCH_PLAIN || mode > PJ_BATCH_SEQUENTIAL)
+        return fail(PJ_ERR_INVALID, "null handle or unknown mode");
     std::lock_guard<std::mutex> g(h->mu);
     if (!h->prepared)
         return fail(PJ_ERR_STATE, "pj_batch_run before pj_batch_prepare");
     hipStream_t st = (hipStream_t)stream;
-    if (use_graph) {
+    if (mode == PJ_BATCH_SEQUENTIAL) {
+        std::vector<uint32_t> bits(h->joins.size());
+        for (size_t i = 0; i < bits.size(); ++i)
+            bits[i] = seq_bits(h, h->joins[i].nbuild);
+        const int rc = run_sequential(h, h->joins, bits, st, h->seq_counts, h->seq_done_us, h->seq_run_us);
+        h->last_sequential = rc == PJ_OK;
+        return rc;
+    }
+    // The first device run of the batched pipeline (graph replay of a
+    // 40-join batch) stopped making progress and its cause has not been
+    // found; the pipeline stays off until it is.
+    if (!h->enabled)
+        return fail(PJ_ERR_UNSUPPORTED, "the batched pipeline is disabled: its first device run hung, cause not "
+                                         "found (PJ_BATCH_SEQUENTIAL runs)");
+    h->last_sequential = false;
+    if (mode == PJ_BATCH_GRAPH) {
         if (!h->exec)
             return fail(PJ_ERR_STATE, "no captured graph");
         PJ_HIP(hipGraphLaunch(h->exec, st));
@@ -780,6 +873,20 @@ int pj_batch_results(pj_batch_t h, uint64_t* out_counts, double* done_us, void* 
         return fail(PJ_ERR_STATE, "no prepared batch");
     hipStream_t st = (hipStream_t)stream;
     const uint32_t nj = (uint32_t)h->joins.size();
+    if (h->last_sequential || !h->enabled) {
+        if (!h->last_sequential || h->seq_counts.size() != nj)
+            return fail(PJ_ERR_STATE, "no completed run");
+        int rc = PJ_OK;
+        for (uint32_t i = 0; i < nj; ++i) {
+            out_counts[i] = h->seq_counts[i];
+            if (done_us)
+                done_us[i] = h->seq_done_us[i];
+            if (h->seq_counts[i] > h->joins[i].out_cap && rc == PJ_OK)
+                rc = fail(PJ_ERR_CAPACITY, "join %u produced %llu rows, its output holds %llu", i,
+                          (unsigned long long)h->seq_counts[i], (unsigned long long)h->joins[i].out_cap);
+        }
+        return rc;
+    }
     std::vector<BState> s(nj);
     unsigned long long t0 = 0;
     PJ_HIP(hipMemcpyAsync(s.data(), h->state.p, nj * sizeof(BState), hipMemcpyDeviceToHost, st));
@@ -804,8 +911,6 @@ int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uin
 {
     if (!h || !joins || njoins == 0 || trial == 0)
         return fail(PJ_ERR_INVALID, "bad arguments");
-    if (!h->enabled)
-        return fail(PJ_ERR_UNSUPPORTED, "batched joins are disabled: their first device run hung, cause not found");
     hipStream_t st = (hipStream_t)stream;
     std::lock_guard<std::mutex> g(h->mu);
     std::vector<std::vector<pj_batch_join>> by(kClasses);
@@ -813,6 +918,41 @@ int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uin
         const uint32_t c = size_class(joins[i].nbuild);
         if (by[c].size() < trial)
             by[c].push_back(joins[i]);
+    }
+    if (!h->enabled) {
+        // the pipeline is off: tune the radix bits of the sequential mode,
+        // around the bits the single-join path picks by itself
+        for (uint32_t c = 0; c < kClasses; ++c) {
+            if (by[c].empty())
+                continue;
+            uint64_t nmax = 0;
+            for (const auto& q : by[c])
+                nmax = std::max<uint64_t>(nmax, q.nbuild);
+            const uint32_t b0 = std::max<uint32_t>(kBMinBits, std::min<uint32_t>(kBMaxBits, choose_build_bits(nmax, -1)));
+            double best = 1e300;
+            pj_batch_cfg bestc = default_cfg(c);
+            for (uint32_t b = b0; b <= std::min<uint32_t>(kBMaxBits, b0 + 3); ++b) {
+                std::vector<uint32_t> bits(by[c].size(), b);
+                std::vector<uint64_t> cnt;
+                std::vector<double> done, run;
+                double t = 1e300;
+                for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
+                    int rc = run_sequential(h, by[c], bits, st, cnt, done, run);
+                    if (rc)
+                        return rc;
+                    if (rep)
+                        t = std::min(t, done.back());
+                }
+                if (t < best) {
+                    best = t;
+                    bestc.bits = (uint8_t)b;
+                    bestc.passes = 1;
+                }
+            }
+            h->cfg[c] = bestc;
+            h->tuned[c] = true;
+        }
+        return PJ_OK;
     }
     const float lfs[3] = {0.5f, 0.7f, 0.9f};
     for (uint32_t c = 0; c < kClasses; ++c) {

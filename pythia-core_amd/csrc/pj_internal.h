@@ -123,6 +123,11 @@ struct RangePlan {
     float loss_lsq = 0.f, loss = 0.f;
 };
 
+// Radix bits of the build side for nr tuples (forced >= 0 overrides): the
+// fewest bits whose mean partition, plus five standard deviations, fits an
+// LDS table (pj_join.hip).
+uint32_t choose_build_bits(uint64_t nr, int32_t forced);
+
 // Device state of the learned partitioner (pj_learned.hip).
 struct LearnedCtx {
     DevBuf sample, tmp, hist, bsum, model, params, moments, resid, bounds, segs, loss;

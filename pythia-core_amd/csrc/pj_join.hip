@@ -127,7 +127,6 @@ __global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uin
 }
 
 constexpr int kWcThreads = 1024;
-constexpr int kWcItems = 4;
 
 // Software write-combining scatter: every partition owns a B-record buffer
 // in LDS and leaves HBM only in aligned chunks of B records. A round claims
@@ -136,34 +135,48 @@ constexpr int kWcItems = 4;
 // leaves as one chunk (a B-lane group writes it), whole chunks of overflow
 // records are written straight after it, and the overflow remainder refills
 // the buffer. Three barriers a round, whatever the skew.
-template <bool IDENT, int B, bool RANGE>
+//
+// SPLIT = 2 halves the LDS a block needs, so that 2048 partitions still get
+// 8-record (128-byte, whole cache line) chunks: blocks b and b + 8 share
+// input chunk `part` (under round-robin dispatch both run on the same XCD, so
+// the second read of a tile mostly hits its L2) and block half h buffers and
+// writes only partitions [h * FL, (h + 1) * FL). On MI355X a 2048-way scatter
+// of 256M 16-byte records takes 2.5 ms with 64-byte chunks and 1.8-1.9 ms
+// split with 128-byte chunks.
+template <bool IDENT, int B, bool RANGE, int SPLIT>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
                                                             SideLayout L, PartFn pf, uint64_t chunk,
                                                             uint32_t nblk, const uint32_t* __restrict__ offs,
                                                             uint4* __restrict__ out, uint4 pad_other, uint4 pad_zero,
                                                             const uint16_t* __restrict__ pid)
 {
+    static_assert(SPLIT == 1 || SPLIT == 2, "one or two blocks an input chunk");
+    constexpr int IT = SPLIT == 1 ? 4 : 8;   // records a thread per round
     extern __shared__ uint4 sm4[];
     const uint32_t F = pf.F;
-    uint4* buf = sm4;                                                          // F * B
-    uint64_t* bnd = reinterpret_cast<uint64_t*>(buf + F * B);                  // F - 1 (range)
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));    // F: buffered records
-    uint32_t* goff = cnt + F;                                                  // F: next chunk's record index
+    const uint32_t FL = SPLIT == 1 ? F : (F + 1) / 2;   // partitions this block buffers
+    const uint32_t part = SPLIT == 1 ? blockIdx.x : (blockIdx.x / 16) * 8 + blockIdx.x % 8;
+    const uint32_t half = SPLIT == 1 ? 0u : (blockIdx.x / 8) & 1u;
+    const uint32_t p0 = half * FL, pn = SPLIT == 1 ? F : (half ? F - FL : FL);
+    uint4* buf = sm4;                                                          // FL * B
+    uint64_t* bnd = reinterpret_cast<uint64_t*>(buf + FL * B);                 // p2 - 1 (range)
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));    // FL: buffered records
+    uint32_t* goff = cnt + FL;                                                 // FL: next chunk's record index
     load_bounds<RANGE>(bnd, pf);
-    for (uint32_t p = threadIdx.x; p < F; p += kWcThreads) {
-        goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
+    for (uint32_t p = threadIdx.x; p < pn; p += kWcThreads) {
+        goff[p] = offs[(uint64_t)(p0 + p) * nblk + part];
         cnt[p] = 0;
     }
     __syncthreads();
-    const uint64_t beg = (uint64_t)blockIdx.x * chunk;
+    const uint64_t beg = (uint64_t)part * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
     const bool have_pid = RANGE && pid != nullptr;
-    uint4 nxt[kWcItems];   // the next round's records, loaded one round ahead
-    uint32_t nxp[kWcItems] = {};
+    uint4 nxt[IT];   // the next round's records, loaded one round ahead
+    uint32_t nxp[IT] = {};
 #pragma unroll
-    for (int k = 0; k < kWcItems; ++k) {
+    for (int k = 0; k < IT; ++k) {
         const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
         if (i < end) {
             nxt[k] = load_rec<IDENT>(in, i, L);
@@ -171,41 +184,47 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                 nxp[k] = pid[i];
         }
     }
-    for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * kWcItems) {
-        uint4 r[kWcItems];
-        uint32_t p[kWcItems], s[kWcItems];
-        bool v[kWcItems];
+    for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * IT) {
+        uint4 r[IT];
+        uint32_t p[IT], s[IT];
+        bool v[IT];
 #pragma unroll
-        for (int k = 0; k < kWcItems; ++k) {
+        for (int k = 0; k < IT; ++k) {
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
             v[k] = i < end;
             r[k] = nxt[k];
             p[k] = nxp[k];
-            const uint64_t i2 = i + (uint64_t)kWcThreads * kWcItems;
+            const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
             if (i2 < end) {
                 nxt[k] = load_rec<IDENT>(in, i2, L);
                 if (have_pid)
                     nxp[k] = pid[i2];
             }
         }
-        // claim
+        // claim (a split block keeps only its half's partitions)
 #pragma unroll
-        for (int k = 0; k < kWcItems; ++k)
+        for (int k = 0; k < IT; ++k)
             if (v[k]) {
                 if (!have_pid)
                     p[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
-                s[k] = atomicAdd(&cnt[p[k]], 1u);
-                if (s[k] < (uint32_t)B)
-                    buf[p[k] * B + s[k]] = r[k];
+                if (SPLIT > 1) {
+                    v[k] = (p[k] >= FL) == (half != 0);
+                    p[k] -= p0;
+                }
+                if (v[k]) {
+                    s[k] = atomicAdd(&cnt[p[k]], 1u);
+                    if (s[k] < (uint32_t)B)
+                        buf[p[k] * B + s[k]] = r[k];
+                }
             }
         __syncthreads();
         // full buffers leave; whole chunks of overflow go straight out
-        for (uint32_t q = grp; q < F; q += ngrp)
+        for (uint32_t q = grp; q < pn; q += ngrp)
             if (cnt[q] >= (uint32_t)B)
                 out[goff[q] + gl] = buf[q * B + gl];
-        uint32_t keep[kWcItems];
+        uint32_t keep[IT];
 #pragma unroll
-        for (int k = 0; k < kWcItems; ++k) {
+        for (int k = 0; k < IT; ++k) {
             keep[k] = ~0u;
             if (v[k] && s[k] >= (uint32_t)B) {
                 const uint32_t rank = s[k] - B;
@@ -218,10 +237,10 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kWcItems; ++k)
+        for (int k = 0; k < IT; ++k)
             if (keep[k] != ~0u)
                 buf[p[k] * B + keep[k]] = r[k];
-        for (uint32_t q = threadIdx.x; q < F; q += kWcThreads) {
+        for (uint32_t q = threadIdx.x; q < pn; q += kWcThreads) {
             const uint32_t c = cnt[q];
             if (c >= (uint32_t)B) {
                 const uint32_t direct = (c - B) / B * B;
@@ -232,10 +251,10 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         __syncthreads();
     }
     // partial buffers leave padded to a whole chunk
-    for (uint32_t q = grp; q < F; q += ngrp) {
+    for (uint32_t q = grp; q < pn; q += ngrp) {
         const uint32_t c = cnt[q];
         if (c > 0)
-            out[goff[q] + gl] = gl < c ? buf[q * B + gl] : (q == 0 ? pad_zero : pad_other);
+            out[goff[q] + gl] = gl < c ? buf[q * B + gl] : (p0 + q == 0 ? pad_zero : pad_other);
     }
 }
 
@@ -808,12 +827,9 @@ struct pj_join_s {
     bool ev_ok = false;
 };
 
-namespace {
-
 constexpr uint32_t kMaxBits = 14;
-constexpr uint32_t kMaxCoarse = 2048;   // probe partitions of a range plan (write-combining fan-out)
 
-uint32_t choose_build_bits(uint64_t nr, int32_t forced)
+uint32_t pj::choose_build_bits(uint64_t nr, int32_t forced)
 {
     if (forced >= 0)
         return (uint32_t)std::min<int32_t>(forced, (int32_t)kMaxBits);
@@ -825,6 +841,10 @@ uint32_t choose_build_bits(uint64_t nr, int32_t forced)
     return kMaxBits;
 }
 
+namespace {
+
+constexpr uint32_t kMaxCoarse = 2048;   // probe partitions of a range plan (write-combining fan-out)
+
 uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes, bool range)
 {
     const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4 - bnd_bytes;
@@ -834,10 +854,10 @@ uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes, bool range)
     return ts > 0 ? (uint32_t)ts : 0;
 }
 
-// Records per write-combining chunk for a fan-out of F: the largest power of
-// two (at most 16) whose buffers fit in 128 KiB of LDS, and next to the
-// range boundaries in the LDS that is left; 1 selects the tile-sorted
-// scatter (no padding).
+// Records per write-combining chunk for a fan-out of F buffered by one block:
+// the largest power of two (at most 16) whose buffers fit in 128 KiB of LDS,
+// and next to the range boundaries in the LDS that is left; 1 selects the
+// tile-sorted scatter (no padding).
 uint32_t wc_chunk(uint32_t F, uint32_t bnd_bytes)
 {
     if (F <= 1)
@@ -846,6 +866,15 @@ uint32_t wc_chunk(uint32_t F, uint32_t bnd_bytes)
     while (B > 1 && ((uint64_t)F * B * 16 > 131072 || (uint64_t)F * B * 16 + 8ull * F + bnd_bytes > kLdsBytes))
         B >>= 1;
     return B < 2 ? 1 : B;
+}
+
+// Blocks an input chunk of the write-combining scatter: two when one block
+// could only buffer chunks of fewer than 8 records (64 bytes), which HBM
+// writes at half the rate of whole 128-byte lines, and two blocks of half
+// the partitions each reach 8.
+uint32_t wc_split(uint32_t F, uint32_t bnd_bytes)
+{
+    return F > 1 && wc_chunk(F, bnd_bytes) < 8 && wc_chunk((F + 1) / 2, bnd_bytes) >= 8 ? 2u : 1u;
 }
 
 uint32_t hist_blocks(uint64_t n)
@@ -867,12 +896,14 @@ uint64_t pad_key_nonzero_part()
             return k;
 }
 
-template <int B, bool RANGE>
+template <int B, bool RANGE, int SPLIT>
 int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
                 uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
 {
-    const uint32_t F = pf.F;
-    const size_t lds = (size_t)F * B * 16 + 2 * F * 4 + bounds_lds_bytes(pf, RANGE);
+    const uint32_t FL = SPLIT == 1 ? pf.F : (pf.F + 1) / 2;
+    if (SPLIT > 1 && nblk % 8)
+        return fail(PJ_ERR_INVALID, "split scatter needs a multiple of 8 input chunks");
+    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4 + bounds_lds_bytes(pf, RANGE);
     uint4 pad_other, pad_zero;
     if (RANGE) {
         // ordered key 0 lies in partition 0, ordered key ~0 in the last one
@@ -883,15 +914,16 @@ int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
         pad_other = make_uint4(0, 0, 0, 0);
         pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
     }
+    const dim3 grid(nblk * SPLIT);
     if (ident) {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B, RANGE>,
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B, RANGE, SPLIT>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_scatter_wc<true, B, RANGE>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, pf, chunk,
-                           nblk, offs, out, pad_other, pad_zero, pid);
+        hipLaunchKernelGGL((k_scatter_wc<true, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
+                           chunk, nblk, offs, out, pad_other, pad_zero, pid);
     } else {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B, RANGE>,
+        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B, RANGE, SPLIT>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL((k_scatter_wc<false, B, RANGE>), dim3(nblk), dim3(kWcThreads), lds, st, in, n, L, pf,
+        hipLaunchKernelGGL((k_scatter_wc<false, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
                            chunk, nblk, offs, out, pad_other, pad_zero, pid);
     }
     PJ_HIP(hipGetLastError());
@@ -899,11 +931,15 @@ int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
 }
 
 template <int B>
-int launch_wc(bool ident, bool range, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
-              uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
+int launch_wc(bool ident, bool range, uint32_t split, const uint32_t* in, uint64_t n, const SideLayout& L,
+              const PartFn& pf, uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
+              hipStream_t st)
 {
-    return range ? launch_wc_t<B, true>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
-                 : launch_wc_t<B, false>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+    if (split > 1)
+        return range ? launch_wc_t<B, true, 2>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
+                     : launch_wc_t<B, false, 2>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+    return range ? launch_wc_t<B, true, 1>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
+                 : launch_wc_t<B, false, 1>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
 }
 
 template <bool RANGE>
@@ -976,8 +1012,11 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     if (n >= (1ull << 31))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
     const uint32_t bb = bounds_lds_bytes(pf, range);
-    const uint32_t B = write_combine ? wc_chunk(F, bb) : 1;
-    const uint32_t nblk = B > 1 ? part_blocks(n) : hist_blocks(n);
+    const uint32_t split = write_combine ? wc_split(F, bb) : 1;
+    const uint32_t B = write_combine ? wc_chunk(split > 1 ? (F + 1) / 2 : F, bb) : 1;
+    uint32_t nblk = B > 1 ? part_blocks(n) : hist_blocks(n);
+    if (split > 1)   // input chunks pair up blocks b and b + 8 (one XCD)
+        nblk = (nblk + 7) / 8 * 8;
     const uint64_t chunk = (n + nblk - 1) / nblk;
     const uint64_t m = (uint64_t)F * nblk;
     const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
@@ -1016,10 +1055,10 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     PJ_HIP(hipGetLastError());
     uint4* out = (uint4*)dst.p;
     switch (B) {
-    case 16: rc = launch_wc<16>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 8: rc = launch_wc<8>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 4: rc = launch_wc<4>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 2: rc = launch_wc<2>(ident, range, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 16: rc = launch_wc<16>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 8: rc = launch_wc<8>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 4: rc = launch_wc<4>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 2: rc = launch_wc<2>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
     default:
         rc = range ? launch_tile_scatter<true>(ident, in, n, L, pf, chunk, nblk, hist, out, pid, st)
                    : launch_tile_scatter<false>(ident, in, n, L, pf, chunk, nblk, hist, out, nullptr, st);

@@ -272,8 +272,10 @@ class Batch:
         check(lib().pj_batch_prepare(self.h, len(joins), arr, C.c_void_p(stream)))
         self.n = len(joins)
 
-    def run(self, graph=True, stream=None):
-        check(lib().pj_batch_run(self.h, 1 if graph else 0, C.c_void_p(stream)))
+    PLAIN, GRAPH, SEQUENTIAL = 0, 1, 2
+
+    def run(self, mode=GRAPH, stream=None):
+        check(lib().pj_batch_run(self.h, mode, C.c_void_p(stream)))
 
     def results(self, stream=None):
         import numpy as np

@@ -77,3 +77,20 @@ def test_probe_before_build_finish_is_a_state_error():
     rc = P.lib().pj_join_probe(j.h, tup, 1, C.byref(cur), out, 1, C.byref(n))
     assert rc == P.PJ_ERR_STATE
     j.close()
+
+
+def test_batch_modes_before_and_after_prepare():
+    """The batch handle checks its mode and state on the host: running before
+    prepare is a state error, an unknown mode is invalid, and the pipeline
+    modes report that they are switched off (no device work is started)."""
+    bt = P.Batch(_desc())
+    assert P.lib().pj_batch_run(bt.h, P.Batch.SEQUENTIAL, None) == P.PJ_ERR_STATE
+    buf = (C.c_uint8 * 64)()
+    bt.prepare([(C.addressof(buf), 1, C.addressof(buf), 1, C.addressof(buf), 1)])
+    assert P.lib().pj_batch_run(bt.h, 7, None) == P.PJ_ERR_INVALID
+    for mode in (P.Batch.PLAIN, P.Batch.GRAPH):
+        assert P.lib().pj_batch_run(bt.h, mode, None) == P.PJ_ERR_UNSUPPORTED
+        assert b"PJ_BATCH_SEQUENTIAL" in P.lib().pj_last_error()
+    cnt = (C.c_uint64 * 1)()
+    assert P.lib().pj_batch_results(bt.h, cnt, None, None) == P.PJ_ERR_STATE
+    bt.close()

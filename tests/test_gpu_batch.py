@@ -1,7 +1,9 @@
 """GPU tests of the batched joins (pj_batch_*): every join of a batch
 against the generator's closed form (count and 64-bit checksum) or the
-CPU oracle's HashJoinOp (sorted rows), under graph replay and plain
-launches, forced configurations and the tuner's.
+CPU oracle's HashJoinOp (sorted rows). The sequential mode (each join a
+whole pj_join_run_device with the radix bits of its size class) runs; the
+batched pipeline's tests (graph replay, plain launches, its forced
+configurations) are skipped while that pipeline is switched off.
 """
 import ctypes as C
 import sys
@@ -12,9 +14,10 @@ import pytest
 import oracle as O
 import pjoin as P
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="batched joins are switched off: the first device run of the batch pipeline "
-                                      "hung and its cause has not been found")]
+pytestmark = pytest.mark.gpu
+
+PIPELINE_OFF = pytest.mark.skip(reason="the batched pipeline is switched off: its first device run hung and the "
+                                       "cause has not been found")
 
 torch = pytest.importorskip("torch")
 
@@ -59,6 +62,7 @@ def _check(seed, joins, cnt):
         assert (int(cnt[j]), s) == (ec, es), "join %d (%d x %d)" % (j, nb, npr)
 
 
+@PIPELINE_OFF
 def test_batch_small_joins_graph_and_plain():
     _say("case: batch of 40 joins, graph replays and plain launches")
     seed = 21
@@ -68,7 +72,7 @@ def test_batch_small_joins_graph_and_plain():
     bt.prepare(joins)
     prev = None
     for rep in range(3):
-        bt.run(graph=True)
+        bt.run(P.Batch.GRAPH)
         torch.cuda.synchronize()
         x = torch.ones(1 << 20, device="cuda").sum().item()   # eager work between replays
         assert x == float(1 << 20)
@@ -78,12 +82,13 @@ def test_batch_small_joins_graph_and_plain():
         if prev is not None:
             assert np.array_equal(cnt, prev)
         prev = cnt
-    bt.run(graph=False)
+    bt.run(P.Batch.PLAIN)
     cnt, _ = bt.results()
     _check(seed, joins, cnt)
     bt.close()
 
 
+@PIPELINE_OFF
 @pytest.mark.parametrize("bits,passes,lf", [(4, 1, 0.5), (8, 2, 0.9), (12, 1, 0.7), (12, 2, 0.5), (9, 2, 0.7)])
 def test_batch_forced_configs(bits, passes, lf):
     _say("case: batch config bits=%d passes=%d lf=%.1f" % (bits, passes, lf))
@@ -94,7 +99,45 @@ def test_batch_forced_configs(bits, passes, lf):
     for c in range(32):
         bt.set_config(c, bits, passes, lf)
     bt.prepare(joins)
-    bt.run(graph=True)
+    bt.run(P.Batch.GRAPH)
+    cnt, _ = bt.results()
+    _check(seed, joins, cnt)
+    bt.close()
+
+
+MODE = P.Batch.SEQUENTIAL
+
+
+def test_batch_sequential_small_joins_repeat():
+    _say("case: batch of 40 joins, sequential mode, two runs")
+    seed = 21
+    sizes = list(P.batch_sizes(seed, 38, 1024, 1 << 16)) + [1, 3]
+    joins, keep = _make(seed, sizes)
+    bt = P.Batch(P.make_desc(**DESC))
+    bt.prepare(joins)
+    prev = None
+    for rep in range(2):
+        bt.run(MODE)
+        cnt, done = bt.results()
+        _check(seed, joins, cnt)
+        assert np.all(done > 0) and np.all(np.diff(done) > 0)
+        if prev is not None:
+            assert np.array_equal(cnt, prev)
+        prev = cnt
+    bt.close()
+
+
+@pytest.mark.parametrize("bits", [4, 9, 12])
+def test_batch_sequential_forced_bits(bits):
+    _say("case: batch sequential, bits=%d" % bits)
+    seed = 22
+    sizes = P.batch_sizes(seed, 12, 1024, 1 << 20)
+    joins, keep = _make(seed, sizes)
+    bt = P.Batch(P.make_desc(**DESC))
+    for c in range(32):
+        bt.set_config(c, bits, 1, 0.7)
+    bt.prepare(joins)
+    bt.run(MODE)
     cnt, _ = bt.results()
     _check(seed, joins, cnt)
     bt.close()
@@ -110,9 +153,9 @@ def test_batch_tuner_choices_and_results():
     for nb in sizes:
         c = int(np.floor(np.log2(int(nb))))
         bits, passes, lf = bt.config(c)
-        assert 4 <= bits <= 12 and passes in (1, 2) and round(lf, 2) in (0.5, 0.7, 0.9)
+        assert 4 <= bits <= 12 and passes in (1, 2) and 0.5 <= round(lf, 2) <= 0.9
     bt.prepare(joins)
-    bt.run(graph=True)
+    bt.run(MODE)
     cnt, _ = bt.results()
     _check(seed, joins, cnt)
     bt.close()
@@ -142,7 +185,7 @@ def test_batch_repeated_keys_match_oracle():
         host.append(exp)
     bt = P.Batch(P.make_desc(**DESC))
     bt.prepare(joins)
-    bt.run(graph=True)
+    bt.run(MODE)
     cnt, _ = bt.results()
     for j, exp in enumerate(host):
         got = keep[3 * j + 2][: int(cnt[j]) * 16].cpu().numpy().reshape(-1, 16)
@@ -158,7 +201,7 @@ def test_batch_output_overflow_reported():
     joins[1] = (b, nb, p, npr, o, 100)
     bt = P.Batch(P.make_desc(**DESC))
     bt.prepare(joins)
-    bt.run(graph=True)
+    bt.run(MODE)
     with pytest.raises(P.PJError) as e:
         bt.results()
     assert e.value.code == P.PJ_ERR_CAPACITY

@@ -93,7 +93,8 @@ def test_queryhashjoin_shape():
 
 
 @pytest.mark.parametrize("nb,npr,bits", [(1 << 16, 1 << 20, -1), (100000, 300000, 0), (100000, 300000, 14),
-                                         (3, 1000, -1), (9000, 50000, 0)])
+                                         (3, 1000, -1), (9000, 50000, 0), (3, 1000, 14),
+                                         (1 << 20, 3 << 20, 11)])
 def test_pkfk_vs_oracle(nb, npr, bits):
     _say("case: pkfk nb=%d np=%d bits=%d" % (nb, npr, bits))
     b, p = O.gen_pkfk(nb + npr, 1 << 40, nb, npr, npr // 10)

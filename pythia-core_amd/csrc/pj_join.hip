@@ -38,7 +38,7 @@ namespace pj {
 // writes whole, aligned chunks. Padding records carry a key of another
 // partition; the join drops any record outside the work item's partition.
 
-constexpr int kHistThreads = 512;
+constexpr int kHistThreads = 1024;
 constexpr int kHistItems = 8;
 
 // hist[p * nblk + blk] = tuples of block blk's chunk in partition p, rounded
@@ -1083,10 +1083,20 @@ int device_count_cus()
     return cus;
 }
 
+constexpr uint32_t kProbeSlicesPerCU = 8;
+
 uint64_t probe_chunk(uint64_t ns)
 {
-    const uint64_t slices = (uint64_t)device_count_cus() * 16;
+    const uint64_t slices = (uint64_t)device_count_cus() * kProbeSlicesPerCU;
     return std::max<uint64_t>((ns + slices - 1) / slices, 32768);
+}
+
+// Work items the s probe records of one build partition are cut into: about
+// `chunk` records each and all of one size, so that no item is a small
+// remainder that builds a whole table for a few probes.
+uint32_t probe_pieces(uint64_t s, uint64_t chunk)
+{
+    return (uint32_t)std::max<uint64_t>(1, (s + chunk / 2) / chunk);
 }
 
 // Builds the radix work list: for every build partition (a sub-partition of
@@ -1111,12 +1121,14 @@ void plan_items_radix(const pj_join_s* h, std::vector<Item>& items, uint64_t* sp
         for (uint32_t rs = 0; rs < nsplit; ++rs) {
             const uint32_t ro = r0 + rs * kTableCap;
             const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
-            for (uint64_t so = s0; so < s1; so += chunk) {
+            const uint64_t sn = s1 - s0;
+            const uint32_t np = probe_pieces(sn, chunk);
+            for (uint32_t k = 0; k < np; ++k) {
                 Item it = {};
                 it.r_off = ro;
                 it.r_cnt = rc;
-                it.s_off = (uint32_t)so;
-                it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
+                it.s_off = s0 + (uint32_t)(sn * k / np);
+                it.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
                 it.part = pr;
                 items.push_back(it);
             }
@@ -1171,12 +1183,14 @@ int plan_items_range(const pj_join_s* h, std::vector<Item>& items, uint64_t* spi
         for (uint32_t rs = 0; rs < nsplit; ++rs) {
             const uint32_t ro = r0 + rs * cap;
             const uint32_t rc = std::min<uint32_t>(cap, r1 - ro);
-            for (uint64_t so = s0; so < s1; so += chunk) {
+            const uint64_t sn = s1 - s0;
+            const uint32_t np = probe_pieces(sn, chunk);
+            for (uint32_t k = 0; k < np; ++k) {
                 Item it = {};
                 it.r_off = ro;
                 it.r_cnt = rc;
-                it.s_off = (uint32_t)so;
-                it.s_cnt = (uint32_t)std::min<uint64_t>(chunk, s1 - so);
+                it.s_off = s0 + (uint32_t)(sn * k / np);
+                it.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
                 it.part = f;
                 it.fmt = fmt;
                 it.lo = lo;

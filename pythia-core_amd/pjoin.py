@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libpjoin.so")
+# PJOIN_LIB selects another build of the library (default: the in-tree one).
+LIB_PATH = os.environ.get("PJOIN_LIB") or os.path.join(HERE, "lib", "libpjoin.so")
 
 PJ_OK = 0
 PJ_ERR_INVALID = -1

@@ -73,6 +73,26 @@ enum : uint32_t {
 constexpr uint32_t kCodeCap = 12928;    // records per code table (12 B each + bucket directory)
 constexpr uint32_t kDenseCap = 20096;   // records per dense table (8 B each + presence bitmap)
 
+// A range build partition's LDS table, for work items derived on the device.
+struct PlanPart {
+    uint32_t fmt, cap;      // kFmtCodes or kFmtDense; records a table holds
+    uint64_t lo, hi, base;  // ordered-key range and smallest ordered key
+};
+
+// The work items of a join, derived on the device (no item list): offs[p]
+// is the first item of build partition p and offs[FR] the item count. Item
+// q of partition p joins table split q / pieces of its build records with
+// piece q % pieces of its probe partition.
+struct PlanView {
+    const uint32_t* offs;        // FR + 1
+    const uint32_t* rst;         // FR + 1 build partition starts
+    const uint32_t* sst;         // probe partition starts
+    const uint32_t* coarse_of;   // range plans: probe partition of each build partition
+    const PlanPart* pp;          // range plans: table of each build partition (null: radix)
+    uint64_t chunk;              // probe records a piece, about
+    uint32_t FR, shift;          // build partitions; radix: probe partition = p >> shift
+};
+
 struct Item {           // one join work item
     uint32_t r_off, r_cnt;   // build records [r_off, r_off + r_cnt) of R_part
     uint32_t s_off, s_cnt;   // probe records of S_part

@@ -464,6 +464,110 @@ __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint3
     __syncthreads();
 }
 
+// ---------------------------------------------------------------- work items
+// The work list is not materialised: k_plan counts each build partition's
+// items (table splits x probe pieces) and scans the counts, and k_join maps
+// an item number back to its partition by binary search over the offsets.
+
+// Pieces the s probe records of one build partition are cut into: about
+// `chunk` records each and all of one size, so that no item is a small
+// remainder that builds a whole table for a few probes.
+__host__ __device__ __forceinline__ uint32_t probe_pieces(uint64_t s, uint64_t chunk)
+{
+    const uint64_t n = (s + chunk / 2) / chunk;
+    return n ? (uint32_t)n : 1u;
+}
+
+__device__ __forceinline__ uint32_t plan_probe_part(const PlanView& v, uint32_t p)
+{
+    return v.coarse_of ? v.coarse_of[p] : p >> v.shift;
+}
+
+// Items of build partition p; *splits gets its table count.
+__device__ __forceinline__ uint32_t plan_count(const PlanView& v, uint32_t p, uint32_t* splits)
+{
+    const uint32_t ps = plan_probe_part(v, p);
+    const uint32_t r = v.rst[p + 1] - v.rst[p], sn = v.sst[ps + 1] - v.sst[ps];
+    const uint32_t cap = v.pp ? v.pp[p].cap : kTableCap;
+    *splits = (r + cap - 1) / cap;
+    return r && sn ? *splits * probe_pieces(sn, v.chunk) : 0u;
+}
+
+// Work item `it` (< offs[FR]).
+__device__ __forceinline__ Item plan_item(const PlanView& v, uint32_t it)
+{
+    uint32_t lo = 0, hi = v.FR;   // the last p with offs[p] <= it
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (v.offs[mid] <= it)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint32_t p = lo, q = it - v.offs[p];
+    const uint32_t ps = plan_probe_part(v, p);
+    const uint32_t r0 = v.rst[p], r = v.rst[p + 1] - r0;
+    const uint32_t s0 = v.sst[ps];
+    const uint64_t sn = v.sst[ps + 1] - s0;
+    const uint32_t cap = v.pp ? v.pp[p].cap : kTableCap;
+    const uint32_t np = probe_pieces(sn, v.chunk);
+    const uint32_t rs = q / np, k = q % np;
+    Item I;
+    I.r_off = r0 + rs * cap;
+    I.r_cnt = r - rs * cap < cap ? r - rs * cap : cap;
+    I.s_off = s0 + (uint32_t)(sn * k / np);
+    I.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
+    I.part = p;
+    I.fmt = v.pp ? v.pp[p].fmt : 0u;
+    I.lo = v.pp ? v.pp[p].lo : 0ull;
+    I.hi = v.pp ? v.pp[p].hi : 0ull;
+    I.base = v.pp ? v.pp[p].base : 0ull;
+    return I;
+}
+
+constexpr uint32_t kPlanThreads = 1024;
+constexpr uint32_t kPlanPer = 16;   // build partitions a thread counts: up to 16384
+
+// One block: offs[p] = first item of build partition p, offs[FR] = items;
+// also clears the join's counters and records the item count and the number
+// of split tables in them (words 6 and 7).
+__global__ __launch_bounds__(kPlanThreads) void k_plan(PlanView v, uint32_t* __restrict__ offs,
+                                                        uint32_t* __restrict__ counters)
+{
+    __shared__ uint32_t wsum[kPlanThreads / 64 + 1];
+    const uint32_t per = (v.FR + kPlanThreads - 1) / kPlanThreads;
+    const uint32_t p0 = threadIdx.x * per;
+    uint32_t c[kPlanPer];
+    uint32_t sum = 0, spilled = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPlanPer; ++q) {
+        c[q] = 0;
+        if (q < per && p0 + q < v.FR) {
+            uint32_t splits;
+            c[q] = plan_count(v, p0 + q, &splits);
+            sum += c[q];
+            if (splits > 1 && c[q])
+                spilled += splits;
+        }
+    }
+    uint32_t total, stotal;
+    uint32_t pre = block_excl_scan<kPlanThreads>(sum, wsum, &total);
+#pragma unroll
+    for (uint32_t q = 0; q < kPlanPer; ++q)
+        if (q < per && p0 + q < v.FR) {
+            offs[p0 + q] = pre;
+            pre += c[q];
+        }
+    block_excl_scan<kPlanThreads>(spilled, wsum, &stotal);
+    if (threadIdx.x == 0) {
+        offs[v.FR] = total;
+        for (int w = 0; w < 6; ++w)
+            counters[w] = 0;
+        counters[6] = total;
+        counters[7] = stotal;
+    }
+}
+
 // Work items come from a shared counter; per item the block builds the LDS
 // table of its build records and streams the probe records past it.
 // RANGE = false: radix partitions, tables of 64-bit keys. RANGE = true:
@@ -473,8 +577,7 @@ __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint3
 // *fail_ctr (the caller then runs the join on radix partitions).
 template <bool RANGE>
 __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
-                                                        const Item* __restrict__ items, uint32_t nitems,
-                                                        uint32_t* __restrict__ item_ctr,
+                                                        PlanView plan, uint32_t* __restrict__ item_ctr,
                                                         uint32_t* __restrict__ out, uint64_t out_cap,
                                                         unsigned long long* __restrict__ out_cursor,
                                                         OutFormat fmt, uint32_t bitsR, uint32_t bkt_shift,
@@ -487,6 +590,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
     uint64_t* table = lds64 + kMiscWords / 2;
     const bool kbp = fmt.words == 6 && fmt.src[0] == SRC_BK0 && fmt.src[1] == SRC_BK1 && fmt.src[2] == SRC_BP0 &&
                      fmt.src[3] == SRC_BP1 && fmt.src[4] == SRC_PP0 && fmt.src[5] == SRC_PP1;
+    const uint32_t nitems = plan.offs[plan.FR];
 
     for (;;) {
         if (tid == 0) {
@@ -497,7 +601,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
         const uint32_t it = misc[0];
         if (it >= nitems)
             break;
-        const Item I = items[it];
+        const Item I = plan_item(plan, it);
         const uint4* Sp = S + I.s_off;
 
         if (!RANGE) {
@@ -810,14 +914,14 @@ struct pj_join_s {
     std::vector<uint8_t> staged;
     bool built = false;
     DevBuf d_raw_build, d_raw_probe, d_out_batch;
-    DevBuf rpart, spart, hist, bsum, starts, items, counters;
+    DevBuf rpart, spart, hist, bsum, rstarts, sstarts, planoffs, counters;
     Partitioned R, S;
     uint32_t bR = 0, bS = 0;  // radix bits of R and of S (bR >= bS)
     // range partitions chosen by the learned model
     bool range = false;
     RangePlan plan;
     LearnedCtx lctx;
-    DevBuf fine_bnd, coarse_bnd, pminmax, pids;
+    DevBuf fine_bnd, coarse_bnd, pminmax, pids, plan_parts, coarse_of;
     std::vector<uint64_t> pmin, pmax;   // per build partition, ordered keys
     const void* raw_build = nullptr;    // device build tuples of the current run (radix re-run)
     uint64_t raw_nbuild = 0;
@@ -1000,15 +1104,21 @@ uint32_t part_blocks(uint64_t n)
 // and a write-combining scatter takes each tuple's partition from the
 // histogram pass through h->pids instead of searching again.
 int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
-                   bool write_combine, DevBuf& dst, Partitioned& P, unsigned long long* pminmax, hipStream_t st)
+                   bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts, bool host_copy,
+                   unsigned long long* pminmax, hipStream_t st)
 {
     int rc;
     const uint32_t F = pf.F;
     P.F = F;
     P.n = n;
-    P.start.assign(F + 1, 0);
-    if (n == 0)
+    P.start.assign(host_copy ? F + 1 : 0, 0);
+    if ((rc = starts.ensure((F + 1) * 4)))
+        return rc;
+    if (n == 0) {   // every partition empty
+        if (!host_copy)
+            PJ_HIP(hipMemsetAsync(starts.p, 0, (F + 1) * 4, st));
         return PJ_OK;
+    }
     if (n >= (1ull << 31))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
     const uint32_t bb = bounds_lds_bytes(pf, range);
@@ -1023,8 +1133,7 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     const uint64_t cap = n + (uint64_t)F * nblk * (B - 1);
     if (cap >= (1ull << 32))
         return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");
-    if ((rc = dst.ensure(cap * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)) ||
-        (rc = h->starts.ensure((F + 1) * 4)))
+    if ((rc = dst.ensure(cap * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)))
         return rc;
     const uint32_t* in = (const uint32_t*)d_in;
     const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
@@ -1051,7 +1160,7 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
     hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)bsum);
     hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk,
-                       (const uint32_t*)(bsum + nsb), (uint32_t*)h->starts.p);
+                       (const uint32_t*)(bsum + nsb), (uint32_t*)starts.p);
     PJ_HIP(hipGetLastError());
     uint4* out = (uint4*)dst.p;
     switch (B) {
@@ -1065,7 +1174,8 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     }
     if (rc)
         return rc;
-    PJ_HIP(hipMemcpyAsync(P.start.data(), h->starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (host_copy)
+        PJ_HIP(hipMemcpyAsync(P.start.data(), starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
     return PJ_OK;
 }
 
@@ -1091,53 +1201,6 @@ uint64_t probe_chunk(uint64_t ns)
     return std::max<uint64_t>((ns + slices - 1) / slices, 32768);
 }
 
-// Work items the s probe records of one build partition are cut into: about
-// `chunk` records each and all of one size, so that no item is a small
-// remainder that builds a whole table for a few probes.
-uint32_t probe_pieces(uint64_t s, uint64_t chunk)
-{
-    return (uint32_t)std::max<uint64_t>(1, (s + chunk / 2) / chunk);
-}
-
-// Builds the radix work list: for every build partition (a sub-partition of
-// a probe partition when R uses more radix bits than S), its records (split
-// by position when they exceed one LDS table) times chunks of the probe
-// partition's records.
-void plan_items_radix(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
-{
-    items.clear();
-    *spilled = 0;
-    const uint32_t FR = 1u << h->bR, shift = h->bR - h->bS;
-    const uint64_t chunk = probe_chunk(h->S.start.back());
-    for (uint32_t pr = 0; pr < FR; ++pr) {
-        const uint32_t ps = pr >> shift;
-        const uint32_t s0 = h->S.start[ps], s1 = h->S.start[ps + 1];
-        const uint32_t r0 = h->R.start[pr], r1 = h->R.start[pr + 1];
-        if (s1 == s0 || r1 == r0)
-            continue;
-        const uint32_t nsplit = (r1 - r0 + kTableCap - 1) / kTableCap;
-        if (nsplit > 1)
-            *spilled += nsplit;
-        for (uint32_t rs = 0; rs < nsplit; ++rs) {
-            const uint32_t ro = r0 + rs * kTableCap;
-            const uint32_t rc = std::min<uint32_t>(kTableCap, r1 - ro);
-            const uint64_t sn = s1 - s0;
-            const uint32_t np = probe_pieces(sn, chunk);
-            for (uint32_t k = 0; k < np; ++k) {
-                Item it = {};
-                it.r_off = ro;
-                it.r_cnt = rc;
-                it.s_off = s0 + (uint32_t)(sn * k / np);
-                it.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
-                it.part = pr;
-                items.push_back(it);
-            }
-        }
-    }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const Item& a, const Item& b) { return a.s_cnt > b.s_cnt; });
-}
-
 // Table format of range build partition f: dense when it holds exactly
 // the keys base..base+cnt-1 (checked for repeats in the kernel), code
 // tables (split into chunks when larger than one) when its keys span less
@@ -1155,54 +1218,6 @@ bool range_format(const pj_join_s* h, uint32_t f, uint32_t cnt, uint32_t* fmt)
     }
     *fmt = kFmtCodes;
     return hi - lo < (1ull << 32);
-}
-
-// Range work list: build partition f (inside probe partition coarse_of[f])
-// times chunks of that probe partition.
-int plan_items_range(const pj_join_s* h, std::vector<Item>& items, uint64_t* spilled)
-{
-    items.clear();
-    *spilled = 0;
-    const RangePlan& pl = h->plan;
-    const uint64_t chunk = probe_chunk(h->S.start.back());
-    for (uint32_t f = 0; f < pl.nfine; ++f) {
-        const uint32_t q = pl.coarse_of[f];
-        const uint32_t s0 = h->S.start[q], s1 = h->S.start[q + 1];
-        const uint32_t r0 = h->R.start[f], r1 = h->R.start[f + 1];
-        if (s1 == s0 || r1 == r0)
-            continue;
-        uint32_t fmt;
-        if (!range_format(h, f, r1 - r0, &fmt))
-            return fail(PJ_ERR_UNSUPPORTED, "range partition %u spans 2^32 keys or more", f);
-        const uint32_t cap = fmt == kFmtDense ? kDenseCap : kCodeCap;
-        const uint32_t nsplit = (r1 - r0 + cap - 1) / cap;
-        if (nsplit > 1)
-            *spilled += nsplit;
-        const uint64_t lo = f ? pl.fine[f - 1] : 0ull;
-        const uint64_t hi = f + 1 < pl.nfine ? pl.fine[f] - 1 : ~0ull;
-        for (uint32_t rs = 0; rs < nsplit; ++rs) {
-            const uint32_t ro = r0 + rs * cap;
-            const uint32_t rc = std::min<uint32_t>(cap, r1 - ro);
-            const uint64_t sn = s1 - s0;
-            const uint32_t np = probe_pieces(sn, chunk);
-            for (uint32_t k = 0; k < np; ++k) {
-                Item it = {};
-                it.r_off = ro;
-                it.r_cnt = rc;
-                it.s_off = s0 + (uint32_t)(sn * k / np);
-                it.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
-                it.part = f;
-                it.fmt = fmt;
-                it.lo = lo;
-                it.hi = hi;
-                it.base = h->pmin[f];
-                items.push_back(it);
-            }
-        }
-    }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const Item& a, const Item& b) { return a.s_cnt > b.s_cnt; });
-    return PJ_OK;
 }
 
 hipStream_t as_stream(void* s) { return (hipStream_t)s; }
@@ -1227,7 +1242,8 @@ int prepare_radix(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     h->bR = choose_build_bits(nb, h->desc.radix_bits);
     h->bS = std::min<uint32_t>(h->bR, 11);
     const PartFn pf = radix_fn(h->bR);
-    return partition_side(h, d_build, nb, h->lay.build, pf, false, false, h->rpart, h->R, nullptr, st);
+    return partition_side(h, d_build, nb, h->lay.build, pf, false, false, h->rpart, h->R, h->rstarts, false, nullptr,
+                          st);
 }
 
 // Range partitions of the build side from the learned model. Leaves
@@ -1259,7 +1275,7 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     if (!pl.coarse.empty())
         PJ_HIP(hipMemcpyAsync(h->coarse_bnd.p, pl.coarse.data(), pl.coarse.size() * 8, hipMemcpyHostToDevice, st));
     const PartFn pf = range_fn(pl.nfine, (const uint64_t*)h->fine_bnd.p);
-    if ((rc = partition_side(h, d_build, nb, h->lay.build, pf, true, false, h->rpart, h->R,
+    if ((rc = partition_side(h, d_build, nb, h->lay.build, pf, true, false, h->rpart, h->R, h->rstarts, true,
                              (unsigned long long*)h->pminmax.p, st)))
         return rc;
     h->pmin.resize(pl.nfine);
@@ -1268,6 +1284,8 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     PJ_HIP(hipMemcpyAsync(h->pmax.data(), (char*)h->pminmax.p + (size_t)pl.nfine * 8, (size_t)pl.nfine * 8,
                           hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
+    // each build partition's table, for the work items derived on the device
+    std::vector<PlanPart> parts(pl.nfine);
     for (uint32_t f = 0; f < pl.nfine; ++f) {
         uint32_t fmt;
         if (!range_format(h, f, h->R.start[f + 1] - h->R.start[f], &fmt)) {
@@ -1275,7 +1293,19 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
             h->plan.why = "a build partition spans 2^32 keys or more";
             return PJ_OK;
         }
+        PlanPart& q = parts[f];
+        q.fmt = fmt;
+        q.cap = fmt == kFmtDense ? kDenseCap : kCodeCap;
+        q.lo = f ? pl.fine[f - 1] : 0ull;
+        q.hi = f + 1 < pl.nfine ? pl.fine[f] - 1 : ~0ull;
+        q.base = h->pmin[f];
     }
+    if ((rc = h->plan_parts.ensure(parts.size() * sizeof(PlanPart))) ||
+        (rc = h->coarse_of.ensure((size_t)pl.nfine * 4)))
+        return rc;
+    PJ_HIP(hipMemcpyAsync(h->plan_parts.p, parts.data(), parts.size() * sizeof(PlanPart), hipMemcpyHostToDevice, st));
+    PJ_HIP(hipMemcpyAsync(h->coarse_of.p, pl.coarse_of.data(), (size_t)pl.nfine * 4, hipMemcpyHostToDevice, st));
+    PJ_HIP(hipStreamSynchronize(st));   // the host arrays are locals
     h->range = true;
     return PJ_OK;
 }
@@ -1319,59 +1349,61 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     record(h, 2, st);
     if (h->range) {
         const PartFn pf = range_fn(h->plan.ncoarse, (const uint64_t*)h->coarse_bnd.p);
-        rc = partition_side(h, d_probe, np, h->lay.probe, pf, true, true, h->spart, h->S, nullptr, st);
+        rc = partition_side(h, d_probe, np, h->lay.probe, pf, true, true, h->spart, h->S, h->sstarts, false, nullptr,
+                            st);
     } else {
         const PartFn pf = radix_fn(h->bS);
-        rc = partition_side(h, d_probe, np, h->lay.probe, pf, false, true, h->spart, h->S, nullptr, st);
+        rc = partition_side(h, d_probe, np, h->lay.probe, pf, false, true, h->spart, h->S, h->sstarts, false, nullptr,
+                            st);
     }
     if (rc)
         return rc;
     record(h, 3, st);
-    PJ_HIP(hipStreamSynchronize(st));  // partition starts are on the host now
     if (h->R.n == 0 || np == 0) {
+        PJ_HIP(hipStreamSynchronize(st));
         memset(&h->last, 0, sizeof(h->last));
         return PJ_OK;
     }
-    std::vector<Item> items;
-    uint64_t spilled = 0;
-    if (h->range) {
-        if ((rc = plan_items_range(h, items, &spilled)))
-            return rc;
-    } else {
-        plan_items_radix(h, items, &spilled);
-    }
-    const uint32_t nitems = (uint32_t)items.size();
-    if ((rc = h->items.ensure(std::max<size_t>(nitems, 1) * sizeof(Item))) || (rc = h->counters.ensure(64)))
+    // the work items, counted on the device from the partition starts
+    PlanView v = {};
+    v.FR = h->range ? h->plan.nfine : 1u << h->bR;
+    v.shift = h->range ? 0u : h->bR - h->bS;
+    v.rst = (const uint32_t*)h->rstarts.p;
+    v.sst = (const uint32_t*)h->sstarts.p;
+    v.coarse_of = h->range ? (const uint32_t*)h->coarse_of.p : nullptr;
+    v.pp = h->range ? (const PlanPart*)h->plan_parts.p : nullptr;
+    v.chunk = probe_chunk(np);
+    if (v.FR > kPlanThreads * kPlanPer)
+        return fail(PJ_ERR_UNSUPPORTED, "%u build partitions exceed the planner", v.FR);
+    if ((rc = h->planoffs.ensure((size_t)(v.FR + 1) * 4)) || (rc = h->counters.ensure(64)))
         return rc;
-    if (nitems)
-        PJ_HIP(hipMemcpyAsync(h->items.p, items.data(), nitems * sizeof(Item), hipMemcpyHostToDevice, st));
-    PJ_HIP(hipMemsetAsync(h->counters.p, 0, 64, st));
-    uint32_t* item_ctr = (uint32_t*)h->counters.p;
-    unsigned long long* cursor = (unsigned long long*)((char*)h->counters.p + 8);
-    uint32_t* fail_ctr = (uint32_t*)((char*)h->counters.p + 16);
+    v.offs = (const uint32_t*)h->planoffs.p;
+    uint32_t* counters = (uint32_t*)h->counters.p;
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, st, v, (uint32_t*)h->planoffs.p, counters);
+    uint32_t* item_ctr = counters;
+    unsigned long long* cursor = (unsigned long long*)(counters + 2);
+    uint32_t* fail_ctr = counters + 4;
     const size_t lds = join_lds_bytes(h->range);
     const void* kfn = h->range ? (const void*)k_join<true> : (const void*)k_join<false>;
     PJ_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const uint32_t bkt_shift = 64 - h->bR - 12;
     record(h, 4, st);
-    if (nitems) {
-        const uint32_t grid = std::min<uint32_t>(nitems, (uint32_t)device_count_cus());
-        if (h->range)
-            hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                               (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr,
-                               (uint32_t*)d_out, out_cap, cursor, h->lay.out, 0u, 0u, fail_ctr);
-        else
-            hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                               (const uint4*)h->spart.p, (const Item*)h->items.p, nitems, item_ctr,
-                               (uint32_t*)d_out, out_cap, cursor, h->lay.out, h->bR, bkt_shift, fail_ctr);
-        PJ_HIP(hipGetLastError());
-    }
+    const uint32_t grid = (uint32_t)device_count_cus();   // blocks without an item exit at once
+    if (h->range)
+        hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)d_out, out_cap, cursor, h->lay.out, 0u,
+                           0u, fail_ctr);
+    else
+        hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)d_out, out_cap, cursor, h->lay.out, h->bR,
+                           bkt_shift, fail_ctr);
+    PJ_HIP(hipGetLastError());
     record(h, 5, st);
-    unsigned long long produced[3] = {0, 0, 0};
-    PJ_HIP(hipMemcpyAsync(produced, h->counters.p, 24, hipMemcpyDeviceToHost, st));
+    uint32_t produced[8] = {};
+    PJ_HIP(hipMemcpyAsync(produced, counters, 32, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
-    *out_n = produced[1];
-    *fails = (uint32_t)produced[2];
+    *out_n = (uint64_t)produced[2] | ((uint64_t)produced[3] << 32);
+    *fails = produced[4];
     pj_timings& t = h->last;
     t.build_partition_ms = elapsed(h, 0, 1) - h->model_ms;
     t.probe_partition_ms = elapsed(h, 2, 3);
@@ -1380,9 +1412,9 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     t.total_ms = elapsed(h, 0, 5);
     t.partitions = h->range ? h->plan.ncoarse : 1u << h->bS;
     t.subpartitions = h->range ? (h->plan.nfine + h->plan.ncoarse - 1) / h->plan.ncoarse : 1u << (h->bR - h->bS);
-    t.work_items = nitems;
+    t.work_items = produced[6];
     t.used_learned = h->range ? 1u : 0u;
-    t.spilled_items = spilled;
+    t.spilled_items = produced[7];
     return PJ_OK;
 }
 
@@ -1461,7 +1493,8 @@ int pj_join_destroy(pj_join_t h)
     if (!h)
         return PJ_OK;
     for (DevBuf* b : {&h->d_raw_build, &h->d_raw_probe, &h->d_out_batch, &h->rpart, &h->spart, &h->hist, &h->bsum,
-                      &h->starts, &h->items, &h->counters})
+                      &h->rstarts, &h->sstarts, &h->planoffs, &h->counters, &h->fine_bnd, &h->coarse_bnd,
+                      &h->pminmax, &h->pids, &h->plan_parts, &h->coarse_of})
         b->release();
     if (h->ev_ok)
         for (int i = 0; i < 8; ++i)

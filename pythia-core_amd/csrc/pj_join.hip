@@ -1156,11 +1156,16 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
                : launch_hist<false>(ident, in, n, L, pf, chunk, nblk, B, hist, nullptr, nullptr, nullptr, st);
     if (rc)
         return rc;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
-    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)bsum);
-    hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk,
-                       (const uint32_t*)(bsum + nsb), (uint32_t*)starts.p);
+    if (m <= kScanSmall) {   // one launch for small histograms
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanSmallThreads), 0, st, hist, (uint32_t)m, F, nblk,
+                           (uint32_t*)starts.p);
+    } else {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
+        hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, m, (const uint32_t*)bsum);
+        hipLaunchKernelGGL(k_part_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, hist, F, nblk,
+                           (const uint32_t*)(bsum + nsb), (uint32_t*)starts.p);
+    }
     PJ_HIP(hipGetLastError());
     uint4* out = (uint4*)dst.p;
     switch (B) {

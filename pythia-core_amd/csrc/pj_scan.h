@@ -139,5 +139,39 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     }
 }
 
+// The whole exclusive scan of a[0..m) (m <= kScanSmall) by one block, then
+// the partition starts of a histogram laid out a[p * nblk + blk]:
+// part_start[p] = a[p * nblk], part_start[F] = the total. Replaces the
+// three-kernel scan and k_part_bounds for small histograms.
+constexpr uint32_t kScanSmallThreads = 1024;
+constexpr uint32_t kScanSmallPer = 16;
+constexpr uint32_t kScanSmall = kScanSmallThreads * kScanSmallPer;
+
+__global__ __launch_bounds__(kScanSmallThreads) void k_scan_small(uint32_t* __restrict__ a, uint32_t m, uint32_t F,
+                                                                   uint32_t nblk, uint32_t* __restrict__ part_start)
+{
+    __shared__ uint32_t wsum[kScanSmallThreads / 64 + 1];
+    const uint32_t per = (m + kScanSmallThreads - 1) / kScanSmallThreads;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t v[kScanSmallPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanSmallPer; ++k) {
+        v[k] = k < per && b + k < m ? a[b + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t pre = block_excl_scan<kScanSmallThreads>(sum, wsum, &total);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanSmallPer; ++k)
+        if (k < per && b + k < m) {
+            a[b + k] = pre;
+            pre += v[k];
+        }
+    __syncthreads();   // the scanned entries of other threads are read below
+    for (uint32_t p = threadIdx.x; p <= F; p += kScanSmallThreads)
+        part_start[p] = p < F ? a[(uint64_t)p * nblk] : total;
+}
+
 }  // namespace
 }  // namespace pj

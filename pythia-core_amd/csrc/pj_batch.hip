@@ -438,6 +438,7 @@ struct pj_batch_s {
     // sequential mode: the joins run one by one through pj_join_run_device,
     // one join handle per radix-bits value
     std::map<uint32_t, pj_join_t> seq;
+    std::vector<hipEvent_t> seq_ev;   // two a join, reused across runs
     bool last_sequential = false;
     std::vector<uint64_t> seq_counts;
     std::vector<double> seq_done_us, seq_run_us;
@@ -709,7 +710,9 @@ uint32_t seq_bits(const pj_batch_s* h, uint64_t nb)
 }
 
 // Runs joins one by one, each a whole pj_join_run_device (partition both
-// sides, join) with `bits` radix bits, timing each with HIP events.
+// sides, join) with `bits` radix bits. HIP events around each join give its
+// time and its completion time within the batch; they are read once, after
+// the last join.
 int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
                    hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
                    std::vector<double>& run_us)
@@ -718,34 +721,37 @@ int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const
     counts.assign(nj, 0);
     done_us.assign(nj, 0.0);
     run_us.assign(nj, 0.0);
-    hipEvent_t e0, a, b;
-    PJ_HIP(hipEventCreate(&e0));
-    PJ_HIP(hipEventCreate(&a));
-    PJ_HIP(hipEventCreate(&b));
+    while (h->seq_ev.size() < 2 * nj + 1) {
+        hipEvent_t e;
+        PJ_HIP(hipEventCreate(&e));
+        h->seq_ev.push_back(e);
+    }
     int rc = PJ_OK;
-    (void)hipEventRecord(e0, st);
+    size_t ran = 0;
+    (void)hipEventRecord(h->seq_ev[0], st);
     for (size_t i = 0; i < nj && rc == PJ_OK; ++i) {
         const pj_batch_join& q = joins[i];
         pj_join_t jh;
         if ((rc = seq_handle(h, bits[i], &jh)))
             break;
-        (void)hipEventRecord(a, st);
+        (void)hipEventRecord(h->seq_ev[2 * i + 1], st);
         uint64_t n = 0;
         rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, st);
         if (rc == PJ_ERR_CAPACITY)   // reported per join by the results
             rc = PJ_OK;
-        (void)hipEventRecord(b, st);
-        (void)hipEventSynchronize(b);
-        float ms = 0.f, t = 0.f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        (void)hipEventElapsedTime(&t, e0, b);
+        (void)hipEventRecord(h->seq_ev[2 * i + 2], st);
         counts[i] = n;
+        ran = i + 1;
+    }
+    if (ran)
+        (void)hipEventSynchronize(h->seq_ev[2 * ran]);
+    for (size_t i = 0; i < ran; ++i) {
+        float ms = 0.f, t = 0.f;
+        (void)hipEventElapsedTime(&ms, h->seq_ev[2 * i + 1], h->seq_ev[2 * i + 2]);
+        (void)hipEventElapsedTime(&t, h->seq_ev[0], h->seq_ev[2 * i + 2]);
         run_us[i] = ms * 1e3;
         done_us[i] = t * 1e3;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     return rc;
 }
 
@@ -781,6 +787,8 @@ int pj_batch_destroy(pj_batch_t h)
         return PJ_OK;
     for (auto& kv : h->seq)
         pj_join_destroy(kv.second);
+    for (hipEvent_t e : h->seq_ev)
+        (void)hipEventDestroy(e);
     if (h->exec)
         (void)hipGraphExecDestroy(h->exec);
     if (h->graph)

@@ -981,14 +981,14 @@ uint32_t wc_split(uint32_t F, uint32_t bnd_bytes)
     return F > 1 && wc_chunk(F, bnd_bytes) < 8 && wc_chunk((F + 1) / 2, bnd_bytes) >= 8 ? 2u : 1u;
 }
 
+// Blocks of a tile-sorted partition pass: about 64K tuples each for large
+// relations, and for small ones enough blocks of at least 2K tuples to
+// spread over the CUs (one block would do a 64K-tuple relation alone).
 uint32_t hist_blocks(uint64_t n)
 {
-    uint64_t b = (n + 65535) / 65536;  // ~64K tuples per block
-    if (b < 1)
-        b = 1;
-    if (b > 2048)
-        b = 2048;
-    return (uint32_t)b;
+    uint64_t b = (n + 65535) / 65536;
+    b = std::max<uint64_t>(b, std::min<uint64_t>((n + 2047) / 2048, 256));
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(b, 1), 2048);
 }
 
 // A key whose top bit is set: its partition is never 0 (for bits >= 1), so

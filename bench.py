@@ -22,7 +22,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one k_join launch moves on each workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy
 # (profiles/k_join_uniform_16Mx256M.md).
-JOIN_TRAFFIC_BYTES = {"uniform_16Mx256M": 11.645e9}
+JOIN_TRAFFIC_BYTES = {"uniform_16Mx256M": 11.122e9}
 
 WORKLOADS = {
     # name: (seed, key domain, |R|, |S|, misses)

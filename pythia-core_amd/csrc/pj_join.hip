@@ -19,6 +19,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -929,6 +930,7 @@ struct pj_join_s {
     pj_timings last;
     hipEvent_t ev[8];
     bool ev_ok = false;
+    uint32_t* host_ctr = nullptr;   // pinned: the join's counters, read back after each probe
 };
 
 constexpr uint32_t kMaxBits = 14;
@@ -1000,6 +1002,28 @@ uint64_t pad_key_nonzero_part()
             return k;
 }
 
+// Lets kernel fn use `bytes` of dynamic LDS. hipFuncSetAttribute costs
+// microseconds of host time and a small join launches about ten kernels, so
+// the largest size set so far is remembered per device and kernel.
+int allow_lds(const void* fn, size_t bytes)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> done;
+    int dev = 0;
+    PJ_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = done.find({dev, fn});
+        if (it != done.end() && it->second >= bytes)
+            return PJ_OK;
+    }
+    PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    std::lock_guard<std::mutex> g(mu);
+    size_t& v = done[{dev, fn}];
+    v = std::max(v, bytes);
+    return PJ_OK;
+}
+
 template <int B, bool RANGE, int SPLIT>
 int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
                 uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
@@ -1020,13 +1044,13 @@ int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
     }
     const dim3 grid(nblk * SPLIT);
     if (ident) {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<true, B, RANGE, SPLIT>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (int rc = allow_lds((const void*)k_scatter_wc<true, B, RANGE, SPLIT>, lds))
+            return rc;
         hipLaunchKernelGGL((k_scatter_wc<true, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
                            chunk, nblk, offs, out, pad_other, pad_zero, pid);
     } else {
-        PJ_HIP(hipFuncSetAttribute((const void*)k_scatter_wc<false, B, RANGE, SPLIT>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (int rc = allow_lds((const void*)k_scatter_wc<false, B, RANGE, SPLIT>, lds))
+            return rc;
         hipLaunchKernelGGL((k_scatter_wc<false, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
                            chunk, nblk, offs, out, pad_other, pad_zero, pid);
     }
@@ -1055,7 +1079,8 @@ int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
     if (lds > kLdsBytes)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the histogram's LDS budget", pf.F);
     const void* fn = ident ? (const void*)k_hist<true, RANGE> : (const void*)k_hist<false, RANGE>;
-    PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (int rc = allow_lds(fn, lds))
+        return rc;
     if (ident)
         hipLaunchKernelGGL((k_hist<true, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk, nblk,
                            pad, hist, pmin, pmax, pid);
@@ -1077,7 +1102,8 @@ int launch_tile_scatter(bool ident, const uint32_t* in, uint64_t n, const SideLa
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the scatter's LDS budget", pf.F);
     const size_t lds = (size_t)ts * (RANGE ? 18 : 16) + bb + (2 * pf.F + 1) * 4 + 16 * 4;
     const void* fn = ident ? (const void*)k_scatter<true, RANGE> : (const void*)k_scatter<false, RANGE>;
-    PJ_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (int rc = allow_lds(fn, lds))
+        return rc;
     if (ident)
         hipLaunchKernelGGL((k_scatter<true, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf, chunk,
                            nblk, offs, out, ts, pid);
@@ -1390,7 +1416,8 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     uint32_t* fail_ctr = counters + 4;
     const size_t lds = join_lds_bytes(h->range);
     const void* kfn = h->range ? (const void*)k_join<true> : (const void*)k_join<false>;
-    PJ_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if ((rc = allow_lds(kfn, lds)))
+        return rc;
     const uint32_t bkt_shift = 64 - h->bR - 12;
     record(h, 4, st);
     const uint32_t grid = (uint32_t)device_count_cus();   // blocks without an item exit at once
@@ -1404,7 +1431,9 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
                            bkt_shift, fail_ctr);
     PJ_HIP(hipGetLastError());
     record(h, 5, st);
-    uint32_t produced[8] = {};
+    if (!h->host_ctr)
+        PJ_HIP(hipHostMalloc((void**)&h->host_ctr, 64, hipHostMallocDefault));
+    uint32_t* produced = h->host_ctr;
     PJ_HIP(hipMemcpyAsync(produced, counters, 32, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
     *out_n = (uint64_t)produced[2] | ((uint64_t)produced[3] << 32);
@@ -1504,6 +1533,8 @@ int pj_join_destroy(pj_join_t h)
     if (h->ev_ok)
         for (int i = 0; i < 8; ++i)
             (void)hipEventDestroy(h->ev[i]);
+    if (h->host_ctr)
+        (void)hipHostFree(h->host_ctr);
     delete h;
     return PJ_OK;
 }

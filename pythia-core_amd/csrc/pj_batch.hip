@@ -438,7 +438,10 @@ struct pj_batch_s {
     // sequential mode: the joins run one by one through pj_join_run_device,
     // one join handle per radix-bits value
     std::map<uint32_t, pj_join_t> seq;
+    std::map<uint32_t, std::pair<uint64_t, uint64_t>> seq_warm;   // largest (build, probe) each handle has run
     std::vector<hipEvent_t> seq_ev;   // two a join, reused across runs
+    uint32_t* seq_slots = nullptr;    // pinned: 8 counter words a join
+    size_t seq_nslots = 0;
     bool last_sequential = false;
     std::vector<uint64_t> seq_counts;
     std::vector<double> seq_done_us, seq_run_us;
@@ -709,10 +712,13 @@ uint32_t seq_bits(const pj_batch_s* h, uint64_t nb)
     return h->tuned[c] ? h->cfg[c].bits : choose_build_bits(nb, -1);
 }
 
-// Runs joins one by one, each a whole pj_join_run_device (partition both
-// sides, join) with `bits` radix bits. HIP events around each join give its
-// time and its completion time within the batch; they are read once, after
-// the last join.
+// Runs joins one after another, each a whole radix join (partition both
+// sides, join) with `bits` radix bits, enqueued on st without waiting in
+// between: each join's counters land in its own pinned slot and HIP events
+// around it give its time and its completion time within the batch, all
+// read once after the last join. Before that, each handle runs the largest
+// build and probe sides it will meet, so that no buffer grows (and waits
+// for the device) inside the sequence.
 int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
                    hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
                    std::vector<double>& run_us)
@@ -726,7 +732,46 @@ int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const
         PJ_HIP(hipEventCreate(&e));
         h->seq_ev.push_back(e);
     }
-    int rc = PJ_OK;
+    if (h->seq_nslots < nj) {
+        if (h->seq_slots)
+            PJ_HIP(hipHostFree(h->seq_slots));
+        h->seq_slots = nullptr;
+        h->seq_nslots = 0;
+        PJ_HIP(hipHostMalloc((void**)&h->seq_slots, nj * 8 * sizeof(uint32_t), hipHostMallocDefault));
+        h->seq_nslots = nj;
+    }
+    int rc;
+    // warm-up: the largest build side and the largest probe side of each handle
+    std::map<uint32_t, std::pair<size_t, size_t>> big;   // bits -> (join with most build, most probe tuples)
+    for (size_t i = 0; i < nj; ++i) {
+        auto it = big.find(bits[i]);
+        if (it == big.end()) {
+            big[bits[i]] = {i, i};
+            continue;
+        }
+        if (joins[i].nbuild > joins[it->second.first].nbuild)
+            it->second.first = i;
+        if (joins[i].nprobe > joins[it->second.second].nprobe)
+            it->second.second = i;
+    }
+    for (const auto& kv : big) {
+        pj_join_t jh;
+        if ((rc = seq_handle(h, kv.first, &jh)))
+            return rc;
+        auto& w = h->seq_warm[kv.first];
+        for (size_t i : {kv.second.first, kv.second.second}) {
+            const pj_batch_join& q = joins[i];
+            if (q.nbuild <= w.first && q.nprobe <= w.second)
+                continue;
+            uint64_t n = 0;
+            rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, st);
+            if (rc && rc != PJ_ERR_CAPACITY)
+                return rc;
+            w.first = std::max<uint64_t>(w.first, q.nbuild);
+            w.second = std::max<uint64_t>(w.second, q.nprobe);
+        }
+    }
+    rc = PJ_OK;
     size_t ran = 0;
     (void)hipEventRecord(h->seq_ev[0], st);
     for (size_t i = 0; i < nj && rc == PJ_OK; ++i) {
@@ -735,17 +780,15 @@ int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const
         if ((rc = seq_handle(h, bits[i], &jh)))
             break;
         (void)hipEventRecord(h->seq_ev[2 * i + 1], st);
-        uint64_t n = 0;
-        rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, st);
-        if (rc == PJ_ERR_CAPACITY)   // reported per join by the results
-            rc = PJ_OK;
+        rc = join_enqueue(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, h->seq_slots + 8 * i, st);
         (void)hipEventRecord(h->seq_ev[2 * i + 2], st);
-        counts[i] = n;
         ran = i + 1;
     }
     if (ran)
-        (void)hipEventSynchronize(h->seq_ev[2 * ran]);
+        PJ_HIP(hipStreamSynchronize(st));
     for (size_t i = 0; i < ran; ++i) {
+        const uint32_t* c = h->seq_slots + 8 * i;
+        counts[i] = (uint64_t)c[2] | ((uint64_t)c[3] << 32);
         float ms = 0.f, t = 0.f;
         (void)hipEventElapsedTime(&ms, h->seq_ev[2 * i + 1], h->seq_ev[2 * i + 2]);
         (void)hipEventElapsedTime(&t, h->seq_ev[0], h->seq_ev[2 * i + 2]);
@@ -789,6 +832,8 @@ int pj_batch_destroy(pj_batch_t h)
         pj_join_destroy(kv.second);
     for (hipEvent_t e : h->seq_ev)
         (void)hipEventDestroy(e);
+    if (h->seq_slots)
+        (void)hipHostFree(h->seq_slots);
     if (h->exec)
         (void)hipGraphExecDestroy(h->exec);
     if (h->graph)

@@ -148,6 +148,15 @@ struct RangePlan {
 // LDS table (pj_join.hip).
 uint32_t choose_build_bits(uint64_t nr, int32_t forced);
 
+// Enqueues a whole radix join (handle h, PJ_PART_RADIX) on st without
+// waiting for it. Its counters land in slot, 8 words of pinned host memory,
+// when the stream gets there: rows produced in words 2-3 (low, high), work
+// items in word 6, split tables in word 7 (pj_join.hip). The handle's device
+// buffers are reused by the next join enqueued on it, in stream order; they
+// grow (which waits for the device) when a join needs more than any before.
+int join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void* d_probe, uint64_t np, void* d_out,
+                 uint64_t out_cap, uint32_t* slot, hipStream_t st);
+
 // Device state of the learned partitioner (pj_learned.hip).
 struct LearnedCtx {
     DevBuf sample, tmp, hist, bsum, model, params, moments, resid, bounds, segs, loss;

@@ -1371,12 +1371,16 @@ int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     return rc;
 }
 
-int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
-               uint32_t* fails, hipStream_t st)
+// Enqueues one probe phase on st: partition S, count the work items, join,
+// and copy the join's counters into slot (8 words of pinned host memory:
+// [2..3] rows produced, [4] range tables that could not be held, [6] work
+// items, [7] split tables) when the stream reaches that copy. *empty: the
+// join has no work (the slot is zeroed at once).
+int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint32_t* slot,
+                  bool* empty, hipStream_t st)
 {
     int rc;
-    *out_n = 0;
-    *fails = 0;
+    *empty = false;
     record(h, 2, st);
     if (h->range) {
         const PartFn pf = range_fn(h->plan.ncoarse, (const uint64_t*)h->coarse_bnd.p);
@@ -1391,8 +1395,9 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
         return rc;
     record(h, 3, st);
     if (h->R.n == 0 || np == 0) {
-        PJ_HIP(hipStreamSynchronize(st));
-        memset(&h->last, 0, sizeof(h->last));
+        *empty = true;
+        for (int i = 0; i < 8; ++i)
+            slot[i] = 0;
         return PJ_OK;
     }
     // the work items, counted on the device from the partition starts
@@ -1431,11 +1436,27 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
                            bkt_shift, fail_ctr);
     PJ_HIP(hipGetLastError());
     record(h, 5, st);
+    PJ_HIP(hipMemcpyAsync(slot, counters, 32, hipMemcpyDeviceToHost, st));
+    return PJ_OK;
+}
+
+int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint64_t* out_n,
+               uint32_t* fails, hipStream_t st)
+{
+    *out_n = 0;
+    *fails = 0;
     if (!h->host_ctr)
         PJ_HIP(hipHostMalloc((void**)&h->host_ctr, 64, hipHostMallocDefault));
     uint32_t* produced = h->host_ctr;
-    PJ_HIP(hipMemcpyAsync(produced, counters, 32, hipMemcpyDeviceToHost, st));
+    bool empty = false;
+    int rc = probe_enqueue(h, d_probe, np, d_out, out_cap, produced, &empty, st);
+    if (rc)
+        return rc;
     PJ_HIP(hipStreamSynchronize(st));
+    if (empty) {
+        memset(&h->last, 0, sizeof(h->last));
+        return PJ_OK;
+    }
     *out_n = (uint64_t)produced[2] | ((uint64_t)produced[3] << 32);
     *fails = produced[4];
     pj_timings& t = h->last;
@@ -1478,6 +1499,23 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
 }
 
 }  // namespace
+
+int pj::join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void* d_probe, uint64_t np, void* d_out,
+                     uint64_t out_cap, uint32_t* slot, hipStream_t st)
+{
+    if (!h || !slot || (nb && !d_build) || (np && !d_probe) || (out_cap && !d_out))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (out_cap && ((uintptr_t)d_out % 16))
+        return fail(PJ_ERR_INVALID, "the output buffer must be 16-byte aligned");
+    if (h->desc.partitioner != PJ_PART_RADIX)
+        return fail(PJ_ERR_INVALID, "enqueued joins use radix partitions");
+    std::lock_guard<std::mutex> g(h->mu);
+    int rc = prepare_build(h, d_build, nb, st);
+    if (rc)
+        return rc;
+    bool empty;
+    return probe_enqueue(h, d_probe, np, d_out, out_cap, slot, &empty, st);
+}
 
 // ======================================================================
 // C ABI

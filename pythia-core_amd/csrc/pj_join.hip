@@ -192,9 +192,14 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
-            v[k] = i < end;
             r[k] = nxt[k];
-            p[k] = nxp[k];
+            p[k] = 0;
+            v[k] = false;
+            if (i < end) {   // partition (a split block keeps only its half's partitions)
+                const uint32_t pp = have_pid ? nxp[k] : part_of<RANGE>(rec_key(r[k]), pf, bnd);
+                v[k] = SPLIT == 1 || (pp >= FL) == (half != 0);
+                p[k] = pp - p0;
+            }
             const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
             if (i2 < end) {
                 nxt[k] = load_rec<IDENT>(in, i2, L);
@@ -202,21 +207,13 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                     nxp[k] = pid[i2];
             }
         }
-        // claim (a split block keeps only its half's partitions)
+        // claim
 #pragma unroll
         for (int k = 0; k < IT; ++k)
             if (v[k]) {
-                if (!have_pid)
-                    p[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
-                if (SPLIT > 1) {
-                    v[k] = (p[k] >= FL) == (half != 0);
-                    p[k] -= p0;
-                }
-                if (v[k]) {
-                    s[k] = atomicAdd(&cnt[p[k]], 1u);
-                    if (s[k] < (uint32_t)B)
-                        buf[p[k] * B + s[k]] = r[k];
-                }
+                s[k] = atomicAdd(&cnt[p[k]], 1u);
+                if (s[k] < (uint32_t)B)
+                    buf[p[k] * B + s[k]] = r[k];
             }
         __syncthreads();
         // full buffers leave; whole chunks of overflow go straight out

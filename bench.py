@@ -54,9 +54,12 @@ def _say(msg):
 
 
 def bench_batch(args):
-    """The batch workload: every join through the single-join path in turn
-    (one launch sequence per join), radix bits tuned per size class. One step
-    is the whole batch; value is aggregate build + probe tuples/s."""
+    """The batch workload: every join a whole radix join with the radix bits
+    tuned for its size class. value: the joins spread over four streams
+    (PJ_BATCH_CONCURRENT); beside it the same joins one after another
+    (PJ_BATCH_SEQUENTIAL, one launch sequence per join) and with untuned
+    bits. One step is the whole batch; value is aggregate build + probe
+    tuples/s. Per-join latency is each join's own device time."""
     import numpy as np
     import torch
     import pjoin as P
@@ -88,29 +91,32 @@ def bench_batch(args):
     desc = P.make_desc(["long", "int"], ["long", "int"], [0], [0], ["B$0", "B$1", "P$1"])
     bt = P.Batch(desc)
     bt.prepare(joins)
+    tuples = sum(5 * nb for nb in sizes)
 
-    def timed(steps):
+    def timed(mode, steps):
+        for _ in range(args.warmup):
+            bt.run(mode, stream)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            bt.run(P.Batch.SEQUENTIAL, stream)
+            bt.run(mode, stream)
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / steps
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        own = bt.run_times()
+        return {"ms_per_step": ms, "value": tuples / (ms * 1e-3), "latency_p50_us": float(np.percentile(own, 50)),
+                "latency_p99_us": float(np.percentile(own, 99)), "latency_max_us": float(own.max())}
 
-    for _ in range(args.warmup):
-        bt.run(P.Batch.SEQUENTIAL, stream)
-    ms_default = timed(max(1, args.steps // 2))
-    _, done_default = bt.results()
-    _say("batch: default bits %.2f ms/step" % ms_default)
+    untuned = timed(P.Batch.SEQUENTIAL, max(1, args.steps // 2))
+    _say("batch: sequential, untuned bits %.2f ms/step" % untuned["ms_per_step"])
     t0 = time.perf_counter()
     bt.tune(joins, trial=4, stream=stream)
     tune_s = time.perf_counter() - t0
     bt.prepare(joins)
-    for _ in range(args.warmup):
-        bt.run(P.Batch.SEQUENTIAL, stream)
-    ms = timed(args.steps)
+    seq = timed(P.Batch.SEQUENTIAL, args.steps)
+    _say("batch: sequential %.2f ms/step (tuning %.1f s)" % (seq["ms_per_step"], tune_s))
+    conc = timed(P.Batch.CONCURRENT, args.steps)
     cnt, done = bt.results()
-    _say("batch: tuned bits %.2f ms/step (tuning %.1f s)" % (ms, tune_s))
+    _say("batch: concurrent %.2f ms/step" % conc["ms_per_step"])
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -121,10 +127,8 @@ def bench_batch(args):
         bad += (int(cnt[j]), s_) != (ec, es)
         if j % 128 == 127:
             _say("batch: parity checked for %d joins" % (j + 1))
-    lat = np.diff(np.concatenate([[0.0], done]))
-    lat_d = np.diff(np.concatenate([[0.0], done_default]))
-    tuples = sum(5 * nb for nb in sizes)
     out_rows = int(cnt.sum())
+    ms = conc["ms_per_step"]
     b_alg = sum(nb * 12 + 4 * nb * 12 for nb in sizes) + out_rows * 16
     achieved = b_alg / (ms * 1e-3) / 1e9
     tuned = {}
@@ -133,21 +137,20 @@ def bench_batch(args):
             tuned[c] = bt.config(c)[0]
     res = {
         "metric": "aggregate join tuples/s (build+probe), 512 int64-key joins, build 1K-4M log-uniform, probe 4x; "
-                  "one launch sequence per join, tuned radix bits",
-        "value": tuples / (ms * 1e-3),
+                  "joins spread over 4 streams, tuned radix bits",
+        "value": conc["value"],
         "unit": "tuples/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {"workload": args.workload, "joins": nj, "build_min": nmin, "build_max": nmax,
                    "build_tuples": sum(sizes), "probe_tuples": 4 * sum(sizes), "tuple_bytes": 12,
-                   "out_tuple_bytes": 16, "out_tuples": out_rows, "mode": "sequential",
+                   "out_tuple_bytes": 16, "out_tuples": out_rows, "mode": "concurrent, 4 streams",
                    "tuned_bits_by_size_class": tuned, "pipeline": "switched off"},
-        "latency_us": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                       "max": float(lat.max()), "note": "per join, one launch sequence per join, tuned bits"},
-        "default_bits": {"ms_per_step": ms_default, "value": tuples / (ms_default * 1e-3),
-                         "latency_p50_us": float(np.percentile(lat_d, 50)),
-                         "latency_p99_us": float(np.percentile(lat_d, 99))},
+        "latency_us": {"p50": conc["latency_p50_us"], "p99": conc["latency_p99_us"], "max": conc["latency_max_us"],
+                       "note": "per join, its own device time on its stream"},
+        "one_launch_per_join": seq,
+        "untuned_bits_one_launch_per_join": untuned,
         "tune_seconds": tune_s,
         "parity": {"joins": nj, "mismatched_joins": bad},
         "roofline": {"bound": "hbm", "kernel": "whole batch (all kernels of every join)", "achieved": achieved,
@@ -160,65 +163,6 @@ def bench_batch(args):
     print(json.dumps(res))
     if bad:
         sys.exit(1)
-
-
-def cpu_baseline(seed, dom, nb, npr, nmiss):
-    """Times the reference HashJoinOp (oracle/_ref) when it was built, else
-    the oracle's single-threaded restatement, on a bounded sample: the whole
-    build relation and the first probe tuples of the same probe relation."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import oracle as O
-
-    sample = min(npr, 1 << 25)
-    b, p = O.gen_pkfk(seed, dom, nb, npr, nmiss, probe_prefix=sample)
-    what = "|R|=%d build + first %d of %d probe tuples" % (nb, sample, npr)
-    import reference_run as R
-    ref = R if R.available() else None
-    if ref is not None:
-        threads = ref.default_threads()
-        r = ref.run_pkfk(b, p, threads)
-        return {"value": (nb + sample) / r["seconds"], "unit": "tuples/s", "cores": threads,
-                "kind": "reference", "sample": what + "; reference HashJoinOp built -O3 from its sources, "
-                "under MergeOp, modulo hash over next_pow2(|R|) buckets of 2 tuples, 1 MiB pages, %d threads"
-                % threads, "seconds": r["seconds"], "build_seconds": r["build_seconds"],
-                "out_tuples": r["out_tuples"]}
-    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                    buckets=1 << max(1, (nb - 1).bit_length()), tuplesperbucket=2)
-    t0 = time.perf_counter()
-    out, st = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
-    dt = time.perf_counter() - t0
-    return {"value": (nb + sample) / dt, "unit": "tuples/s", "cores": 1, "kind": "port",
-            "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
-
-
-def cpu_baseline_zipf(seed, nb, npr, cdf):
-    """The reference HashJoinOp (bytes hash over the two key columns, as
-    queryhashjoincomposite.cpp runs it) on the whole build side and the first
-    probe tuples; the oracle restatement when the reference build is absent."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import oracle as O
-    import reference_run as R
-    sample = min(npr, 1 << 24)
-    b, p = O.gen_pairs_zipf(seed, nb, sample, cdf)
-    cols = ["int", "int", "long"]
-    proj = ["B$0", "B$1", "B$2", "P$2"]
-    buckets = 1 << max(1, (nb - 1).bit_length())
-    what = "|R|=%d build + %d Zipf probe tuples" % (nb, sample)
-    if R.available():
-        threads = R.default_threads()
-        rows, tb, tt = R.run_join(cols, cols, [0, 1], [0, 1], proj, b, p, threads=threads, fn="bytes",
-                                  buckets=buckets, tuplesperbucket=2)
-        return {"value": (nb + sample) / tt, "unit": "tuples/s", "cores": threads, "kind": "reference",
-                "sample": what + "; reference HashJoinOp, bytes hash, %d threads" % threads, "seconds": tt,
-                "out_tuples": int(rows.shape[0])}
-    d = O.join_desc(cols, cols, [0, 1], [0, 1], proj, fn="bytes", buckets=buckets)
-    t0 = time.perf_counter()
-    out, st = O.hashjoin(d, b, p)
-    dt = time.perf_counter() - t0
-    return {"value": (nb + sample) / dt, "unit": "tuples/s", "cores": 1, "kind": "port",
-            "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
 
 
 def cpu_baseline_batch(seed, sizes):

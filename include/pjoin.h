@@ -295,21 +295,27 @@ int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uin
 /* Plans the batch and captures its pipeline into a graph. */
 int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, void* stream);
 /* Runs the prepared batch on stream: the pipeline's kernels launched one by
- * one (PJ_BATCH_PLAIN), as one graph launch (PJ_BATCH_GRAPH), or every join
- * through pj_join_run_device in turn with the radix bits of its size class
- * (PJ_BATCH_SEQUENTIAL, the one-launch-per-join baseline). This build ships
- * the pipeline switched off: PJ_BATCH_PLAIN and PJ_BATCH_GRAPH return
- * PJ_ERR_UNSUPPORTED, and pj_batch_tune then tunes the sequential mode's
- * radix bits. */
+ * one (PJ_BATCH_PLAIN), as one graph launch (PJ_BATCH_GRAPH), every join as
+ * a whole radix join in turn with the radix bits of its size class
+ * (PJ_BATCH_SEQUENTIAL, the one-launch-per-join baseline), or the same
+ * joins spread round-robin over four internal streams that start when
+ * stream reaches the batch and that stream waits for (PJ_BATCH_CONCURRENT).
+ * This build ships the pipeline switched off: PJ_BATCH_PLAIN and
+ * PJ_BATCH_GRAPH return PJ_ERR_UNSUPPORTED, and pj_batch_tune then tunes the
+ * radix bits of the other two modes. */
 #define PJ_BATCH_PLAIN 0
 #define PJ_BATCH_GRAPH 1
 #define PJ_BATCH_SEQUENTIAL 2
+#define PJ_BATCH_CONCURRENT 3   /* as SEQUENTIAL, joins spread over 4 streams */
 int pj_batch_run(pj_batch_t h, int mode, void* stream);
 /* After a run: rows each join produced, and (done_us, optional) the time
  * from the batch's start until each join's last work item finished (device
  * clock for the pipeline, HIP events for PJ_BATCH_SEQUENTIAL).
  * PJ_ERR_CAPACITY when a join's output overflowed. */
 int pj_batch_results(pj_batch_t h, uint64_t* out_counts, double* done_us, void* stream);
+/* After a PJ_BATCH_SEQUENTIAL or PJ_BATCH_CONCURRENT run: each join's own
+ * device time in microseconds, from its start on its stream to its end. */
+int pj_batch_run_times(pj_batch_t h, double* run_us);
 
 /* Batch workload: join j of a batch of njoins has build tuples (int64 key,
  * int32 payload) with unique keys (a seeded permutation of a uniform

@@ -46,6 +46,7 @@ constexpr uint32_t kBMinBits = 4, kBMaxBits = 12;
 constexpr uint32_t kBChunk = 65536;       // probe records per work item at most
 constexpr uint32_t kBScatterTile = 4096;  // records per LDS-sorted scatter round
 constexpr uint32_t kClasses = 32;
+constexpr uint32_t kLanes = 4;            // streams of the concurrent mode (4 hardware queues a process)
 
 struct BJoin {                 // one join of the batch (device copy)
     const uint32_t* in[2];     // raw build / probe tuples
@@ -437,8 +438,11 @@ struct pj_batch_s {
     bool enabled = false;
     // sequential mode: the joins run one by one through pj_join_run_device,
     // one join handle per radix-bits value
-    std::map<uint32_t, pj_join_t> seq;
-    std::map<uint32_t, std::pair<uint64_t, uint64_t>> seq_warm;   // largest (build, probe) each handle has run
+    // one join handle per (radix bits, lane): the lanes of the concurrent
+    // mode each run on a stream of their own, with buffers of their own
+    std::map<std::pair<uint32_t, uint32_t>, pj_join_t> seq;
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint64_t, uint64_t>> seq_warm;   // largest (build, probe) run
+    std::vector<hipStream_t> lanes;   // created for the concurrent mode, default priority
     std::vector<hipEvent_t> seq_ev;   // two a join, reused across runs
     uint32_t* seq_slots = nullptr;    // pinned: 8 counter words a join
     size_t seq_nslots = 0;
@@ -686,9 +690,9 @@ std::vector<pj_batch_cfg> cfgs_for(pj_batch_s* h, const std::vector<pj_batch_joi
     return c;
 }
 
-int seq_handle(pj_batch_s* h, uint32_t bits, pj_join_t* out)
+int seq_handle(pj_batch_s* h, uint32_t bits, uint32_t lane, pj_join_t* out)
 {
-    auto it = h->seq.find(bits);
+    auto it = h->seq.find({bits, lane});
     if (it != h->seq.end()) {
         *out = it->second;
         return PJ_OK;
@@ -700,7 +704,7 @@ int seq_handle(pj_batch_s* h, uint32_t bits, pj_join_t* out)
     int rc = pj_join_create(&d, out);
     if (rc)
         return rc;
-    h->seq[bits] = *out;
+    h->seq[{bits, lane}] = *out;
     return PJ_OK;
 }
 
@@ -712,22 +716,30 @@ uint32_t seq_bits(const pj_batch_s* h, uint64_t nb)
     return h->tuned[c] ? h->cfg[c].bits : choose_build_bits(nb, -1);
 }
 
-// Runs joins one after another, each a whole radix join (partition both
-// sides, join) with `bits` radix bits, enqueued on st without waiting in
-// between: each join's counters land in its own pinned slot and HIP events
-// around it give its time and its completion time within the batch, all
-// read once after the last join. Before that, each handle runs the largest
-// build and probe sides it will meet, so that no buffer grows (and waits
-// for the device) inside the sequence.
-int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
-                   hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
-                   std::vector<double>& run_us)
+// Runs every join as a whole radix join (partition both sides, join) with
+// `bits` radix bits. With one lane the joins run one after another on st;
+// with more, join i runs on lane i % nlanes, each lane a stream of its own
+// that starts when st reaches the batch and that st waits for at the end.
+// Nothing waits between joins: each join's counters land in its own pinned
+// slot, and HIP events around each join give its own device time and its
+// completion time within the batch, all read after the last join. Before
+// that, every handle runs the largest build and probe sides it will meet,
+// so that no buffer grows (and waits for the device) inside the batch.
+int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+              uint32_t nlanes, hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
+              std::vector<double>& run_us)
 {
     const size_t nj = joins.size();
     counts.assign(nj, 0);
     done_us.assign(nj, 0.0);
     run_us.assign(nj, 0.0);
-    while (h->seq_ev.size() < 2 * nj + 1) {
+    while (h->lanes.size() < (nlanes > 1 ? nlanes : 0)) {
+        hipStream_t ls;
+        PJ_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
+        h->lanes.push_back(ls);
+    }
+    auto lane_stream = [&](uint32_t l) { return nlanes > 1 ? h->lanes[l] : st; };
+    while (h->seq_ev.size() < 2 * nj + 1 + nlanes) {
         hipEvent_t e;
         PJ_HIP(hipEventCreate(&e));
         h->seq_ev.push_back(e);
@@ -742,11 +754,12 @@ int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const
     }
     int rc;
     // warm-up: the largest build side and the largest probe side of each handle
-    std::map<uint32_t, std::pair<size_t, size_t>> big;   // bits -> (join with most build, most probe tuples)
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<size_t, size_t>> big;
     for (size_t i = 0; i < nj; ++i) {
-        auto it = big.find(bits[i]);
+        const std::pair<uint32_t, uint32_t> key{bits[i], (uint32_t)(i % nlanes)};
+        auto it = big.find(key);
         if (it == big.end()) {
-            big[bits[i]] = {i, i};
+            big[key] = {i, i};
             continue;
         }
         if (joins[i].nbuild > joins[it->second.first].nbuild)
@@ -754,44 +767,61 @@ int run_sequential(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const
         if (joins[i].nprobe > joins[it->second.second].nprobe)
             it->second.second = i;
     }
+    bool waited = false;
     for (const auto& kv : big) {
         pj_join_t jh;
-        if ((rc = seq_handle(h, kv.first, &jh)))
+        if ((rc = seq_handle(h, kv.first.first, kv.first.second, &jh)))
             return rc;
         auto& w = h->seq_warm[kv.first];
         for (size_t i : {kv.second.first, kv.second.second}) {
             const pj_batch_join& q = joins[i];
             if (q.nbuild <= w.first && q.nprobe <= w.second)
                 continue;
+            if (!waited) {   // the inputs may still be in flight on st
+                PJ_HIP(hipStreamSynchronize(st));
+                waited = true;
+            }
             uint64_t n = 0;
-            rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, st);
+            rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n,
+                                    lane_stream(kv.first.second));
             if (rc && rc != PJ_ERR_CAPACITY)
                 return rc;
             w.first = std::max<uint64_t>(w.first, q.nbuild);
             w.second = std::max<uint64_t>(w.second, q.nprobe);
         }
     }
+    hipEvent_t* ev = h->seq_ev.data();
+    hipEvent_t* lane_end = ev + 2 * nj + 1;
+    PJ_HIP(hipEventRecord(ev[0], st));
+    if (nlanes > 1)
+        for (uint32_t l = 0; l < nlanes; ++l)
+            PJ_HIP(hipStreamWaitEvent(h->lanes[l], ev[0], 0));
     rc = PJ_OK;
     size_t ran = 0;
-    (void)hipEventRecord(h->seq_ev[0], st);
     for (size_t i = 0; i < nj && rc == PJ_OK; ++i) {
         const pj_batch_join& q = joins[i];
+        const uint32_t l = (uint32_t)(i % nlanes);
+        const hipStream_t ls = lane_stream(l);
         pj_join_t jh;
-        if ((rc = seq_handle(h, bits[i], &jh)))
+        if ((rc = seq_handle(h, bits[i], l, &jh)))
             break;
-        (void)hipEventRecord(h->seq_ev[2 * i + 1], st);
-        rc = join_enqueue(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, h->seq_slots + 8 * i, st);
-        (void)hipEventRecord(h->seq_ev[2 * i + 2], st);
+        (void)hipEventRecord(ev[2 * i + 1], ls);
+        rc = join_enqueue(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, h->seq_slots + 8 * i, ls);
+        (void)hipEventRecord(ev[2 * i + 2], ls);
         ran = i + 1;
     }
-    if (ran)
-        PJ_HIP(hipStreamSynchronize(st));
+    if (nlanes > 1)
+        for (uint32_t l = 0; l < nlanes; ++l) {
+            PJ_HIP(hipEventRecord(lane_end[l], h->lanes[l]));
+            PJ_HIP(hipStreamWaitEvent(st, lane_end[l], 0));
+        }
+    PJ_HIP(hipStreamSynchronize(st));
     for (size_t i = 0; i < ran; ++i) {
         const uint32_t* c = h->seq_slots + 8 * i;
         counts[i] = (uint64_t)c[2] | ((uint64_t)c[3] << 32);
         float ms = 0.f, t = 0.f;
-        (void)hipEventElapsedTime(&ms, h->seq_ev[2 * i + 1], h->seq_ev[2 * i + 2]);
-        (void)hipEventElapsedTime(&t, h->seq_ev[0], h->seq_ev[2 * i + 2]);
+        (void)hipEventElapsedTime(&ms, ev[2 * i + 1], ev[2 * i + 2]);
+        (void)hipEventElapsedTime(&t, ev[0], ev[2 * i + 2]);
         run_us[i] = ms * 1e3;
         done_us[i] = t * 1e3;
     }
@@ -830,6 +860,8 @@ int pj_batch_destroy(pj_batch_t h)
         return PJ_OK;
     for (auto& kv : h->seq)
         pj_join_destroy(kv.second);
+    for (hipStream_t ls : h->lanes)
+        (void)hipStreamDestroy(ls);
     for (hipEvent_t e : h->seq_ev)
         (void)hipEventDestroy(e);
     if (h->seq_slots)
@@ -887,17 +919,18 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
 
 int pj_batch_run(pj_batch_t h, int mode, void* stream)
 {
-    if (!h || mode < PJ_BATCH_PLAIN || mode > PJ_BATCH_SEQUENTIAL)
+    if (!h || mode < PJ_BATCH_PLAIN || mode > PJ_BATCH_CONCURRENT)
         return fail(PJ_ERR_INVALID, "null handle or unknown mode");
     std::lock_guard<std::mutex> g(h->mu);
     if (!h->prepared)
         return fail(PJ_ERR_STATE, "pj_batch_run before pj_batch_prepare");
     hipStream_t st = (hipStream_t)stream;
-    if (mode == PJ_BATCH_SEQUENTIAL) {
+    if (mode == PJ_BATCH_SEQUENTIAL || mode == PJ_BATCH_CONCURRENT) {
         std::vector<uint32_t> bits(h->joins.size());
         for (size_t i = 0; i < bits.size(); ++i)
             bits[i] = seq_bits(h, h->joins[i].nbuild);
-        const int rc = run_sequential(h, h->joins, bits, st, h->seq_counts, h->seq_done_us, h->seq_run_us);
+        const uint32_t nlanes = mode == PJ_BATCH_CONCURRENT ? kLanes : 1u;
+        const int rc = run_lanes(h, h->joins, bits, nlanes, st, h->seq_counts, h->seq_done_us, h->seq_run_us);
         h->last_sequential = rc == PJ_OK;
         return rc;
     }
@@ -957,6 +990,18 @@ int pj_batch_results(pj_batch_t h, uint64_t* out_counts, double* done_us, void* 
     return rc;
 }
 
+int pj_batch_run_times(pj_batch_t h, double* run_us)
+{
+    if (!h || !run_us)
+        return fail(PJ_ERR_INVALID, "null argument");
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->last_sequential || h->seq_run_us.size() != h->joins.size())
+        return fail(PJ_ERR_STATE, "no completed sequential or concurrent run");
+    for (size_t i = 0; i < h->seq_run_us.size(); ++i)
+        run_us[i] = h->seq_run_us[i];
+    return PJ_OK;
+}
+
 // Times each candidate configuration of every size class present in the
 // batch on up to `trial` of its joins (the batch's own buffers) and keeps
 // the fastest.
@@ -990,7 +1035,7 @@ int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uin
                 std::vector<double> done, run;
                 double t = 1e300;
                 for (int rep = 0; rep < 3; ++rep) {   // the first run warms up
-                    int rc = run_sequential(h, by[c], bits, st, cnt, done, run);
+                    int rc = run_lanes(h, by[c], bits, 1, st, cnt, done, run);
                     if (rc)
                         return rc;
                     if (rep)

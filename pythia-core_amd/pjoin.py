@@ -127,6 +127,7 @@ _SIGS = {
     "pj_batch_prepare": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchJoin), C.c_void_p]),
     "pj_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "pj_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pj_batch_run_times": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pj_batch_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     "pj_gen_batch_join": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),
@@ -273,7 +274,7 @@ class Batch:
         check(lib().pj_batch_prepare(self.h, len(joins), arr, C.c_void_p(stream)))
         self.n = len(joins)
 
-    PLAIN, GRAPH, SEQUENTIAL = 0, 1, 2
+    PLAIN, GRAPH, SEQUENTIAL, CONCURRENT = 0, 1, 2, 3
 
     def run(self, mode=GRAPH, stream=None):
         check(lib().pj_batch_run(self.h, mode, C.c_void_p(stream)))
@@ -285,6 +286,14 @@ class Batch:
         check(lib().pj_batch_results(self.h, cnt.ctypes.data_as(C.c_void_p), done.ctypes.data_as(C.c_void_p),
                                      C.c_void_p(stream)))
         return cnt, done
+
+    def run_times(self):
+        """Each join's own device time (us) in the last sequential or
+        concurrent run."""
+        import numpy as np
+        t = np.zeros(self.n, np.float64)
+        check(lib().pj_batch_run_times(self.h, t.ctypes.data_as(C.c_void_p)))
+        return t
 
     def close(self):
         if self.h:

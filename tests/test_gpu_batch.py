@@ -105,11 +105,12 @@ def test_batch_forced_configs(bits, passes, lf):
     bt.close()
 
 
-MODE = P.Batch.SEQUENTIAL
+MODES = pytest.mark.parametrize("mode", [P.Batch.SEQUENTIAL, P.Batch.CONCURRENT], ids=["sequential", "concurrent"])
 
 
-def test_batch_sequential_small_joins_repeat():
-    _say("case: batch of 40 joins, sequential mode, two runs")
+@MODES
+def test_batch_sequential_small_joins_repeat(mode):
+    _say("case: batch of 40 joins, mode %d, two runs" % mode)
     seed = 21
     sizes = list(P.batch_sizes(seed, 38, 1024, 1 << 16)) + [1, 3]
     joins, keep = _make(seed, sizes)
@@ -117,19 +118,23 @@ def test_batch_sequential_small_joins_repeat():
     bt.prepare(joins)
     prev = None
     for rep in range(2):
-        bt.run(MODE)
+        bt.run(mode)
         cnt, done = bt.results()
         _check(seed, joins, cnt)
-        assert np.all(done > 0) and np.all(np.diff(done) > 0)
+        own = bt.run_times()
+        assert np.all(done > 0) and np.all(own > 0) and np.all(own <= done + 1e-3)
+        if mode == P.Batch.SEQUENTIAL:
+            assert np.all(np.diff(done) > 0)
         if prev is not None:
             assert np.array_equal(cnt, prev)
         prev = cnt
     bt.close()
 
 
+@MODES
 @pytest.mark.parametrize("bits", [4, 9, 12])
-def test_batch_sequential_forced_bits(bits):
-    _say("case: batch sequential, bits=%d" % bits)
+def test_batch_sequential_forced_bits(bits, mode):
+    _say("case: batch mode %d, bits=%d" % (mode, bits))
     seed = 22
     sizes = P.batch_sizes(seed, 12, 1024, 1 << 20)
     joins, keep = _make(seed, sizes)
@@ -137,13 +142,39 @@ def test_batch_sequential_forced_bits(bits):
     for c in range(32):
         bt.set_config(c, bits, 1, 0.7)
     bt.prepare(joins)
-    bt.run(MODE)
+    bt.run(mode)
     cnt, _ = bt.results()
     _check(seed, joins, cnt)
     bt.close()
 
 
-def test_batch_tuner_choices_and_results():
+@MODES
+def test_batch_sequential_empty_sides(mode):
+    """Joins with no build tuples, no probe tuples or neither, between normal
+    ones: zero rows for the empty ones, the closed form for the others."""
+    _say("case: batch sequential, empty sides")
+    seed = 26
+    joins, keep = _make(seed, [3000, 0, 700, 5000])
+    b0, nb0, p0, np0, o0, c0 = joins[0]
+    b2, nb2, p2, np2, o2, c2 = joins[2]
+    joins[1] = (b2, 0, p0, np0, joins[1][4], 1)      # probe tuples, no build tuples
+    joins[2] = (b2, nb2, p2, 0, o2, c2)                 # build tuples, no probe tuples
+    joins.append((b0, 0, p0, 0, o0, 1))                 # neither
+    bt = P.Batch(P.make_desc(**DESC))
+    bt.prepare(joins)
+    bt.run(mode)
+    cnt, _ = bt.results()
+    assert [int(x) for x in cnt[[1, 2, 4]]] == [0, 0, 0]
+    for j in (0, 3):
+        b, nb, p, npr, o, cap = joins[j]
+        es, ec = O.expected_batch_join(seed, j, nb, npr, threads=16)
+        s_, _ = P.checksum_device(o, int(cnt[j]), 16)
+        assert (int(cnt[j]), s_) == (ec, es), j
+    bt.close()
+
+
+@MODES
+def test_batch_tuner_choices_and_results(mode):
     _say("case: batch tuner")
     seed = 23
     sizes = P.batch_sizes(seed, 24, 1024, 1 << 20)
@@ -155,13 +186,14 @@ def test_batch_tuner_choices_and_results():
         bits, passes, lf = bt.config(c)
         assert 4 <= bits <= 12 and passes in (1, 2) and 0.5 <= round(lf, 2) <= 0.9
     bt.prepare(joins)
-    bt.run(MODE)
+    bt.run(mode)
     cnt, _ = bt.results()
     _check(seed, joins, cnt)
     bt.close()
 
 
-def test_batch_repeated_keys_match_oracle():
+@MODES
+def test_batch_repeated_keys_match_oracle(mode):
     """Joins with repeated build keys (many-to-many) against HashJoinOp."""
     _say("case: batch, repeated keys")
     rng = np.random.default_rng(24)
@@ -185,7 +217,7 @@ def test_batch_repeated_keys_match_oracle():
         host.append(exp)
     bt = P.Batch(P.make_desc(**DESC))
     bt.prepare(joins)
-    bt.run(MODE)
+    bt.run(mode)
     cnt, _ = bt.results()
     for j, exp in enumerate(host):
         got = keep[3 * j + 2][: int(cnt[j]) * 16].cpu().numpy().reshape(-1, 16)
@@ -193,7 +225,8 @@ def test_batch_repeated_keys_match_oracle():
     bt.close()
 
 
-def test_batch_output_overflow_reported():
+@MODES
+def test_batch_output_overflow_reported(mode):
     _say("case: batch, output too small")
     seed = 25
     joins, keep = _make(seed, [5000, 6000])
@@ -201,7 +234,7 @@ def test_batch_output_overflow_reported():
     joins[1] = (b, nb, p, npr, o, 100)
     bt = P.Batch(P.make_desc(**DESC))
     bt.prepare(joins)
-    bt.run(MODE)
+    bt.run(mode)
     with pytest.raises(P.PJError) as e:
         bt.results()
     assert e.value.code == P.PJ_ERR_CAPACITY

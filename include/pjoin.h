@@ -300,13 +300,18 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
  * (PJ_BATCH_SEQUENTIAL, the one-launch-per-join baseline), or the same
  * joins spread round-robin over four internal streams that start when
  * stream reaches the batch and that stream waits for (PJ_BATCH_CONCURRENT).
- * This build ships the pipeline switched off: PJ_BATCH_PLAIN and
- * PJ_BATCH_GRAPH return PJ_ERR_UNSUPPORTED, and pj_batch_tune then tunes the
- * radix bits of the other two modes. */
+ * PJ_BATCH_CONCURRENT_GRAPH captures the concurrent mode's work (four
+ * parallel branches) into a graph at its first run after pj_batch_prepare
+ * or a change of radix bits, and then runs the batch as one graph launch;
+ * done_us then holds the whole batch's time for every join, and
+ * pj_batch_run_times zeros. This build ships the pipeline switched off:
+ * PJ_BATCH_PLAIN and PJ_BATCH_GRAPH return PJ_ERR_UNSUPPORTED, and
+ * pj_batch_tune then tunes the radix bits of the other modes. */
 #define PJ_BATCH_PLAIN 0
 #define PJ_BATCH_GRAPH 1
 #define PJ_BATCH_SEQUENTIAL 2
 #define PJ_BATCH_CONCURRENT 3   /* as SEQUENTIAL, joins spread over 4 streams */
+#define PJ_BATCH_CONCURRENT_GRAPH 4   /* CONCURRENT captured into one graph, one launch */
 int pj_batch_run(pj_batch_t h, int mode, void* stream);
 /* After a run: rows each join produced, and (done_us, optional) the time
  * from the batch's start until each join's last work item finished (device

@@ -443,6 +443,8 @@ struct pj_batch_s {
     std::map<std::pair<uint32_t, uint32_t>, pj_join_t> seq;
     std::map<std::pair<uint32_t, uint32_t>, std::pair<uint64_t, uint64_t>> seq_warm;   // largest (build, probe) run
     std::vector<hipStream_t> lanes;   // created for the concurrent mode, default priority
+    hipGraphExec_t lane_exec = nullptr;     // the concurrent mode captured (PJ_BATCH_CONCURRENT_GRAPH)
+    std::vector<uint32_t> lane_graph_bits;  // radix bits of each join in that capture
     std::vector<hipEvent_t> seq_ev;   // two a join, reused across runs
     uint32_t* seq_slots = nullptr;    // pinned: 8 counter words a join
     size_t seq_nslots = 0;
@@ -716,29 +718,30 @@ uint32_t seq_bits(const pj_batch_s* h, uint64_t nb)
     return h->tuned[c] ? h->cfg[c].bits : choose_build_bits(nb, -1);
 }
 
-// Runs every join as a whole radix join (partition both sides, join) with
-// `bits` radix bits. With one lane the joins run one after another on st;
-// with more, join i runs on lane i % nlanes, each lane a stream of its own
-// that starts when st reaches the batch and that st waits for at the end.
-// Nothing waits between joins: each join's counters land in its own pinned
-// slot, and HIP events around each join give its own device time and its
-// completion time within the batch, all read after the last join. Before
-// that, every handle runs the largest build and probe sides it will meet,
-// so that no buffer grows (and waits for the device) inside the batch.
-int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
-              uint32_t nlanes, hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
-              std::vector<double>& run_us)
+// Drops the captured lane graph. Its nodes hold the addresses of the join
+// handles' device buffers, which any other run or the tuner may grow.
+void drop_lane_graph(pj_batch_s* h)
+{
+    if (h->lane_exec) {
+        (void)hipGraphExecDestroy(h->lane_exec);
+        h->lane_exec = nullptr;
+    }
+    h->lane_graph_bits.clear();
+}
+
+// Sets up the per-join state of the lane modes: lane streams, events, one
+// pinned counter slot a join; then every handle (radix bits, lane) runs the
+// largest build and probe sides it will meet, so that no buffer grows (and
+// waits for the device, or allocates inside a graph capture) later.
+int prepare_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+                  uint32_t nlanes, hipStream_t st)
 {
     const size_t nj = joins.size();
-    counts.assign(nj, 0);
-    done_us.assign(nj, 0.0);
-    run_us.assign(nj, 0.0);
     while (h->lanes.size() < (nlanes > 1 ? nlanes : 0)) {
         hipStream_t ls;
         PJ_HIP(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking));
         h->lanes.push_back(ls);
     }
-    auto lane_stream = [&](uint32_t l) { return nlanes > 1 ? h->lanes[l] : st; };
     while (h->seq_ev.size() < 2 * nj + 1 + nlanes) {
         hipEvent_t e;
         PJ_HIP(hipEventCreate(&e));
@@ -753,8 +756,7 @@ int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std:
         h->seq_nslots = nj;
     }
     int rc;
-    // warm-up: the largest build side and the largest probe side of each handle
-    std::map<std::pair<uint32_t, uint32_t>, std::pair<size_t, size_t>> big;
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<size_t, size_t>> big;   // most build, most probe tuples
     for (size_t i = 0; i < nj; ++i) {
         const std::pair<uint32_t, uint32_t> key{bits[i], (uint32_t)(i % nlanes)};
         auto it = big.find(key);
@@ -773,6 +775,7 @@ int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std:
         if ((rc = seq_handle(h, kv.first.first, kv.first.second, &jh)))
             return rc;
         auto& w = h->seq_warm[kv.first];
+        const hipStream_t ls = nlanes > 1 ? h->lanes[kv.first.second] : st;
         for (size_t i : {kv.second.first, kv.second.second}) {
             const pj_batch_join& q = joins[i];
             if (q.nbuild <= w.first && q.nprobe <= w.second)
@@ -782,50 +785,144 @@ int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std:
                 waited = true;
             }
             uint64_t n = 0;
-            rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n,
-                                    lane_stream(kv.first.second));
+            rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, ls);
             if (rc && rc != PJ_ERR_CAPACITY)
                 return rc;
             w.first = std::max<uint64_t>(w.first, q.nbuild);
             w.second = std::max<uint64_t>(w.second, q.nprobe);
         }
     }
+    return PJ_OK;
+}
+
+// Enqueues every join as a whole radix join (partition both sides, join)
+// with `bits` radix bits, nothing waiting in between: with one lane one after
+// another on st; with more, join i on lane i % nlanes, each lane a stream of
+// its own that starts where st is and that st waits for at the end. Each
+// join's counters land in its own pinned slot. With `timed`, HIP events
+// around each join give its own device time and its completion time within
+// the batch (not inside a graph capture).
+int enqueue_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+                  uint32_t nlanes, hipStream_t st, bool timed, size_t* ran)
+{
+    const size_t nj = joins.size();
     hipEvent_t* ev = h->seq_ev.data();
     hipEvent_t* lane_end = ev + 2 * nj + 1;
     PJ_HIP(hipEventRecord(ev[0], st));
     if (nlanes > 1)
         for (uint32_t l = 0; l < nlanes; ++l)
             PJ_HIP(hipStreamWaitEvent(h->lanes[l], ev[0], 0));
-    rc = PJ_OK;
-    size_t ran = 0;
+    int rc = PJ_OK;
+    *ran = 0;
     for (size_t i = 0; i < nj && rc == PJ_OK; ++i) {
         const pj_batch_join& q = joins[i];
         const uint32_t l = (uint32_t)(i % nlanes);
-        const hipStream_t ls = lane_stream(l);
+        const hipStream_t ls = nlanes > 1 ? h->lanes[l] : st;
         pj_join_t jh;
         if ((rc = seq_handle(h, bits[i], l, &jh)))
             break;
-        (void)hipEventRecord(ev[2 * i + 1], ls);
+        if (timed)
+            (void)hipEventRecord(ev[2 * i + 1], ls);
         rc = join_enqueue(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, h->seq_slots + 8 * i, ls);
-        (void)hipEventRecord(ev[2 * i + 2], ls);
-        ran = i + 1;
+        if (timed)
+            (void)hipEventRecord(ev[2 * i + 2], ls);
+        *ran = i + 1;
     }
     if (nlanes > 1)
         for (uint32_t l = 0; l < nlanes; ++l) {
             PJ_HIP(hipEventRecord(lane_end[l], h->lanes[l]));
             PJ_HIP(hipStreamWaitEvent(st, lane_end[l], 0));
         }
-    PJ_HIP(hipStreamSynchronize(st));
+    return rc;
+}
+
+// Counts from the slots, and with `timed` the events' times.
+void collect_lanes(pj_batch_s* h, size_t nj, size_t ran, bool timed, std::vector<uint64_t>& counts,
+                   std::vector<double>& done_us, std::vector<double>& run_us)
+{
+    counts.assign(nj, 0);
+    done_us.assign(nj, 0.0);
+    run_us.assign(nj, 0.0);
+    hipEvent_t* ev = h->seq_ev.data();
     for (size_t i = 0; i < ran; ++i) {
         const uint32_t* c = h->seq_slots + 8 * i;
         counts[i] = (uint64_t)c[2] | ((uint64_t)c[3] << 32);
+        if (!timed)
+            continue;
         float ms = 0.f, t = 0.f;
         (void)hipEventElapsedTime(&ms, ev[2 * i + 1], ev[2 * i + 2]);
         (void)hipEventElapsedTime(&t, ev[0], ev[2 * i + 2]);
         run_us[i] = ms * 1e3;
         done_us[i] = t * 1e3;
     }
+}
+
+// The sequential and concurrent modes: prepare, enqueue, wait once, collect.
+int run_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+              uint32_t nlanes, hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
+              std::vector<double>& run_us)
+{
+    drop_lane_graph(h);
+    int rc = prepare_lanes(h, joins, bits, nlanes, st);
+    if (rc)
+        return rc;
+    size_t ran = 0;
+    rc = enqueue_lanes(h, joins, bits, nlanes, st, true, &ran);
+    PJ_HIP(hipStreamSynchronize(st));
+    collect_lanes(h, joins.size(), ran, true, counts, done_us, run_us);
     return rc;
+}
+
+// The concurrent mode captured into one graph (lanes as parallel branches),
+// captured again only when the joins or their radix bits change; one
+// hipGraphLaunch runs the batch. No per-join times: done_us is the whole
+// batch's time for every join.
+int run_lanes_graph(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const std::vector<uint32_t>& bits,
+                    hipStream_t st, std::vector<uint64_t>& counts, std::vector<double>& done_us,
+                    std::vector<double>& run_us)
+{
+    int rc;
+    if (!h->lane_exec || h->lane_graph_bits != bits) {
+        drop_lane_graph(h);
+        if ((rc = prepare_lanes(h, joins, bits, kLanes, st)))
+            return rc;
+        PJ_HIP(hipStreamSynchronize(st));
+        hipStream_t cs;
+        PJ_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        size_t ran = 0;
+        rc = PJ_OK;
+        if (e == hipSuccess) {
+            rc = enqueue_lanes(h, joins, bits, kLanes, cs, false, &ran);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(cs, &g);
+            if (rc == PJ_OK && e == hipSuccess)
+                e = hipGraphInstantiate(&h->lane_exec, g, nullptr, nullptr, 0);
+            if (g)
+                (void)hipGraphDestroy(g);
+        }
+        (void)hipStreamDestroy(cs);
+        if (rc)
+            return rc;
+        if (e != hipSuccess) {
+            h->lane_exec = nullptr;
+            return fail(PJ_ERR_DEVICE, "graph capture of the batch failed: %s", hipGetErrorString(e));
+        }
+        if (ran != joins.size())
+            return fail(PJ_ERR_DEVICE, "graph capture stopped after %zu joins", ran);
+        h->lane_graph_bits = bits;
+    }
+    hipEvent_t* ev = h->seq_ev.data();
+    PJ_HIP(hipEventRecord(ev[0], st));
+    PJ_HIP(hipGraphLaunch(h->lane_exec, st));
+    PJ_HIP(hipEventRecord(ev[1], st));
+    PJ_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+    collect_lanes(h, joins.size(), joins.size(), false, counts, done_us, run_us);
+    for (auto& d : done_us)
+        d = ms * 1e3;
+    return PJ_OK;
 }
 
 }  // namespace
@@ -860,6 +957,8 @@ int pj_batch_destroy(pj_batch_t h)
         return PJ_OK;
     for (auto& kv : h->seq)
         pj_join_destroy(kv.second);
+    if (h->lane_exec)
+        (void)hipGraphExecDestroy(h->lane_exec);
     for (hipStream_t ls : h->lanes)
         (void)hipStreamDestroy(ls);
     for (hipEvent_t e : h->seq_ev)
@@ -906,7 +1005,8 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
     std::vector<pj_batch_join> js(joins, joins + njoins);
     h->last_sequential = false;
     h->seq_counts.clear();
-    if (!h->enabled) {   // only the sequential mode runs: keep the list
+    drop_lane_graph(h);   // captured for the previous list
+    if (!h->enabled) {   // only the lane modes run: keep the list
         h->joins = js;
         h->prepared = true;
         return PJ_OK;
@@ -919,18 +1019,20 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
 
 int pj_batch_run(pj_batch_t h, int mode, void* stream)
 {
-    if (!h || mode < PJ_BATCH_PLAIN || mode > PJ_BATCH_CONCURRENT)
+    if (!h || mode < PJ_BATCH_PLAIN || mode > PJ_BATCH_CONCURRENT_GRAPH)
         return fail(PJ_ERR_INVALID, "null handle or unknown mode");
     std::lock_guard<std::mutex> g(h->mu);
     if (!h->prepared)
         return fail(PJ_ERR_STATE, "pj_batch_run before pj_batch_prepare");
     hipStream_t st = (hipStream_t)stream;
-    if (mode == PJ_BATCH_SEQUENTIAL || mode == PJ_BATCH_CONCURRENT) {
+    if (mode == PJ_BATCH_SEQUENTIAL || mode == PJ_BATCH_CONCURRENT || mode == PJ_BATCH_CONCURRENT_GRAPH) {
         std::vector<uint32_t> bits(h->joins.size());
         for (size_t i = 0; i < bits.size(); ++i)
             bits[i] = seq_bits(h, h->joins[i].nbuild);
-        const uint32_t nlanes = mode == PJ_BATCH_CONCURRENT ? kLanes : 1u;
-        const int rc = run_lanes(h, h->joins, bits, nlanes, st, h->seq_counts, h->seq_done_us, h->seq_run_us);
+        const uint32_t nlanes = mode == PJ_BATCH_SEQUENTIAL ? 1u : kLanes;
+        const int rc = mode == PJ_BATCH_CONCURRENT_GRAPH
+                           ? run_lanes_graph(h, h->joins, bits, st, h->seq_counts, h->seq_done_us, h->seq_run_us)
+                           : run_lanes(h, h->joins, bits, nlanes, st, h->seq_counts, h->seq_done_us, h->seq_run_us);
         h->last_sequential = rc == PJ_OK;
         return rc;
     }
@@ -1011,6 +1113,7 @@ int pj_batch_tune(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, uin
         return fail(PJ_ERR_INVALID, "bad arguments");
     hipStream_t st = (hipStream_t)stream;
     std::lock_guard<std::mutex> g(h->mu);
+    drop_lane_graph(h);   // the trial runs may grow the handles' buffers
     std::vector<std::vector<pj_batch_join>> by(kClasses);
     for (uint32_t i = 0; i < njoins; ++i) {
         const uint32_t c = size_class(joins[i].nbuild);

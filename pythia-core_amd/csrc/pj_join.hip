@@ -116,6 +116,15 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     }
 }
 
+// a[0..n) = v. A kernel rather than hipMemsetAsync, so that streams a
+// caller captures into a graph hold no memset nodes.
+__global__ void k_fill_u32(uint32_t* __restrict__ a, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
 // part_start[p] = first record of partition p; part_start[F] = total.
 __global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uint32_t nblk,
                               const uint32_t* __restrict__ total, uint32_t* __restrict__ part_start)
@@ -1138,8 +1147,10 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     if ((rc = starts.ensure((F + 1) * 4)))
         return rc;
     if (n == 0) {   // every partition empty
-        if (!host_copy)
-            PJ_HIP(hipMemsetAsync(starts.p, 0, (F + 1) * 4, st));
+        if (!host_copy) {
+            hipLaunchKernelGGL(k_fill_u32, dim3((F + 1 + 255) / 256), dim3(256), 0, st, (uint32_t*)starts.p, F + 1, 0u);
+            PJ_HIP(hipGetLastError());
+        }
         return PJ_OK;
     }
     if (n >= (1ull << 31))

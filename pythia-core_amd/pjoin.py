@@ -274,7 +274,7 @@ class Batch:
         check(lib().pj_batch_prepare(self.h, len(joins), arr, C.c_void_p(stream)))
         self.n = len(joins)
 
-    PLAIN, GRAPH, SEQUENTIAL, CONCURRENT = 0, 1, 2, 3
+    PLAIN, GRAPH, SEQUENTIAL, CONCURRENT, CONCURRENT_GRAPH = 0, 1, 2, 3, 4
 
     def run(self, mode=GRAPH, stream=None):
         check(lib().pj_batch_run(self.h, mode, C.c_void_p(stream)))

@@ -131,6 +131,37 @@ def test_batch_sequential_small_joins_repeat(mode):
     bt.close()
 
 
+def test_batch_concurrent_graph_replays():
+    """The concurrent mode captured into one graph: three replays with eager
+    work in between (the outputs are overwritten before each replay), every
+    replay against the closed form; then an eager run and a fresh capture."""
+    _say("case: batch graph replays")
+    seed = 27
+    sizes = list(P.batch_sizes(seed, 37, 1024, 1 << 16)) + [1, 3, 0]
+    joins, keep = _make(seed, sizes)
+    bt = P.Batch(P.make_desc(**DESC))
+    bt.prepare(joins)
+    for rep in range(3):
+        for j in range(len(joins)):
+            keep[3 * j + 2].fill_(0xAB)
+        x = torch.ones(1 << 20, device="cuda").sum().item()   # eager work between replays
+        assert x == float(1 << 20)
+        torch.cuda.synchronize()
+        bt.run(P.Batch.CONCURRENT_GRAPH)
+        cnt, done = bt.results()
+        _check(seed, joins, cnt)
+        assert np.all(done > 0)
+    bt.run(P.Batch.CONCURRENT)
+    cnt, _ = bt.results()
+    _check(seed, joins, cnt)
+    for j in range(len(joins)):
+        keep[3 * j + 2].fill_(0xCD)
+    bt.run(P.Batch.CONCURRENT_GRAPH)
+    cnt, _ = bt.results()
+    _check(seed, joins, cnt)
+    bt.close()
+
+
 @MODES
 @pytest.mark.parametrize("bits", [4, 9, 12])
 def test_batch_sequential_forced_bits(bits, mode):

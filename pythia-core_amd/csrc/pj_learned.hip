@@ -515,7 +515,14 @@ uint32_t grid_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); 
 
 // Root: segments of the sorted sample and their leaves. Returns false when
 // the sample has more segments than the root can hold.
-int build_root(const FitBufs& f, uint64_t n, uint64_t n_keys, pj_model* M, bool* ok, hipStream_t st)
+// With too_many, the root first estimates the build partitions a plan
+// would need (each segment's keys at the LDS cost its sample density gives,
+// at the highest fill, plus one boundary in every gap) and stops with
+// *too_many set when they exceed kMaxFine by more than 10%: then neither the
+// fit nor the training runs (a large, evenly spread key domain is declined
+// this way).
+int build_root(const FitBufs& f, uint64_t n, uint64_t n_keys, pj_model* M, bool* ok, hipStream_t st,
+               bool* too_many = nullptr)
 {
     *ok = false;
     const uint32_t T = 256;
@@ -538,6 +545,20 @@ int build_root(const FitBufs& f, uint64_t n, uint64_t n_keys, pj_model* M, bool*
     PJ_HIP(hipMemcpyAsync(start.data(), f.seg_start, nseg * 4, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
     start[nseg] = (uint32_t)n;
+    if (too_many) {
+        const double scale = (double)n_keys / (double)n;   // keys per sample
+        double cost = 0.0;
+        for (uint32_t s = 0; s < nseg; ++s) {
+            const double keys = (double)(start[s + 1] - start[s]) * scale;
+            const double span = (double)(hi[s] - lo[s]) + 1.0;
+            cost += keys / (keys / span >= kDenseDensity ? (double)kDenseCap : (double)kCodeCap);
+        }
+        // 10% headroom: the plan itself counts keys from the trained slopes,
+        // which can come out a little below these sample counts
+        *too_many = std::ceil(cost / kFillMax) + (double)(nseg - 1) > 1.1 * (double)kMaxFine;
+        if (*too_many)
+            return PJ_OK;
+    }
 
     // leaves in proportion to sample counts, at least one per segment, one
     // per kLeafSamples samples in all (so a window holds enough samples)
@@ -599,12 +620,12 @@ int train_step(const FitBufs& f, uint32_t L, int cur, float eta, float* grad, hi
 // Sorts the sample in f.sorted, builds the root, fits the leaves and runs
 // `steps` training steps; M (host) receives the model.
 int fit_model(const FitBufs& f, uint64_t n, uint64_t n_keys, uint32_t steps, pj_model* M, bool* ok,
-              hipStream_t st)
+              hipStream_t st, bool* too_many = nullptr)
 {
     int rc;
     if ((rc = sort_keys(f.sorted, f.alt, n, f.scan, f.bsum, st)))
         return rc;
-    if ((rc = build_root(f, n, n_keys, M, ok, st)) || !*ok)
+    if ((rc = build_root(f, n, n_keys, M, ok, st, too_many)) || !*ok)
         return rc;
     const uint32_t L = M->nleaves;
     hipLaunchKernelGGL(k_leaf_moments, dim3(L), dim3(kFitThreads), 0, st, f.dm, f.sorted, f.mom, f.xs);
@@ -736,9 +757,13 @@ int learned_plan(LearnedCtx& ctx, const void* d_build, uint64_t nb, const SideLa
     PJ_HIP(hipGetLastError());
     ctx.host_model.resize(sizeof(pj_model));
     pj_model* M = (pj_model*)ctx.host_model.data();
-    bool ok = false;
-    if ((rc = fit_model(f, ns, nb, kSteps, M, &ok, st)))
+    bool ok = false, too_many = false;
+    if ((rc = fit_model(f, ns, nb, kSteps, M, &ok, st, &too_many)))
         return rc;
+    if (too_many) {
+        plan->why = "too many partitions";
+        return PJ_OK;
+    }
     if (!ok) {
         plan->why = "key domain too fragmented for the root";
         return PJ_OK;

@@ -269,3 +269,56 @@ def expected_clustered(table, n_probe, cdf, threads=8):
     lib().or_expected_clustered(table.ctypes.data_as(C.c_void_p), n_probe, cdf.ctypes.data_as(C.c_void_p), threads,
                                 C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+_NP = {"int": "<i4", "long": "<i8", "date": "<i8", "dec": "<f8"}
+
+
+def aggregate(cols, fields, op, sumfield=None, tuples=None):
+    """The reference's global-mode aggregation over packed tuples of schema
+    cols: GenericAggregate::remember (operators/genericaggregate.cpp:247-289)
+    keeps one output tuple per distinct value of the grouping columns
+    `fields` (the whole input is one group when there are none; an empty
+    input gives no tuple), its schema the grouping columns followed by the
+    fold's (genericaggregate.cpp:51-95). op "sum": AggregateSum
+    (operators/aggregatesum.cpp:37-107) adds column sumfield in that column's
+    type, INTEGER sums wrapping at 32 bits and LONG sums at 64; op "count":
+    AggregateCount (operators/aggregatecount.cpp:37-52) counts tuples as a
+    LONG. Returns the output tuples (n x width uint8) in no particular order.
+    DECIMAL sums depend on the order of the additions and are not restated.
+    """
+    dt = np.dtype([("c%d" % i, _NP[c]) for i, c in enumerate(cols)])
+    rows = np.ascontiguousarray(tuples, dtype=np.uint8).reshape(-1)
+    t = rows.view(dt) if rows.size else np.zeros(0, dt)
+    if op == "sum":
+        if cols[sumfield] not in ("int", "long"):
+            raise ValueError("sums of INTEGER or LONG columns")
+        agg_t = _NP[cols[sumfield]]
+    elif op == "count":
+        agg_t = "<i8"
+    else:
+        raise ValueError("op is sum or count")
+    out_dt = np.dtype([("g%d" % k, _NP[cols[f]]) for k, f in enumerate(fields)] + [("a", agg_t)])
+    if t.size == 0:
+        return np.zeros((0, out_dt.itemsize), np.uint8)
+    if fields:
+        keys = np.zeros(t.size, np.dtype([("g%d" % k, _NP[cols[f]]) for k, f in enumerate(fields)]))
+        for k, f in enumerate(fields):
+            keys["g%d" % k] = t["c%d" % f]
+        uniq, inv = np.unique(keys, return_inverse=True)
+        inv = inv.reshape(-1)
+        ng = uniq.size
+    else:
+        uniq, inv, ng = None, np.zeros(t.size, np.int64), 1
+    if op == "count":
+        acc = np.bincount(inv, minlength=ng).astype(np.int64)
+    else:
+        acc = np.zeros(ng, np.int64)
+        np.add.at(acc, inv, t["c%d" % sumfield].astype(np.int64))   # wraps at 64 bits
+        if agg_t == "<i4":
+            acc = acc.astype(np.int32)   # the low 32 bits: the INTEGER sum's wrap
+    out = np.zeros(ng, out_dt)
+    for k in range(len(fields)):
+        out["g%d" % k] = uniq["g%d" % k]
+    out["a"] = acc
+    return out.view(np.uint8).reshape(ng, out_dt.itemsize)

@@ -1,5 +1,6 @@
-// ref_harness.cpp -- drives the reference HashJoinOp, compiled from the
-// reference tree by oracle/ref/Makefile, on in-memory inputs.
+// ref_harness.cpp -- drives the reference HashJoinOp and the aggregates
+// over it, compiled from the reference tree by oracle/ref/Makefile, on
+// in-memory inputs.
 //
 // Checker and CPU baseline only: tests/, smoke() and bench.py's cpu_baseline
 // load the resulting oracle/_ref/libpythiaref.so; the product never does.
@@ -44,6 +45,8 @@ class MemSourceOp : public ZeroInputOp {
         for (size_t i = 0; i < coltypes.size(); ++i) {
             if (coltypes[i] == CT_CHAR)
                 schema.add(CT_CHAR, colwidths[i]);
+            else if (coltypes[i] == CT_DATE)   // DATE columns carry a format string (schema.cpp:62-74)
+                schema.add(CT_DATE, std::string("%Y-%m-%d"));
             else
                 schema.add(static_cast<ColumnType>(coltypes[i]));
         }
@@ -184,6 +187,95 @@ int ref_hashjoin_run(int threads, int nbcols, const int* btypes, const int* bwid
         *out_n = produced;
         *t_build = t1 - t0;
         *t_total = t2 - t0;
+        return produced > out_cap ? -2 : 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// Runs AggregateSum (op 0, over column sumfield) or AggregateCount (op 1),
+// grouped by nfields columns (0: the whole input), in the reference's global
+// mode: MergeOp(threads) over one aggregate whose hash table all threads
+// share (operators/genericaggregate.cpp:130-165). Grouping hashes with
+// `hashfn` over the first grouping column, or with "bytes" over the range
+// of grouping columns. Returns 0, -1 on a reference error, -2 when out_cap tuples are too
+// few (out_n then holds the count).
+int ref_aggregate_run(int threads, int op, int ncols, const int* types, const int* widths, int nfields,
+                      const int* fields, int sumfield, const char* hashfn, int buckets, const void* data,
+                      uint64_t n, void* out, uint64_t out_cap, uint64_t* out_n, unsigned* out_width,
+                      double* t_total)
+{
+    try {
+        Query q;
+        MemSourceOp src;
+        AggregateSum asum;
+        AggregateCount acount;
+        GenericAggregate* agg = op == 0 ? static_cast<GenericAggregate*>(&asum) : static_cast<GenericAggregate*>(&acount);
+        MergeOp merge;
+        src.setup(types, widths, ncols, data, n);
+
+        Config cfg;
+        cfg.getRoot().add("path", Setting::TypeString) = "./";
+        cfg.getRoot().add("buffsize", Setting::TypeInt) = 1 << 20;
+        Setting& sn = cfg.getRoot().add("src", Setting::TypeGroup);
+        Setting& an = cfg.getRoot().add("agg", Setting::TypeGroup);
+        Setting& fl = an.add("fields", Setting::TypeArray);
+        for (int i = 0; i < nfields; ++i)
+            fl.add(Setting::TypeInt) = fields[i];
+        if (op == 0)
+            an.add("sumfield", Setting::TypeInt) = sumfield;
+        if (nfields > 0) {
+            Setting& hn = an.add("hash", Setting::TypeGroup);
+            hn.add("fn", Setting::TypeString) = hashfn;
+            hn.add("buckets", Setting::TypeInt) = buckets;
+            if (strcmp(hashfn, "bytes") != 0) {
+                hn.add("field", Setting::TypeInt) = fields[0];
+            } else {
+                Setting& fr = hn.add("fieldrange", Setting::TypeArray);
+                fr.add(Setting::TypeInt) = fields[0];
+                fr.add(Setting::TypeInt) = fields[nfields - 1];
+            }
+        }
+        an.add("global", Setting::TypeBoolean) = true;
+        an.add("threads", Setting::TypeInt) = threads;
+        Setting& mn = cfg.getRoot().add("merge", Setting::TypeGroup);
+        mn.add("threads", Setting::TypeInt) = threads;
+
+        q.tree = &merge;
+        merge.nextOp = agg;
+        agg->nextOp = &src;
+        src.init(cfg, sn);
+        agg->init(cfg, an);
+        merge.init(cfg, mn);
+
+        const unsigned ow = q.getOutSchema().getTupleSize();
+        *out_width = ow;
+        q.threadInit();
+        const double t0 = now();
+        if (q.scanStart() == Operator::Error)
+            return -1;
+        uint64_t produced = 0;
+        char* o = static_cast<char*>(out);
+        Operator::GetNextResultT res;
+        res.first = Operator::Ready;
+        while (res.first == Operator::Ready) {
+            res = q.getNext();
+            if (res.first == Operator::Error)
+                return -1;
+            Operator::Page::Iterator it = res.second->createIterator();
+            void* tup;
+            while ((tup = it.next())) {
+                if (produced < out_cap)
+                    memcpy(o + produced * ow, tup, ow);
+                produced++;
+            }
+        }
+        if (q.scanStop() == Operator::Error)
+            return -1;
+        *t_total = now() - t0;
+        q.threadClose();
+        q.destroynofree();
+        *out_n = produced;
         return produced > out_cap ? -2 : 0;
     } catch (...) {
         return -1;

@@ -36,6 +36,11 @@ def lib():
         L.ref_tuple_hash.restype = C.c_uint
         L.ref_tuple_hash.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                      C.c_int]
+        L.ref_aggregate_run.restype = C.c_int
+        L.ref_aggregate_run.argtypes = [
+            C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+            C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+            C.POINTER(C.c_uint64), C.POINTER(C.c_uint), C.POINTER(C.c_double)]
         L.ref_tuple_hash_bytes.restype = C.c_uint
         L.ref_tuple_hash_bytes.argtypes = [C.c_int, C.c_void_p, C.c_int]
         _lib = L
@@ -120,3 +125,30 @@ def tuple_hash(fn, buckets, coltype, value, range_min=0, range_max=0, offset=0, 
 def bytes_hash(buckets, data):
     data = np.ascontiguousarray(data, dtype=np.uint8)
     return lib().ref_tuple_hash_bytes(buckets, data.ctypes.data_as(C.c_void_p), data.size)
+
+
+def run_aggregate(cols, fields, op, sumfield, tuples, threads=4, fn="modulo", buckets=1024):
+    """The reference AggregateSum (op "sum") or AggregateCount (op "count")
+    in global mode under MergeOp(threads), grouped by `fields` (hash fn over
+    the first grouping column; "bytes" over the field range for two, or
+    when fn is "bytes").
+    Returns (rows uint8 [n, w], seconds)."""
+    types, widths, size = _schema(cols)
+    data = np.ascontiguousarray(tuples, dtype=np.uint8)
+    n = data.size // size if size else 0
+    cap = max(n, 1)
+    while True:
+        out = np.zeros(cap * 64, np.uint8)
+        n_out, w, secs = C.c_uint64(), C.c_uint(), C.c_double()
+        rc = lib().ref_aggregate_run(threads, 0 if op == "sum" else 1, len(cols), _ints(types), _ints(widths),
+                                     len(fields), _ints(list(fields)), sumfield if sumfield is not None else 0,
+                                     ("bytes" if len(fields) > 1 else fn).encode(), buckets,
+                                     data.ctypes.data_as(C.c_void_p), n, out.ctypes.data_as(C.c_void_p), cap,
+                                     C.byref(n_out), C.byref(w), C.byref(secs))
+        if rc == -2:
+            cap = n_out.value
+            continue
+        if rc != 0:
+            raise RuntimeError("reference aggregate failed")
+        ow = w.value
+        return out[: n_out.value * ow].reshape(n_out.value, ow), secs.value

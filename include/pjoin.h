@@ -331,6 +331,46 @@ int pj_batch_sizes(uint64_t seed, uint32_t njoins, uint64_t nmin, uint64_t nmax,
 int pj_gen_batch_join(uint64_t seed, uint32_t j, uint64_t nbuild, uint64_t nprobe, void* build_dst,
                       void* probe_dst, int on_device, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Device aggregation: the aggregate_sum / aggregate_count plan nodes
+ * (query.cpp:206-209) in global mode over packed tuples in HBM, for
+ * instance the output of pj_join_run_device, so that only the groups leave
+ * the device. Replaces GenericAggregate::scanStart / remember / getNext
+ * (operators/genericaggregate.cpp:247-515) with AggregateSum's fold
+ * (aggregatesum.cpp:37-107) or AggregateCount's (aggregatecount.cpp:37-52):
+ * one output tuple per distinct value of the grouping columns (one for the
+ * whole input when there are none, none for an empty input), the grouping
+ * columns followed by the sum (the sum column's type: INTEGER sums wrap at
+ * 32 bits, LONG sums at 64) or the count (LONG). Output order unspecified.
+ * ---------------------------------------------------------------------- */
+enum { PJ_AGG_SUM = 0, PJ_AGG_COUNT = 1 };
+
+typedef struct {
+    pj_schema in;           /* schema of the input tuples */
+    uint32_t op;            /* PJ_AGG_SUM or PJ_AGG_COUNT */
+    uint32_t nfields;       /* grouping columns: 0, 1 (INTEGER, LONG or DATE) or 2 (INTEGER, INTEGER) */
+    uint16_t fields[2];     /* their column indexes ("field" / "fields") */
+    uint16_t sumfield;      /* PJ_AGG_SUM: an INTEGER or LONG column ("sumfield") */
+    uint16_t reserved;
+    uint64_t max_groups;    /* distinct groups the device table holds (the plan's hash buckets) */
+} pj_agg_desc;
+
+typedef struct pj_agg_s* pj_agg_t;
+
+/* Validates desc as GenericAggregate::init does (operators/genericaggregate.cpp:51-95,
+ * aggregatesum.cpp:37-51); PJ_ERR_UNSUPPORTED for DECIMAL sums (their value
+ * depends on the order of the additions), other grouping types, or input
+ * columns the device reads off 4-byte boundaries. */
+int pj_agg_create(const pj_agg_desc* desc, pj_agg_t* out);
+int pj_agg_out_width(pj_agg_t h, uint32_t* width);
+/* Aggregates n tuples at d_tuples (4-byte aligned) into d_groups (device
+ * memory, room for groups_cap output tuples). *ngroups gets the number of
+ * groups; PJ_ERR_CAPACITY when they exceed groups_cap or the table's
+ * max_groups (twice that many slots). Synchronous. */
+int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_groups, uint64_t groups_cap,
+                      uint64_t* ngroups, void* stream);
+int pj_agg_destroy(pj_agg_t h);
+
 #ifdef __cplusplus
 }
 #endif

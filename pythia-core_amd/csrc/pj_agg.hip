@@ -1,0 +1,352 @@
+// pj_agg.hip -- device aggregation over packed tuples in HBM: the
+// aggregate_sum / aggregate_count plan nodes in global mode
+// (operators/genericaggregate.cpp, aggregatesum.cpp, aggregatecount.cpp),
+// typically over the join's output so that only the groups leave the device.
+//
+//   k_agg      every block folds its share of the tuples into an LDS hash
+//              table of kAggSlots groups (linear probing, at most kAggProbes
+//              probes); a tuple whose group finds no slot goes straight to the
+//              global table; at the end the block flushes its table. Without
+//              grouping each thread folds in registers and the block makes one
+//              global atomic.
+//   k_agg_emit one output tuple per used slot of the global table.
+//
+// The global table (open addressing, at least twice max_groups slots) is
+// lock-free: a 64-bit CAS claims a slot for a key, a 64-bit atomic add folds
+// the value into it, and nothing waits. The key kEmpty marks an empty slot;
+// a group whose key happens to be kEmpty is folded into a slot of its own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "pj_device.h"
+#include "pj_internal.h"
+
+namespace pj {
+namespace {
+
+constexpr uint64_t kEmpty = 0xA5A5A5A5A5A5A5A5ull;
+constexpr uint32_t kAggThreads = 512;
+constexpr uint32_t kAggSlots = 4096;     // LDS groups a block: 64 KiB of keys and sums
+constexpr uint32_t kAggProbes = 8;
+
+// How a tuple's key and value are read: 32-bit words at byte offsets.
+struct AggLayout {
+    uint32_t tw;            // tuple words
+    uint32_t nkey;          // 0, 1 or 2 grouping columns
+    uint32_t key_off[2];    // word offsets
+    uint32_t key_wide[2];   // 1: 8-byte column (LONG, DATE)
+    uint32_t val_off;       // word offset of the summed column
+    uint32_t val_wide;      // 1: LONG
+    uint32_t count;         // 1: AggregateCount
+};
+
+struct AggTable {
+    unsigned long long* key;   // G slots
+    unsigned long long* val;
+    unsigned long long* misc;  // [0] kEmpty group's value, [1] its tuples, [2] overflow, [3] groups emitted
+    uint32_t mask;             // G - 1
+};
+
+__device__ __forceinline__ uint64_t agg_key(const uint32_t* t, const AggLayout& L)
+{
+    if (L.nkey == 0)
+        return 0;
+    const uint64_t a = L.key_wide[0] ? ((uint64_t)t[L.key_off[0]] | ((uint64_t)t[L.key_off[0] + 1] << 32))
+                                     : (uint64_t)t[L.key_off[0]];
+    if (L.nkey == 1)
+        return a;
+    return (a & 0xFFFFFFFFull) | ((uint64_t)t[L.key_off[1]] << 32);   // two INTEGER columns
+}
+
+__device__ __forceinline__ uint64_t agg_val(const uint32_t* t, const AggLayout& L)
+{
+    if (L.count)
+        return 1;
+    if (L.val_wide)
+        return (uint64_t)t[L.val_off] | ((uint64_t)t[L.val_off + 1] << 32);
+    return (uint64_t)(int64_t)(int32_t)t[L.val_off];   // INTEGER, sign-extended; its low 32 bits wrap as int32
+}
+
+__device__ __forceinline__ void global_fold(const AggTable& G, uint64_t k, uint64_t v)
+{
+    if (k == kEmpty) {
+        atomicAdd(&G.misc[0], (unsigned long long)v);
+        atomicAdd(&G.misc[1], 1ull);
+        return;
+    }
+    uint32_t s = (uint32_t)pj_fin64(k) & G.mask;
+    for (uint32_t i = 0; i <= G.mask; ++i) {
+        const unsigned long long prev = atomicCAS(&G.key[s], (unsigned long long)kEmpty, (unsigned long long)k);
+        if (prev == kEmpty || prev == k) {
+            atomicAdd(&G.val[s], (unsigned long long)v);
+            return;
+        }
+        s = (s + 1) & G.mask;
+    }
+    atomicOr(&G.misc[2], 1ull);   // every slot taken: more groups than the table holds
+}
+
+__global__ __launch_bounds__(kAggThreads) void k_agg(const uint32_t* __restrict__ in, uint64_t n, AggLayout L,
+                                                      AggTable G)
+{
+    __shared__ unsigned long long lds[2 * kAggSlots];   // 64 KiB: keys, then sums
+    unsigned long long* lkey = lds;
+    unsigned long long* lval = lds + kAggSlots;
+    unsigned long long* wpart = lds;                      // per-wave sums, without grouping
+    const uint64_t stride = (uint64_t)gridDim.x * kAggThreads;
+    const uint64_t first = (uint64_t)blockIdx.x * kAggThreads + threadIdx.x;
+    if (L.nkey == 0) {   // one group: fold in registers, one atomic a block
+        uint64_t acc = 0;
+        for (uint64_t i = first; i < n; i += stride)
+            acc += agg_val(in + i * L.tw, L);
+        for (int d = 32; d > 0; d >>= 1)
+            acc += __shfl_down(acc, d, 64);
+        if ((threadIdx.x & 63) == 0)
+            wpart[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t b = 0;
+            for (uint32_t w = 0; w < kAggThreads / 64; ++w)
+                b += wpart[w];
+            const bool any = (uint64_t)blockIdx.x * kAggThreads < n;
+            if (any)
+                global_fold(G, 0, b);
+        }
+        return;
+    }
+    for (uint32_t s = threadIdx.x; s < kAggSlots; s += kAggThreads) {
+        lkey[s] = kEmpty;
+        lval[s] = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = first; i < n; i += stride) {
+        const uint32_t* t = in + i * L.tw;
+        const uint64_t k = agg_key(t, L), v = agg_val(t, L);
+        bool done = false;
+        if (k != kEmpty) {
+            uint32_t s = (uint32_t)pj_fin64(k) & (kAggSlots - 1);
+            for (uint32_t p = 0; p < kAggProbes; ++p) {
+                const unsigned long long prev = atomicCAS(&lkey[s], (unsigned long long)kEmpty, (unsigned long long)k);
+                if (prev == kEmpty || prev == k) {
+                    atomicAdd(&lval[s], (unsigned long long)v);
+                    done = true;
+                    break;
+                }
+                s = (s + 1) & (kAggSlots - 1);
+            }
+        }
+        if (!done)
+            global_fold(G, k, v);
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < kAggSlots; s += kAggThreads)
+        if (lkey[s] != kEmpty)
+            global_fold(G, lkey[s], lval[s]);
+}
+
+// Writes a word-aligned 8-byte value as two 4-byte stores (packed output
+// tuples put 8-byte columns at any 4-byte offset).
+__device__ __forceinline__ void put64(uint32_t* o, uint64_t v)
+{
+    o[0] = (uint32_t)v;
+    o[1] = (uint32_t)(v >> 32);
+}
+
+__device__ __forceinline__ void emit_group(uint32_t* __restrict__ out, uint64_t cap, const AggLayout& L,
+                                           uint32_t ow, const AggTable& G, uint64_t k, uint64_t v)
+{
+    const unsigned long long pos = atomicAdd(&G.misc[3], 1ull);
+    if (pos >= cap)
+        return;
+    uint32_t* o = out + pos * ow;
+    uint32_t w = 0;
+    for (uint32_t c = 0; c < L.nkey; ++c) {
+        if (L.nkey == 2) {   // two INTEGER columns: low then high half of the key
+            o[w++] = c ? (uint32_t)(k >> 32) : (uint32_t)k;
+        } else if (L.key_wide[c]) {
+            put64(o + w, k);
+            w += 2;
+        } else {
+            o[w++] = (uint32_t)k;
+        }
+    }
+    if (L.count || L.val_wide)
+        put64(o + w, v);
+    else
+        o[w] = (uint32_t)v;
+}
+
+__global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __restrict__ out, uint64_t cap)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= G.mask) {
+        const uint64_t k = G.key[s];
+        if (k != kEmpty)
+            emit_group(out, cap, L, ow, G, k, G.val[s]);
+    }
+    if (s == 0 && G.misc[1])
+        emit_group(out, cap, L, ow, G, kEmpty, G.misc[0]);
+}
+
+__global__ void k_agg_clear(AggTable G)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= G.mask) {
+        G.key[s] = kEmpty;
+        G.val[s] = 0;
+    }
+    if (s < 4)
+        G.misc[s] = 0;
+}
+
+}  // namespace
+}  // namespace pj
+
+using namespace pj;
+
+struct pj_agg_s {
+    pj_agg_desc desc;
+    AggLayout L;
+    uint32_t out_words = 0;
+    uint32_t slots = 0;
+    DevBuf table;
+    uint64_t* host_misc = nullptr;   // pinned readback of the table's counters
+    std::mutex mu;
+};
+
+extern "C" {
+
+int pj_agg_create(const pj_agg_desc* desc, pj_agg_t* out)
+{
+    if (!desc || !out)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int off[PJ_MAX_COLS], size;
+    int rc = schema_offsets(desc->in, off, &size);
+    if (rc)
+        return rc;
+    if (desc->op != PJ_AGG_SUM && desc->op != PJ_AGG_COUNT)
+        return fail(PJ_ERR_INVALID, "op must be PJ_AGG_SUM or PJ_AGG_COUNT");
+    if (desc->nfields > 2)
+        return fail(PJ_ERR_UNSUPPORTED, "at most two grouping columns");
+    if (desc->max_groups == 0 || desc->max_groups > (1ull << 30))
+        return fail(PJ_ERR_INVALID, "max_groups must be 1..2^30");
+    if (size % 4)
+        return fail(PJ_ERR_UNSUPPORTED, "input tuples of %d bytes: the device reads 4-byte words", size);
+    AggLayout L = {};
+    L.tw = (uint32_t)size / 4;
+    L.nkey = desc->nfields;
+    uint32_t ow = 0;
+    for (uint32_t k = 0; k < desc->nfields; ++k) {
+        const uint32_t f = desc->fields[k];
+        if (f >= desc->in.ncols)
+            return fail(PJ_ERR_INVALID, "grouping column %u out of range", f);
+        const uint32_t t = desc->in.type[f];
+        const bool wide = t == PJ_CT_LONG || t == PJ_CT_DATE;
+        if (!(t == PJ_CT_INTEGER || wide) || (desc->nfields == 2 && t != PJ_CT_INTEGER))
+            return fail(PJ_ERR_UNSUPPORTED, "grouping column %u: one INTEGER, LONG or DATE column, or two INTEGER", f);
+        if (off[f] % 4)
+            return fail(PJ_ERR_UNSUPPORTED, "grouping column %u is not 4-byte aligned", f);
+        L.key_off[k] = (uint32_t)off[f] / 4;
+        L.key_wide[k] = wide ? 1u : 0u;
+        ow += wide ? 2u : 1u;
+    }
+    if (desc->op == PJ_AGG_SUM) {
+        const uint32_t f = desc->sumfield;
+        if (f >= desc->in.ncols)
+            return fail(PJ_ERR_INVALID, "sumfield %u out of range", f);
+        const uint32_t t = desc->in.type[f];
+        if (t == PJ_CT_DECIMAL)
+            return fail(PJ_ERR_UNSUPPORTED, "DECIMAL sums depend on the order of the additions");
+        if (t != PJ_CT_INTEGER && t != PJ_CT_LONG)
+            return fail(PJ_ERR_INVALID, "sumfield %u must be a numeric column", f);   // aggregatesum.cpp:42-45
+        if (off[f] % 4)
+            return fail(PJ_ERR_UNSUPPORTED, "sumfield %u is not 4-byte aligned", f);
+        L.val_off = (uint32_t)off[f] / 4;
+        L.val_wide = t == PJ_CT_LONG ? 1u : 0u;
+        ow += L.val_wide ? 2u : 1u;
+    } else {
+        L.count = 1;
+        ow += 2;   // LONG count (aggregatecount.cpp:37-42)
+    }
+    pj_agg_s* h = new (std::nothrow) pj_agg_s();
+    if (!h)
+        return fail(PJ_ERR_INVALID, "out of host memory");
+    h->desc = *desc;
+    h->L = L;
+    h->out_words = ow;
+    uint64_t slots = 1024;
+    while (slots < 2 * desc->max_groups)
+        slots <<= 1;
+    h->slots = (uint32_t)slots;
+    *out = h;
+    return PJ_OK;
+}
+
+int pj_agg_out_width(pj_agg_t h, uint32_t* width)
+{
+    if (!h || !width)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *width = h->out_words * 4;
+    return PJ_OK;
+}
+
+int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_groups, uint64_t groups_cap,
+                      uint64_t* ngroups, void* stream)
+{
+    if (!h || !ngroups || (n && !d_tuples) || (groups_cap && !d_groups))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if ((uintptr_t)d_tuples % 4 || (uintptr_t)d_groups % 4)
+        return fail(PJ_ERR_INVALID, "input and output must be 4-byte aligned");
+    std::lock_guard<std::mutex> g(h->mu);
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t slots = h->slots;
+    int rc;
+    if ((rc = h->table.ensure((2 * slots + 4) * 8)))
+        return rc;
+    if (!h->host_misc)
+        PJ_HIP(hipHostMalloc((void**)&h->host_misc, 64, hipHostMallocDefault));
+    AggTable G;
+    G.key = (unsigned long long*)h->table.p;
+    G.val = G.key + slots;
+    G.misc = G.val + slots;
+    G.mask = (uint32_t)(slots - 1);
+    hipLaunchKernelGGL(k_agg_clear, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G);
+    if (n) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        const uint64_t want = (n + 8191) / 8192;
+        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
+        hipLaunchKernelGGL(k_agg, dim3(grid), dim3(kAggThreads), 0, st, (const uint32_t*)d_tuples, n, h->L, G);
+        hipLaunchKernelGGL(k_agg_emit, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G, h->L,
+                           h->out_words, (uint32_t*)d_groups, groups_cap);
+    }
+    PJ_HIP(hipGetLastError());
+    PJ_HIP(hipMemcpyAsync(h->host_misc, G.misc, 32, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipStreamSynchronize(st));
+    *ngroups = h->host_misc[3];
+    if (h->host_misc[2])
+        return fail(PJ_ERR_CAPACITY, "more distinct groups than the table's %llu slots (max_groups %llu)",
+                    (unsigned long long)slots, (unsigned long long)h->desc.max_groups);
+    if (*ngroups > groups_cap)
+        return fail(PJ_ERR_CAPACITY, "%llu groups, the output holds %llu", (unsigned long long)*ngroups,
+                    (unsigned long long)groups_cap);
+    return PJ_OK;
+}
+
+int pj_agg_destroy(pj_agg_t h)
+{
+    if (!h)
+        return PJ_OK;
+    h->table.release();
+    if (h->host_misc)
+        (void)hipHostFree(h->host_misc);
+    delete h;
+    return PJ_OK;
+}
+
+}  // extern "C"

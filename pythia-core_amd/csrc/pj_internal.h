@@ -57,6 +57,10 @@ struct Layout {
 // Validates a descriptor and derives the layouts (see pj_join.hip).
 int derive_layout(const pj_join_desc* d, Layout* L);
 
+// Byte offsets of a packed schema's columns (the reference's Schema layout,
+// no alignment) and the tuple size; PJ_ERR_INVALID for a bad schema.
+int schema_offsets(const pj_schema& s, int* off, int* size);
+
 // ---------------------------------------------------------------- device
 constexpr uint32_t kJoinThreads = 1024;
 constexpr uint32_t kTableCap = 8944;       // records per LDS table (~140 KiB)

@@ -732,7 +732,7 @@ static int col_width(const pj_schema& s, uint32_t c)
     }
 }
 
-static int schema_offsets(const pj_schema& s, int* off, int* size)
+int schema_offsets(const pj_schema& s, int* off, int* size)
 {
     if (s.ncols == 0 || s.ncols > PJ_MAX_COLS)
         return fail(PJ_ERR_INVALID, "schema must have 1..%d columns", PJ_MAX_COLS);

@@ -71,6 +71,11 @@ class BatchCfg(C.Structure):
     _fields_ = [("bits", C.c_uint8), ("passes", C.c_uint8), ("reserved", C.c_uint16), ("load_factor", C.c_float)]
 
 
+class AggDesc(C.Structure):
+    _fields_ = [("schema", Schema), ("op", C.c_uint32), ("nfields", C.c_uint32), ("fields", C.c_uint16 * 2),
+                ("sumfield", C.c_uint16), ("reserved", C.c_uint16), ("max_groups", C.c_uint64)]
+
+
 class Cursor(C.Structure):
     _fields_ = [("impl", C.c_void_p), ("pending", C.c_uint64)]
 
@@ -128,6 +133,11 @@ _SIGS = {
     "pj_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "pj_batch_results": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pj_batch_run_times": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pj_agg_create": (C.c_int, [C.POINTER(AggDesc), C.POINTER(C.c_void_p)]),
+    "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_agg_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                    C.POINTER(C.c_uint64), C.c_void_p]),
+    "pj_agg_destroy": (C.c_int, [C.c_void_p]),
     "pj_batch_sizes": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     "pj_gen_batch_join": (C.c_int, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_int, C.c_void_p]),
@@ -388,3 +398,42 @@ def cluster_table(seed, n_keys, nclusters=256, log2_wmin=10, log2_wmax=24, weigh
     check(lib().pj_clusters_make(seed, nclusters, n_keys, log2_wmin, log2_wmax, weight_z,
                                  t.ctypes.data_as(C.c_void_p)))
     return t
+
+
+PJ_AGG_SUM, PJ_AGG_COUNT = 0, 1
+
+
+class Aggregate:
+    """Owns a pj_agg_t: aggregate_sum / aggregate_count over device tuples."""
+
+    def __init__(self, cols, fields, op, sumfield=None, max_groups=1 << 20):
+        d = AggDesc()
+        d.schema = make_schema(cols)
+        d.op = PJ_AGG_SUM if op == "sum" else PJ_AGG_COUNT
+        d.nfields = len(fields)
+        for i, f in enumerate(fields):
+            d.fields[i] = f
+        d.sumfield = sumfield or 0
+        d.max_groups = max_groups
+        self.h = C.c_void_p()
+        check(lib().pj_agg_create(C.byref(d), C.byref(self.h)))
+        w = C.c_uint32()
+        check(lib().pj_agg_out_width(self.h, C.byref(w)))
+        self.out_width = w.value
+
+    def run_device(self, d_tuples, n, d_groups, groups_cap, stream=None):
+        g = C.c_uint64()
+        check(lib().pj_agg_run_device(self.h, C.c_void_p(d_tuples), n, C.c_void_p(d_groups), groups_cap,
+                                      C.byref(g), C.c_void_p(stream)))
+        return g.value
+
+    def close(self):
+        if self.h:
+            lib().pj_agg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

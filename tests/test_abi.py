@@ -94,3 +94,32 @@ def test_batch_modes_before_and_after_prepare():
     cnt = (C.c_uint64 * 1)()
     assert P.lib().pj_batch_results(bt.h, cnt, None, None) == P.PJ_ERR_STATE
     bt.close()
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(op="sum", fields=[0], sumfield=5), P.PJ_ERR_INVALID),               # sumfield out of range
+    (dict(op="sum", fields=[0], sumfield=2), P.PJ_ERR_UNSUPPORTED),           # DECIMAL sum
+    (dict(op="count", fields=[0, 1, 2]), P.PJ_ERR_UNSUPPORTED),               # three grouping columns
+    (dict(op="count", fields=[0, 1]), P.PJ_ERR_UNSUPPORTED),                  # composite of LONG columns
+])
+def test_bad_aggregates_are_rejected(kw, code):
+    d = P.AggDesc()
+    d.schema = P.make_schema(["long", "long", "dec"])
+    d.op = P.PJ_AGG_SUM if kw["op"] == "sum" else P.PJ_AGG_COUNT
+    d.nfields = len(kw["fields"])
+    for i, f in enumerate(kw["fields"][:2]):
+        d.fields[i] = f
+    d.sumfield = kw.get("sumfield", 0)
+    d.max_groups = 100
+    h = C.c_void_p()
+    assert P.lib().pj_agg_create(C.byref(d), C.byref(h)) == code
+    assert P.lib().pj_last_error()
+
+
+def test_aggregate_output_width():
+    a = P.Aggregate(["int", "long", "int"], [0], "sum", 1)
+    assert a.out_width == 12          # INTEGER group, LONG sum
+    a.close()
+    a = P.Aggregate(["int", "int", "long"], [0, 1], "count", None)
+    assert a.out_width == 16          # two INTEGER groups, LONG count
+    a.close()

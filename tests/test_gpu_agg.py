@@ -1,0 +1,136 @@
+"""GPU tests of the device aggregation (pj_agg_*): aggregate_sum and
+aggregate_count in global mode against the oracle (oracle.aggregate, pinned
+against the reference's aggregates), alone and over the output of a device
+join."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pjoin as P
+from test_aggregate_oracle import CASES, _table
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    P.lib()
+    yield
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _device_agg(cols, fields, op, sumfield, rows, max_groups=1 << 20):
+    agg = P.Aggregate(cols, fields, op, sumfield, max_groups=max_groups)
+    n = rows.shape[0]
+    d = torch.from_numpy(np.ascontiguousarray(rows).reshape(-1).copy()).cuda() if n else \
+        torch.zeros(16, dtype=torch.uint8, device="cuda")
+    cap = max(n, 1)
+    out = torch.empty(cap * agg.out_width, dtype=torch.uint8, device="cuda")
+    g = agg.run_device(d.data_ptr(), n, out.data_ptr(), cap)
+    got = out[: g * agg.out_width].cpu().numpy().reshape(g, agg.out_width)
+    agg.close()
+    return got
+
+
+@pytest.mark.parametrize("cols,fields,op,sumfield,groups", CASES)
+@pytest.mark.parametrize("n", [1, 1000, 1 << 20])
+def test_device_aggregate_equals_oracle(cols, fields, op, sumfield, groups, n):
+    _say("case: agg %s fields=%s %s n=%d" % (cols, fields, op, n))
+    rng = np.random.default_rng(n)
+    _, rows = _table(cols, n, rng, groups)
+    exp = O.aggregate(cols, fields, op, sumfield, rows)
+    got = _device_agg(cols, fields, op, sumfield, rows)
+    assert got.shape == exp.shape
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+@pytest.mark.parametrize("op", ["sum", "count"])
+def test_device_aggregate_many_groups(op):
+    """More distinct groups than a block's LDS table holds: most tuples fold
+    straight into the global table."""
+    _say("case: agg many groups %s" % op)
+    rng = np.random.default_rng(8)
+    cols = ["long", "int"]
+    _, rows = _table(cols, 600000, rng, {0: 300000})
+    exp = O.aggregate(cols, [0], op, 1, rows)
+    got = _device_agg(cols, [0], op, 1, rows, max_groups=300000)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+def test_device_aggregate_key_equal_to_the_empty_slot_marker():
+    """A LONG group key and a composite key whose bits equal the table's
+    empty-slot marker are groups like any other."""
+    _say("case: agg marker keys")
+    marker = np.uint64(0xA5A5A5A5A5A5A5A5).view(np.int64)
+    t = np.zeros(7, np.dtype([("c0", "<i8"), ("c1", "<i8")]))
+    t["c0"] = [marker, 1, marker, 2, 1, marker, 0]
+    t["c1"] = [5, 6, 7, 8, 9, 10, 11]
+    rows = t.view(np.uint8).reshape(7, 16)
+    for op in ("sum", "count"):
+        exp = O.aggregate(["long", "long"], [0], op, 1, rows)
+        got = _device_agg(["long", "long"], [0], op, 1, rows)
+        assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+    half = np.uint32(0xA5A5A5A5).view(np.int32)
+    u = np.zeros(4, np.dtype([("a", "<i4"), ("b", "<i4"), ("v", "<i4")]))
+    u["a"] = [half, half, 3, half]
+    u["b"] = [half, half, 4, 0]
+    u["v"] = [1, 2, 3, 4]
+    rows = u.view(np.uint8).reshape(4, 12)
+    exp = O.aggregate(["int", "int", "int"], [0, 1], "sum", 2, rows)
+    got = _device_agg(["int", "int", "int"], [0, 1], "sum", 2, rows)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+def test_device_aggregate_empty_and_capacity():
+    _say("case: agg empty / capacity")
+    got = _device_agg(["long", "long"], [0], "sum", 1, np.zeros((0, 16), np.uint8))
+    assert got.shape[0] == 0
+    got = _device_agg(["long", "long"], [], "count", None, np.zeros((0, 16), np.uint8))
+    assert got.shape[0] == 0
+    rng = np.random.default_rng(9)
+    _, rows = _table(["long", "long"], 5000, rng, {0: 4000})
+    with pytest.raises(P.PJError) as e:    # more groups than the table's 1024 slots
+        _device_agg(["long", "long"], [0], "count", None, rows, max_groups=16)
+    assert e.value.code == P.PJ_ERR_CAPACITY
+    agg = P.Aggregate(["long", "long"], [0], "count", None)
+    d = torch.from_numpy(rows.reshape(-1).copy()).cuda()
+    out = torch.empty(10 * agg.out_width, dtype=torch.uint8, device="cuda")
+    with pytest.raises(P.PJError) as e:    # more groups than the output holds
+        agg.run_device(d.data_ptr(), 5000, out.data_ptr(), 10)
+    assert e.value.code == P.PJ_ERR_CAPACITY
+    agg.close()
+
+
+@pytest.mark.parametrize("fields,op,sumfield", [([0], "count", None), ([], "sum", 2), ([1], "sum", 2)])
+def test_join_then_aggregate_on_the_device(fields, op, sumfield):
+    """aggregate over hashjoin with the join's output left in HBM: count per
+    build key, sum of the probe payload, sum per build payload."""
+    _say("case: join -> agg %s %s" % (fields, op))
+    nb, npr = 1 << 16, 1 << 20
+    b, p = O.gen_pkfk(57, 1 << 40, nb, npr, npr // 10)
+    proj = ["B$0", "B$1", "P$1"]
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], proj, buckets=1 << 16)
+    exp_join, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    cols = ["long", "long", "long"]
+    exp = O.aggregate(cols, fields, op, sumfield, exp_join)
+    db = torch.from_numpy(b.view(np.uint8).copy()).cuda()
+    dp = torch.from_numpy(p.view(np.uint8).copy()).cuda()
+    j = P.Join(P.make_desc(["long", "long"], ["long", "long"], [0], [0], proj))
+    out = torch.empty(npr * j.out_width, dtype=torch.uint8, device="cuda")
+    n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr)
+    agg = P.Aggregate(cols, fields, op, sumfield, max_groups=nb)
+    groups = torch.empty(nb * agg.out_width, dtype=torch.uint8, device="cuda")
+    g = agg.run_device(out.data_ptr(), n, groups.data_ptr(), nb)
+    got = groups[: g * agg.out_width].cpu().numpy().reshape(g, agg.out_width)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+    j.close()
+    agg.close()

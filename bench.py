@@ -173,6 +173,67 @@ def bench_batch(args):
         sys.exit(1)
 
 
+def cpu_baseline(seed, dom, nb, npr, nmiss):
+    """Times the reference HashJoinOp (oracle/_ref) when it was built, else
+    the oracle's single-threaded restatement, on a bounded sample: the whole
+    build relation and the first probe tuples of the same probe relation."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    sample = min(npr, 1 << 25)
+    b, p = O.gen_pkfk(seed, dom, nb, npr, nmiss, probe_prefix=sample)
+    what = "|R|=%d build + first %d of %d probe tuples" % (nb, sample, npr)
+    import reference_run as R
+    ref = R if R.available() else None
+    if ref is not None:
+        threads = ref.default_threads()
+        r = ref.run_pkfk(b, p, threads)
+        return {"value": (nb + sample) / r["seconds"], "unit": "tuples/s", "cores": threads,
+                "kind": "reference", "sample": what + "; reference HashJoinOp built -O3 from its sources, "
+                "under MergeOp, modulo hash over next_pow2(|R|) buckets of 2 tuples, 1 MiB pages, %d threads"
+                % threads, "seconds": r["seconds"], "build_seconds": r["build_seconds"],
+                "out_tuples": r["out_tuples"]}
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                    buckets=1 << max(1, (nb - 1).bit_length()), tuplesperbucket=2)
+    t0 = time.perf_counter()
+    out, st = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    dt = time.perf_counter() - t0
+    return {"value": (nb + sample) / dt, "unit": "tuples/s", "cores": 1, "kind": "port",
+            "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
+
+
+def cpu_baseline_zipf(seed, nb, npr, cdf):
+    """The reference HashJoinOp (bytes hash over the two key columns, as
+    queryhashjoincomposite.cpp runs it) on the whole build side and the first
+    probe tuples; the oracle restatement when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import reference_run as R
+    sample = min(npr, 1 << 24)
+    b, p = O.gen_pairs_zipf(seed, nb, sample, cdf)
+    cols = ["int", "int", "long"]
+    proj = ["B$0", "B$1", "B$2", "P$2"]
+    buckets = 1 << max(1, (nb - 1).bit_length())
+    what = "|R|=%d build + %d Zipf probe tuples" % (nb, sample)
+    if R.available():
+        threads = R.default_threads()
+        rows, tb, tt = R.run_join(cols, cols, [0, 1], [0, 1], proj, b, p, threads=threads, fn="bytes",
+                                  buckets=buckets, tuplesperbucket=2)
+        return {"value": (nb + sample) / tt, "unit": "tuples/s", "cores": threads, "kind": "reference",
+                "sample": what + "; reference HashJoinOp, bytes hash, %d threads" % threads, "seconds": tt,
+                "out_tuples": int(rows.shape[0])}
+    d = O.join_desc(cols, cols, [0, 1], [0, 1], proj, fn="bytes", buckets=buckets)
+    t0 = time.perf_counter()
+    out, st = O.hashjoin(d, b, p)
+    dt = time.perf_counter() - t0
+    return {"value": (nb + sample) / dt, "unit": "tuples/s", "cores": 1, "kind": "port",
+            "sample": what + "; restated HashJoinOp, one thread", "out_tuples": int(st.out_tuples)}
+
+
+
+
 def cpu_baseline_batch(seed, sizes):
     """The reference HashJoinOp (oracle/_ref), else the oracle restatement,
     on every 8th join of the batch (a sample of the same size mix), one join
@@ -219,6 +280,9 @@ def main():
                     sorted(BATCH_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
+    ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key"],
+                    help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
+                         "per build key (int64-key workloads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB with torch, a kernel of known traffic for scaling HBM counters")
@@ -273,12 +337,31 @@ def main():
                                     C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
                            partitioner=args.partitioner, timing=True)
+    if zipf and args.aggregate != "none":
+        print("--aggregate runs on the int64-key workloads", file=sys.stderr)
+        sys.exit(2)
     j = P.Join(desc)
     cap = npr
     out = torch.empty(cap * j.out_width, dtype=torch.uint8, device=dev)
+    agg = None
+    if args.aggregate != "none":
+        # join output: (B$0 key, B$1 build payload, P$1 probe payload), three LONG columns
+        agg = (P.Aggregate(["long"] * 3, [], "sum", 2, max_groups=1) if args.aggregate == "sum" else
+               P.Aggregate(["long"] * 3, [0], "count", None, max_groups=nb))
+        gcap = 1 if args.aggregate == "sum" else nb
+        groups = torch.empty(gcap * agg.out_width, dtype=torch.uint8, device=dev)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        agg_ms = []
 
     def step():
-        return j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), cap, stream)
+        n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), cap, stream)
+        if agg is not None:
+            ev[0].record()
+            step.groups = agg.run_device(out.data_ptr(), n, groups.data_ptr(), gcap, stream)
+            ev[1].record()
+            ev[1].synchronize()
+            agg_ms.append(ev[0].elapsed_time(ev[1]))
+        return n
 
     for _ in range(args.warmup):
         n = step()
@@ -309,12 +392,49 @@ def main():
         es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
     parity = (n == ec and s == es)
 
+    agg_res = None
+    if agg is not None:
+        # size-independent checks against torch's own reductions over the same join output
+        rows = out[: n * j.out_width].view(torch.int64).view(-1, 3)
+        g = step.groups
+        gr = groups[: g * agg.out_width].view(torch.int64)
+        if args.aggregate == "sum":
+            ok = g == 1 and int(gr[0]) == int(rows[:, 2].sum())
+        else:
+            gr = gr.view(-1, 2)
+            ok = (int(gr[:, 1].sum()) == n and int((gr[:, 0] * gr[:, 1]).sum()) == int(rows[:, 0].sum())
+                  and g == int(torch.unique(rows[:, 0]).numel()))
+        parity = parity and ok
+        a_ms = sorted(agg_ms[args.warmup:])[len(agg_ms[args.warmup:]) // 2]
+        a_bytes = n * j.out_width + g * agg.out_width
+        agg_res = {"op": args.aggregate, "groups": g, "ms_median": a_ms, "checks_ok": ok,
+                   "tuples_per_s": n / (a_ms * 1e-3),
+                   "roofline": {"bound": "hbm", "kernel": "k_agg", "achieved": a_bytes / (a_ms * 1e-3) / 1e9,
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": a_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None}}
+        if not args.no_cpu_baseline:
+            # the oracle's aggregation (numpy, one core) over the first 16M rows of the same join output
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            m = min(n, 1 << 24)
+            host = out[: m * j.out_width].cpu().numpy().reshape(m, j.out_width)
+            c0 = time.perf_counter()
+            if args.aggregate == "sum":
+                O.aggregate(["long"] * 3, [], "sum", 2, host)
+            else:
+                O.aggregate(["long"] * 3, [0], "count", None, host)
+            cs = time.perf_counter() - c0
+            agg_res["cpu_baseline"] = {"value": m / cs, "unit": "input tuples/s", "cores": 1, "kind": "port",
+                                       "sample": "first %d rows of the join output" % m}
+        agg.close()
     w_in, w_out = 16, j.out_width
     b_alg = nb * w_in + npr * w_in + n * w_out
     join_alg = b_alg  # the join kernel reads R and S partitions once and writes the output
     achieved = join_alg / (phases["join_ms"] * 1e-3) / 1e9
     job_gbs = b_alg / (ms * 1e-3) / 1e9
     t = j.timings()
+    if agg_res is not None:
+        phases["aggregate_ms"] = agg_res["ms_median"]
     res = {
         "metric": ("join tuples/s (build+probe), 64M x 512M clustered int64 keys, Zipf z=%g fk; %s"
                    % (args.zipf, args.partitioner)) if clustered else
@@ -346,6 +466,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": JOIN_TRAFFIC_BYTES.get(args.workload)},
     }
+    if agg_res is not None:
+        res["metric"] = res["metric"].replace("join tuples/s", "join + %s tuples/s" % args.aggregate)
+        res["config"]["aggregate"] = args.aggregate
+        res["aggregate"] = agg_res
     j.close()
     del db, dp, out
     torch.cuda.empty_cache()

@@ -95,11 +95,15 @@ __global__ __launch_bounds__(kAggThreads) void k_agg(const uint32_t* __restrict_
     unsigned long long* lkey = lds;
     unsigned long long* lval = lds + kAggSlots;
     unsigned long long* wpart = lds;                      // per-wave sums, without grouping
-    const uint64_t stride = (uint64_t)gridDim.x * kAggThreads;
-    const uint64_t first = (uint64_t)blockIdx.x * kAggThreads + threadIdx.x;
+    // Each block folds one contiguous range of tuples (a multiple of the block
+    // size), so that input clustered by group -- the join writes each probe
+    // round's matches as one run -- meets the LDS table while its groups are hot.
+    const uint64_t per = ((n + gridDim.x - 1) / gridDim.x + kAggThreads - 1) / kAggThreads * kAggThreads;
+    const uint64_t lo = (uint64_t)blockIdx.x * per;
+    const uint64_t hi = lo + per < n ? lo + per : n;
     if (L.nkey == 0) {   // one group: fold in registers, one atomic a block
         uint64_t acc = 0;
-        for (uint64_t i = first; i < n; i += stride)
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += kAggThreads)
             acc += agg_val(in + i * L.tw, L);
         for (int d = 32; d > 0; d >>= 1)
             acc += __shfl_down(acc, d, 64);
@@ -110,8 +114,7 @@ __global__ __launch_bounds__(kAggThreads) void k_agg(const uint32_t* __restrict_
             uint64_t b = 0;
             for (uint32_t w = 0; w < kAggThreads / 64; ++w)
                 b += wpart[w];
-            const bool any = (uint64_t)blockIdx.x * kAggThreads < n;
-            if (any)
+            if (lo < hi)
                 global_fold(G, 0, b);
         }
         return;
@@ -121,15 +124,18 @@ __global__ __launch_bounds__(kAggThreads) void k_agg(const uint32_t* __restrict_
         lval[s] = 0;
     }
     __syncthreads();
-    for (uint64_t i = first; i < n; i += stride) {
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kAggThreads) {
         const uint32_t* t = in + i * L.tw;
         const uint64_t k = agg_key(t, L), v = agg_val(t, L);
         bool done = false;
         if (k != kEmpty) {
             uint32_t s = (uint32_t)pj_fin64(k) & (kAggSlots - 1);
             for (uint32_t p = 0; p < kAggProbes; ++p) {
-                const unsigned long long prev = atomicCAS(&lkey[s], (unsigned long long)kEmpty, (unsigned long long)k);
-                if (prev == kEmpty || prev == k) {
+                // Read first: only an empty slot needs the CAS that claims it.
+                unsigned long long cur = __hip_atomic_load(&lkey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cur == kEmpty)
+                    cur = atomicCAS(&lkey[s], (unsigned long long)kEmpty, (unsigned long long)k);
+                if (cur == kEmpty || cur == k) {
                     atomicAdd(&lval[s], (unsigned long long)v);
                     done = true;
                     break;

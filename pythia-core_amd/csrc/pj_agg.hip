@@ -15,6 +15,15 @@
 // lock-free: a 64-bit CAS claims a slot for a key, a 64-bit atomic add folds
 // the value into it, and nothing waits. The key kEmpty marks an empty slot;
 // a group whose key happens to be kEmpty is folded into a slot of its own.
+//
+// Many groups (max_groups >= kPartMinGroups) take the partitioned path
+// instead: the join's radix partition pass turns the tuples into 16-byte
+// {group key, value} records in 2^bits partitions of about kPartGroups
+// groups each, and k_agg_parts aggregates one partition a block in an LDS
+// table of kPartSlots groups and writes the groups itself. Partitions hold
+// disjoint keys, so only a group that finds no LDS slot (a partition with
+// more groups than expected) goes to the global table, and the global table
+// is cleared and scanned only after a run that used it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,6 +39,14 @@ constexpr uint64_t kEmpty = 0xA5A5A5A5A5A5A5A5ull;
 constexpr uint32_t kAggThreads = 512;
 constexpr uint32_t kAggSlots = 4096;     // LDS groups a block: 64 KiB of keys and sums
 constexpr uint32_t kAggProbes = 8;
+constexpr uint32_t kPartThreads = 1024;
+constexpr uint32_t kPartSlots = 8192;    // LDS groups a partition: 128 KiB of keys and sums
+constexpr uint32_t kPartProbes = 16;
+constexpr uint32_t kPartGroups = 4096;   // expected groups a partition (half the slots)
+constexpr uint64_t kPartMinGroups = 1ull << 16;
+constexpr uint64_t kPartMinTuples = 1ull << 20;
+constexpr uint32_t kPartMaxBits = 13;
+constexpr uint32_t kMiscWords = 8;
 
 // How a tuple's key and value are read: 32-bit words at byte offsets.
 struct AggLayout {
@@ -45,7 +62,8 @@ struct AggLayout {
 struct AggTable {
     unsigned long long* key;   // G slots
     unsigned long long* val;
-    unsigned long long* misc;  // [0] kEmpty group's value, [1] its tuples, [2] overflow, [3] groups emitted
+    unsigned long long* misc;  // [0] kEmpty group's value, [1] its tuples, [2] overflow, [3] groups emitted,
+                               // [4] nonzero once a key went to the table (partitioned path)
     uint32_t mask;             // G - 1
 };
 
@@ -160,13 +178,9 @@ __device__ __forceinline__ void put64(uint32_t* o, uint64_t v)
     o[1] = (uint32_t)(v >> 32);
 }
 
-__device__ __forceinline__ void emit_group(uint32_t* __restrict__ out, uint64_t cap, const AggLayout& L,
-                                           uint32_t ow, const AggTable& G, uint64_t k, uint64_t v)
+// One output tuple: the grouping columns, then the sum or count.
+__device__ __forceinline__ void write_group(uint32_t* __restrict__ o, const AggLayout& L, uint64_t k, uint64_t v)
 {
-    const unsigned long long pos = atomicAdd(&G.misc[3], 1ull);
-    if (pos >= cap)
-        return;
-    uint32_t* o = out + pos * ow;
     uint32_t w = 0;
     for (uint32_t c = 0; c < L.nkey; ++c) {
         if (L.nkey == 2) {   // two INTEGER columns: low then high half of the key
@@ -184,9 +198,20 @@ __device__ __forceinline__ void emit_group(uint32_t* __restrict__ out, uint64_t 
         o[w] = (uint32_t)v;
 }
 
-__global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __restrict__ out, uint64_t cap)
+__device__ __forceinline__ void emit_group(uint32_t* __restrict__ out, uint64_t cap, const AggLayout& L,
+                                           uint32_t ow, const AggTable& G, uint64_t k, uint64_t v)
+{
+    const unsigned long long pos = atomicAdd(&G.misc[3], 1ull);
+    if (pos < cap)
+        write_group(out + pos * ow, L, k, v);
+}
+
+__global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __restrict__ out, uint64_t cap,
+                           uint32_t only_if_used)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (only_if_used && G.misc[4] == 0)
+        return;
     if (s <= G.mask) {
         const uint64_t k = G.key[s];
         if (k != kEmpty)
@@ -196,15 +221,96 @@ __global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __res
         emit_group(out, cap, L, ow, G, kEmpty, G.misc[0]);
 }
 
-__global__ void k_agg_clear(AggTable G)
+__global__ void k_agg_clear(AggTable G, uint32_t table)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s <= G.mask) {
+    if (table && s <= G.mask) {
         G.key[s] = kEmpty;
         G.val[s] = 0;
     }
-    if (s < 4)
+    if (s < kMiscWords)
         G.misc[s] = 0;
+}
+
+// One partition of {key, value} records a block (grid-stride over the
+// partitions): fold into an LDS table, then write the table's groups to the
+// output, one atomic a wave-chunk of slots for their positions. Records
+// whose key belongs to another partition are the partition pass's padding.
+__global__ __launch_bounds__(kPartThreads) void k_agg_parts(const uint4* __restrict__ recs,
+                                                            const uint32_t* __restrict__ starts, uint32_t bits,
+                                                            uint32_t F, uint32_t count, AggTable G, AggLayout L,
+                                                            uint32_t ow, uint32_t* __restrict__ out, uint64_t cap)
+{
+    extern __shared__ unsigned long long pl[];
+    unsigned long long* lkey = pl;                 // kPartSlots
+    unsigned long long* lval = pl + kPartSlots;    // kPartSlots
+    __shared__ uint32_t wcnt[kPartThreads / 64];
+    __shared__ unsigned long long cbase;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t p = blockIdx.x; p < F; p += gridDim.x) {
+        for (uint32_t s = tid; s < kPartSlots; s += kPartThreads) {
+            lkey[s] = kEmpty;
+            lval[s] = 0;
+        }
+        __syncthreads();
+        const uint32_t lo = starts[p], hi = starts[p + 1];
+        bool used_global = false;
+        for (uint32_t i = lo + tid; i < hi; i += kPartThreads) {
+            const uint4 r = recs[i];
+            const uint64_t k = rec_key(r);
+            if (radix_part(key_mix(k), bits) != p)
+                continue;   // padding
+            const uint64_t v = count ? 1ull : ((uint64_t)r.z | ((uint64_t)r.w << 32));
+            bool done = false;
+            if (k != kEmpty) {
+                uint32_t s = (uint32_t)key_mix(k) & (kPartSlots - 1);
+                for (uint32_t q = 0; q < kPartProbes; ++q) {
+                    unsigned long long cur = __hip_atomic_load(&lkey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (cur == kEmpty)
+                        cur = atomicCAS(&lkey[s], (unsigned long long)kEmpty, (unsigned long long)k);
+                    if (cur == kEmpty || cur == k) {
+                        atomicAdd(&lval[s], (unsigned long long)v);
+                        done = true;
+                        break;
+                    }
+                    s = (s + 1) & (kPartSlots - 1);
+                }
+            }
+            if (!done) {
+                global_fold(G, k, v);
+                used_global = true;
+            }
+        }
+        if (used_global)
+            G.misc[4] = 1;
+        __syncthreads();
+        for (uint32_t c = 0; c < kPartSlots; c += kPartThreads) {
+            const uint32_t s = c + tid;
+            const uint64_t k = lkey[s];
+            const bool has = k != kEmpty;
+            const uint64_t m = __ballot(has);
+            const uint32_t rank = (uint32_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+            if (lane == 0)
+                wcnt[wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t tot = 0;
+                for (uint32_t w = 0; w < kPartThreads / 64; ++w) {
+                    const uint32_t x = wcnt[w];
+                    wcnt[w] = tot;
+                    tot += x;
+                }
+                cbase = tot ? atomicAdd(&G.misc[3], (unsigned long long)tot) : 0ull;
+            }
+            __syncthreads();
+            if (has) {
+                const uint64_t pos = cbase + wcnt[wave] + rank;
+                if (pos < cap)
+                    write_group(out + pos * ow, L, k, lval[s]);
+            }
+            __syncthreads();
+        }
+    }
 }
 
 }  // namespace
@@ -215,12 +321,36 @@ using namespace pj;
 struct pj_agg_s {
     pj_agg_desc desc;
     AggLayout L;
+    SideLayout rec;                  // tuple -> {group key, value} record of the partitioned path
+    uint32_t bits = 0;               // radix bits of the partitioned path
     uint32_t out_words = 0;
     uint32_t slots = 0;
     DevBuf table;
+    bool table_dirty = true;         // the global table may hold keys from an earlier run
+    RadixRecords parts;
     uint64_t* host_misc = nullptr;   // pinned readback of the table's counters
     std::mutex mu;
 };
+
+namespace {
+
+int allow_part_lds()
+{
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    PJ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    if (dev >= 0 && dev < 64 && done[dev])
+        return PJ_OK;
+    PJ_HIP(hipFuncSetAttribute((const void*)k_agg_parts, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(2 * kPartSlots * 8)));
+    if (dev >= 0 && dev < 64)
+        done[dev] = true;
+    return PJ_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -283,6 +413,16 @@ int pj_agg_create(const pj_agg_desc* desc, pj_agg_t* out)
     h->desc = *desc;
     h->L = L;
     h->out_words = ow;
+    SideLayout& R = h->rec;   // key words as the partition pass reads them; value words, or none for a count
+    R.tw = L.tw;
+    R.k0 = (int32_t)L.key_off[0];
+    R.k1 = L.nkey == 2 ? (int32_t)L.key_off[1] : L.key_wide[0] ? (int32_t)L.key_off[0] + 1 : -1;
+    R.p0 = L.count ? -1 : (int32_t)L.val_off;
+    R.p1 = L.count || !L.val_wide ? -1 : (int32_t)L.val_off + 1;
+    R.ident = R.tw == 4 && R.k0 == 0 && R.k1 == 1 && R.p0 == 2 && R.p1 == 3;
+    while (h->bits < kPartMaxBits && ((uint64_t)kPartGroups << h->bits) < desc->max_groups)
+        ++h->bits;
+    h->bits = std::max<uint32_t>(h->bits, 1);
     uint64_t slots = 1024;
     while (slots < 2 * desc->max_groups)
         slots <<= 1;
@@ -310,17 +450,31 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
     hipStream_t st = (hipStream_t)stream;
     const uint64_t slots = h->slots;
     int rc;
-    if ((rc = h->table.ensure((2 * slots + 4) * 8)))
+    if ((rc = h->table.ensure((2 * slots + kMiscWords) * 8)))
         return rc;
     if (!h->host_misc)
-        PJ_HIP(hipHostMalloc((void**)&h->host_misc, 64, hipHostMallocDefault));
+        PJ_HIP(hipHostMalloc((void**)&h->host_misc, kMiscWords * 8, hipHostMallocDefault));
     AggTable G;
     G.key = (unsigned long long*)h->table.p;
     G.val = G.key + slots;
     G.misc = G.val + slots;
     G.mask = (uint32_t)(slots - 1);
-    hipLaunchKernelGGL(k_agg_clear, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G);
-    if (n) {
+    const bool parts = h->L.nkey > 0 && h->desc.max_groups >= kPartMinGroups && n >= kPartMinTuples &&
+                       n < (1ull << 31);
+    const bool clear_table = !parts || h->table_dirty;
+    h->table_dirty = true;   // until a finished run shows otherwise
+    hipLaunchKernelGGL(k_agg_clear, dim3(clear_table ? (uint32_t)((slots + 255) / 256) : 1u), dim3(256), 0, st, G,
+                       clear_table ? 1u : 0u);
+    if (parts) {
+        if ((rc = radix_records(h->parts, d_tuples, n, h->rec, h->bits, st)) || (rc = allow_part_lds()))
+            return rc;
+        const uint32_t F = 1u << h->bits;
+        hipLaunchKernelGGL(k_agg_parts, dim3(F), dim3(kPartThreads), 2 * kPartSlots * 8, st,
+                           (const uint4*)h->parts.recs.p, (const uint32_t*)h->parts.starts.p, h->bits, F, h->L.count,
+                           G, h->L, h->out_words, (uint32_t*)d_groups, groups_cap);
+        hipLaunchKernelGGL(k_agg_emit, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G, h->L,
+                           h->out_words, (uint32_t*)d_groups, groups_cap, 1u);
+    } else if (n) {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
@@ -329,11 +483,12 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
         hipLaunchKernelGGL(k_agg, dim3(grid), dim3(kAggThreads), 0, st, (const uint32_t*)d_tuples, n, h->L, G);
         hipLaunchKernelGGL(k_agg_emit, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G, h->L,
-                           h->out_words, (uint32_t*)d_groups, groups_cap);
+                           h->out_words, (uint32_t*)d_groups, groups_cap, 0u);
     }
     PJ_HIP(hipGetLastError());
-    PJ_HIP(hipMemcpyAsync(h->host_misc, G.misc, 32, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipMemcpyAsync(h->host_misc, G.misc, kMiscWords * 8, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
+    h->table_dirty = !parts || h->host_misc[4] != 0;
     *ngroups = h->host_misc[3];
     if (h->host_misc[2])
         return fail(PJ_ERR_CAPACITY, "more distinct groups than the table's %llu slots (max_groups %llu)",

@@ -135,6 +135,16 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Radix partitioning for consumers other than the join: n tuples of layout L
+// become 16-byte records {key, payload} in 2^bits partitions by the top bits
+// of the mixed key, partition p at records [starts[p], starts[p + 1]) of
+// recs (device words). A partition holds padding records whose key belongs
+// to another partition; consumers skip them by recomputing the partition.
+struct RadixRecords {
+    DevBuf recs, starts, hist, bsum, pids;
+};
+int radix_records(RadixRecords& r, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits, hipStream_t st);
+
 // The range partitioning chosen by the learned model for one build side.
 struct RangePlan {
     bool ok = false;                 // false: the model declined (radix is used)

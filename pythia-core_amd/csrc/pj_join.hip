@@ -1135,9 +1135,9 @@ uint32_t part_blocks(uint64_t n)
 // ordered keys, build side only) then receives each partition's key range,
 // and a write-combining scatter takes each tuple's partition from the
 // histogram pass through h->pids instead of searching again.
-int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
-                   bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts, bool host_copy,
-                   unsigned long long* pminmax, hipStream_t st)
+int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in, uint64_t n, const SideLayout& L,
+                   const PartFn& pf, bool range, bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts,
+                   bool host_copy, unsigned long long* pminmax, hipStream_t st)
 {
     int rc;
     const uint32_t F = pf.F;
@@ -1167,12 +1167,12 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     const uint64_t cap = n + (uint64_t)F * nblk * (B - 1);
     if (cap >= (1ull << 32))
         return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");
-    if ((rc = dst.ensure(cap * 16)) || (rc = h->hist.ensure(m * 4)) || (rc = h->bsum.ensure((nsb + 1) * 4)))
+    if ((rc = dst.ensure(cap * 16)) || (rc = histb.ensure(m * 4)) || (rc = bsumb.ensure((nsb + 1) * 4)))
         return rc;
     const uint32_t* in = (const uint32_t*)d_in;
     const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
-    uint32_t* hist = (uint32_t*)h->hist.p;
-    uint32_t* bsum = (uint32_t*)h->bsum.p;
+    uint32_t* hist = (uint32_t*)histb.p;
+    uint32_t* bsum = (uint32_t*)bsumb.p;
     unsigned long long* pmin = pminmax;
     unsigned long long* pmax = pminmax ? pminmax + F : nullptr;
     if (pminmax) {
@@ -1182,9 +1182,9 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     }
     uint16_t* pid = nullptr;
     if (range && F <= 65536) {
-        if ((rc = h->pids.ensure(n * 2)))
+        if ((rc = pidsb.ensure(n * 2)))
             return rc;
-        pid = (uint16_t*)h->pids.p;
+        pid = (uint16_t*)pidsb.p;
     }
     rc = range ? launch_hist<true>(ident, in, n, L, pf, chunk, nblk, B, hist, pmin, pmax, pid, st)
                : launch_hist<false>(ident, in, n, L, pf, chunk, nblk, B, hist, nullptr, nullptr, nullptr, st);
@@ -1216,6 +1216,14 @@ int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout&
     if (host_copy)
         PJ_HIP(hipMemcpyAsync(P.start.data(), starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
     return PJ_OK;
+}
+
+int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
+                   bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts, bool host_copy,
+                   unsigned long long* pminmax, hipStream_t st)
+{
+    return partition_bufs(h->hist, h->bsum, h->pids, d_in, n, L, pf, range, write_combine, dst, P, starts, host_copy,
+                          pminmax, st);
 }
 
 int device_count_cus()
@@ -1528,6 +1536,17 @@ int pj::join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void*
 // ======================================================================
 // C ABI
 // ======================================================================
+
+namespace pj {
+
+int radix_records(RadixRecords& r, const void* d_in, uint64_t n, const SideLayout& L, uint32_t bits, hipStream_t st)
+{
+    Partitioned P;
+    return partition_bufs(r.hist, r.bsum, r.pids, d_in, n, L, radix_fn(bits), false, true, r.recs, P, r.starts, false,
+                          nullptr, st);
+}
+
+}  // namespace pj
 
 extern "C" {
 

@@ -134,3 +134,73 @@ def test_join_then_aggregate_on_the_device(fields, op, sumfield):
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
     j.close()
     agg.close()
+
+
+# The partitioned path: at least 2^16 groups expected and 2^20 tuples.
+PART_CASES = [
+    (["long", "long"], [0], "count", None, {0: 200000}, 1 << 18),
+    (["long", "long"], [0], "sum", 1, {0: 200000}, 1 << 18),
+    (["int", "int", "int"], [0, 1], "sum", 2, {0: 400, 1: 500}, 1 << 17),   # INTEGER sums that wrap
+    (["int", "long"], [0], "sum", 1, {0: 100000}, 1 << 17),
+    (["long", "int", "date"], [2], "count", None, {2: 70000}, 1 << 16),
+]
+
+
+@pytest.mark.parametrize("cols,fields,op,sumfield,groups,max_groups", PART_CASES)
+def test_partitioned_aggregate_equals_oracle(cols, fields, op, sumfield, groups, max_groups):
+    _say("case: partitioned agg %s fields=%s %s max_groups=%d" % (cols, fields, op, max_groups))
+    rng = np.random.default_rng(max_groups + len(cols))
+    _, rows = _table(cols, 1 << 21, rng, groups)
+    exp = O.aggregate(cols, fields, op, sumfield, rows)
+    got = _device_agg(cols, fields, op, sumfield, rows, max_groups=max_groups)
+    assert got.shape == exp.shape
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+def test_partitioned_aggregate_overflow_and_reuse():
+    """One handle, several runs: partitions with more groups than their LDS
+    table send the rest to the global table (which the next run clears),
+    then a run that fits, then the empty-slot marker key among many groups."""
+    _say("case: partitioned agg overflow / reuse")
+    cols = ["long", "long"]
+    agg = P.Aggregate(cols, [0], "sum", 1, max_groups=1 << 16)   # 16 partitions of 8192 LDS slots
+    cap = 1 << 18
+    out = torch.empty(cap * agg.out_width, dtype=torch.uint8, device="cuda")
+    marker = np.uint64(0xA5A5A5A5A5A5A5A5).view(np.int64)
+    for run, (g, n, mark) in enumerate([(120000, 1 << 21, False), (30000, 1 << 20, False),
+                                        (120000, 1 << 21, True), (50000, 1 << 20, False)]):
+        rng = np.random.default_rng(100 + run)
+        t, rows = _table(cols, n, rng, {0: g})
+        if mark:
+            t["c0"][::1000] = marker
+        exp = O.aggregate(cols, [0], "sum", 1, rows)
+        d = torch.from_numpy(rows.reshape(-1).copy()).cuda()
+        ng = agg.run_device(d.data_ptr(), n, out.data_ptr(), cap)
+        got = out[: ng * agg.out_width].cpu().numpy().reshape(ng, agg.out_width)
+        assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp)), "run %d" % run
+    agg.close()
+
+
+def test_join_then_partitioned_count_by_key():
+    """count per build key over a 2M-probe join's output in HBM: the
+    partitioned path (2^17 expected groups)."""
+    _say("case: join -> partitioned count by key")
+    nb, npr = 1 << 17, 1 << 21
+    b, p = O.gen_pkfk(58, 1 << 40, nb, npr, npr // 10)
+    proj = ["B$0", "B$1", "P$1"]
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], proj, buckets=1 << 17)
+    exp_join, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    cols = ["long", "long", "long"]
+    exp = O.aggregate(cols, [0], "count", None, exp_join)
+    db = torch.from_numpy(b.view(np.uint8).copy()).cuda()
+    dp = torch.from_numpy(p.view(np.uint8).copy()).cuda()
+    j = P.Join(P.make_desc(["long", "long"], ["long", "long"], [0], [0], proj))
+    out = torch.empty(npr * j.out_width, dtype=torch.uint8, device="cuda")
+    n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr)
+    agg = P.Aggregate(cols, [0], "count", None, max_groups=nb)
+    groups = torch.empty(nb * agg.out_width, dtype=torch.uint8, device="cuda")
+    g = agg.run_device(out.data_ptr(), n, groups.data_ptr(), nb)
+    got = groups[: g * agg.out_width].cpu().numpy().reshape(g, agg.out_width)
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+    j.close()
+    agg.close()

@@ -42,6 +42,7 @@ constexpr uint32_t kAggProbes = 8;
 constexpr uint32_t kPartThreads = 1024;
 constexpr uint32_t kPartSlots = 8192;    // LDS groups a partition: 128 KiB of keys and sums
 constexpr uint32_t kPartProbes = 16;
+constexpr uint32_t kPartUnroll = 8;
 constexpr uint32_t kPartGroups = 4096;   // expected groups a partition (half the slots)
 constexpr uint64_t kPartMinGroups = 1ull << 16;
 constexpr uint64_t kPartMinTuples = 1ull << 20;
@@ -209,15 +210,15 @@ __device__ __forceinline__ void emit_group(uint32_t* __restrict__ out, uint64_t 
 __global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __restrict__ out, uint64_t cap,
                            uint32_t only_if_used)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (only_if_used && G.misc[4] == 0)
         return;
-    if (s <= G.mask) {
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t s = first; s <= G.mask; s += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t k = G.key[s];
         if (k != kEmpty)
             emit_group(out, cap, L, ow, G, k, G.val[s]);
     }
-    if (s == 0 && G.misc[1])
+    if (first == 0 && G.misc[1])
         emit_group(out, cap, L, ow, G, kEmpty, G.misc[0]);
 }
 
@@ -255,30 +256,57 @@ __global__ __launch_bounds__(kPartThreads) void k_agg_parts(const uint4* __restr
         __syncthreads();
         const uint32_t lo = starts[p], hi = starts[p + 1];
         bool used_global = false;
-        for (uint32_t i = lo + tid; i < hi; i += kPartThreads) {
-            const uint4 r = recs[i];
-            const uint64_t k = rec_key(r);
-            if (radix_part(key_mix(k), bits) != p)
-                continue;   // padding
-            const uint64_t v = count ? 1ull : ((uint64_t)r.z | ((uint64_t)r.w << 32));
-            bool done = false;
-            if (k != kEmpty) {
-                uint32_t s = (uint32_t)key_mix(k) & (kPartSlots - 1);
-                for (uint32_t q = 0; q < kPartProbes; ++q) {
-                    unsigned long long cur = __hip_atomic_load(&lkey[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cur == kEmpty)
-                        cur = atomicCAS(&lkey[s], (unsigned long long)kEmpty, (unsigned long long)k);
-                    if (cur == kEmpty || cur == k) {
-                        atomicAdd(&lval[s], (unsigned long long)v);
-                        done = true;
-                        break;
-                    }
-                    s = (s + 1) & (kPartSlots - 1);
-                }
+        // kPartUnroll records a thread in flight before any of them is folded
+        for (uint32_t b0 = lo; b0 < hi; b0 += kPartThreads * kPartUnroll) {
+            uint4 r[kPartUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kPartUnroll; ++u) {
+                const uint32_t i = b0 + u * kPartThreads + tid;
+                if (i < hi)
+                    r[u] = ld_stream(recs + i);
             }
-            if (!done) {
-                global_fold(G, k, v);
-                used_global = true;
+#pragma unroll
+            for (uint32_t u = 0; u < kPartUnroll; ++u) {
+                const uint32_t i = b0 + u * kPartThreads + tid;
+                if (i >= hi)
+                    break;
+                const uint64_t k = rec_key(r[u]);
+                const uint64_t h = key_mix(k);
+                if (radix_part(h, bits) != p)
+                    continue;   // padding
+                const uint64_t v = count ? 1ull : ((uint64_t)r[u].z | ((uint64_t)r[u].w << 32));
+                bool done = false;
+                if (k != kEmpty) {
+                    // linear probing over aligned 4-slot buckets: the four keys are
+                    // read together, then the first empty slot is claimed
+                    uint32_t s = (uint32_t)h & (kPartSlots - 4);
+                    uint32_t hs = 0;
+                    for (uint32_t q = 0; q < kPartProbes / 4 && !done; ++q) {
+                        unsigned long long c[4];
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; ++j)
+                            c[j] = __hip_atomic_load(&lkey[s + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+                        for (uint32_t j = 0; j < 4; ++j) {
+                            if (!done) {
+                                unsigned long long cur = c[j];
+                                if (cur == kEmpty)
+                                    cur = atomicCAS(&lkey[s + j], (unsigned long long)kEmpty, (unsigned long long)k);
+                                if (cur == kEmpty || cur == k) {
+                                    hs = s + j;
+                                    done = true;
+                                }
+                            }
+                        }
+                        s = (s + 4) & (kPartSlots - 1);
+                    }
+                    if (done)
+                        atomicAdd(&lval[hs], (unsigned long long)v);
+                }
+                if (!done) {
+                    global_fold(G, k, v);
+                    used_global = true;
+                }
             }
         }
         if (used_global)
@@ -333,6 +361,12 @@ struct pj_agg_s {
 };
 
 namespace {
+
+// Blocks of 256 threads for a grid-stride pass over the global table.
+uint32_t emit_blocks(uint64_t slots)
+{
+    return (uint32_t)std::min<uint64_t>((slots + 255) / 256, 2048);
+}
 
 int allow_part_lds()
 {
@@ -472,8 +506,8 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
         hipLaunchKernelGGL(k_agg_parts, dim3(F), dim3(kPartThreads), 2 * kPartSlots * 8, st,
                            (const uint4*)h->parts.recs.p, (const uint32_t*)h->parts.starts.p, h->bits, F, h->L.count,
                            G, h->L, h->out_words, (uint32_t*)d_groups, groups_cap);
-        hipLaunchKernelGGL(k_agg_emit, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G, h->L,
-                           h->out_words, (uint32_t*)d_groups, groups_cap, 1u);
+        hipLaunchKernelGGL(k_agg_emit, dim3(emit_blocks(slots)), dim3(256), 0, st, G, h->L, h->out_words,
+                           (uint32_t*)d_groups, groups_cap, 1u);
     } else if (n) {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) != hipSuccess ||
@@ -482,8 +516,8 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
         const uint64_t want = (n + 8191) / 8192;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)cus * 4));
         hipLaunchKernelGGL(k_agg, dim3(grid), dim3(kAggThreads), 0, st, (const uint32_t*)d_tuples, n, h->L, G);
-        hipLaunchKernelGGL(k_agg_emit, dim3((uint32_t)((slots + 255) / 256)), dim3(256), 0, st, G, h->L,
-                           h->out_words, (uint32_t*)d_groups, groups_cap, 0u);
+        hipLaunchKernelGGL(k_agg_emit, dim3(emit_blocks(slots)), dim3(256), 0, st, G, h->L, h->out_words,
+                           (uint32_t*)d_groups, groups_cap, 0u);
     }
     PJ_HIP(hipGetLastError());
     PJ_HIP(hipMemcpyAsync(h->host_misc, G.misc, kMiscWords * 8, hipMemcpyDeviceToHost, st));

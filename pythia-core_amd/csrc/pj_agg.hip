@@ -41,13 +41,14 @@ constexpr uint32_t kAggSlots = 4096;     // LDS groups a block: 64 KiB of keys a
 constexpr uint32_t kAggProbes = 8;
 constexpr uint32_t kPartThreads = 1024;
 constexpr uint32_t kPartSlots = 8192;    // LDS groups a partition: 128 KiB of keys and sums
-constexpr uint32_t kPartProbes = 16;
+constexpr uint32_t kPartProbes = 32;
 constexpr uint32_t kPartUnroll = 8;
 constexpr uint32_t kPartGroups = 4096;   // expected groups a partition (half the slots)
 constexpr uint64_t kPartMinGroups = 1ull << 16;
 constexpr uint64_t kPartMinTuples = 1ull << 20;
 constexpr uint32_t kPartMaxBits = 13;
 constexpr uint32_t kMiscWords = 8;
+constexpr uint32_t kClaimedCap = 1u << 16;   // claimed slots listed for a short emit and clear
 
 // How a tuple's key and value are read: 32-bit words at byte offsets.
 struct AggLayout {
@@ -64,7 +65,8 @@ struct AggTable {
     unsigned long long* key;   // G slots
     unsigned long long* val;
     unsigned long long* misc;  // [0] kEmpty group's value, [1] its tuples, [2] overflow, [3] groups emitted,
-                               // [4] nonzero once a key went to the table (partitioned path)
+                               // [4] nonzero once a key went to the table (partitioned path), [5] slots claimed
+    uint32_t* claimed;         // the first kClaimedCap slots claimed, in claim order
     uint32_t mask;             // G - 1
 };
 
@@ -100,6 +102,11 @@ __device__ __forceinline__ void global_fold(const AggTable& G, uint64_t k, uint6
         const unsigned long long prev = atomicCAS(&G.key[s], (unsigned long long)kEmpty, (unsigned long long)k);
         if (prev == kEmpty || prev == k) {
             atomicAdd(&G.val[s], (unsigned long long)v);
+            if (prev == kEmpty) {
+                const unsigned long long c = atomicAdd(&G.misc[5], 1ull);
+                if (c < kClaimedCap)
+                    G.claimed[c] = s;
+            }
             return;
         }
         s = (s + 1) & G.mask;
@@ -213,21 +220,37 @@ __global__ void k_agg_emit(AggTable G, AggLayout L, uint32_t ow, uint32_t* __res
     if (only_if_used && G.misc[4] == 0)
         return;
     const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t s = first; s <= G.mask; s += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = G.key[s];
-        if (k != kEmpty)
-            emit_group(out, cap, L, ow, G, k, G.val[s]);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t nclaimed = G.misc[5];
+    if (only_if_used && nclaimed <= kClaimedCap) {   // few groups in the table: the claim list names them
+        for (uint64_t i = first; i < nclaimed; i += stride) {
+            const uint32_t s = G.claimed[i];
+            emit_group(out, cap, L, ow, G, G.key[s], G.val[s]);
+        }
+    } else {
+        for (uint64_t s = first; s <= G.mask; s += stride) {
+            const uint64_t k = G.key[s];
+            if (k != kEmpty)
+                emit_group(out, cap, L, ow, G, k, G.val[s]);
+        }
     }
     if (first == 0 && G.misc[1])
         emit_group(out, cap, L, ow, G, kEmpty, G.misc[0]);
 }
 
-__global__ void k_agg_clear(AggTable G, uint32_t table)
+// Clears the whole table (table = 1), or only the nlisted slots of the
+// previous run's claim list, and the counters.
+__global__ void k_agg_clear(AggTable G, uint32_t table, uint32_t nlisted)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (table && s <= G.mask) {
         G.key[s] = kEmpty;
         G.val[s] = 0;
+    }
+    if (!table && s < nlisted) {
+        const uint32_t t = G.claimed[s];
+        G.key[t] = kEmpty;
+        G.val[t] = 0;
     }
     if (s < kMiscWords)
         G.misc[s] = 0;
@@ -355,6 +378,7 @@ struct pj_agg_s {
     uint32_t slots = 0;
     DevBuf table;
     bool table_dirty = true;         // the global table may hold keys from an earlier run
+    uint64_t claimed_prev = ~0ull;   // slots the last run claimed (all listed when <= kClaimedCap)
     RadixRecords parts;
     uint64_t* host_misc = nullptr;   // pinned readback of the table's counters
     std::mutex mu;
@@ -484,7 +508,7 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
     hipStream_t st = (hipStream_t)stream;
     const uint64_t slots = h->slots;
     int rc;
-    if ((rc = h->table.ensure((2 * slots + kMiscWords) * 8)))
+    if ((rc = h->table.ensure((2 * slots + kMiscWords) * 8 + kClaimedCap * 4)))
         return rc;
     if (!h->host_misc)
         PJ_HIP(hipHostMalloc((void**)&h->host_misc, kMiscWords * 8, hipHostMallocDefault));
@@ -492,13 +516,19 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
     G.key = (unsigned long long*)h->table.p;
     G.val = G.key + slots;
     G.misc = G.val + slots;
+    G.claimed = (uint32_t*)(G.misc + kMiscWords);
     G.mask = (uint32_t)(slots - 1);
     const bool parts = h->L.nkey > 0 && h->desc.max_groups >= kPartMinGroups && n >= kPartMinTuples &&
                        n < (1ull << 31);
-    const bool clear_table = !parts || h->table_dirty;
+    // the whole table, unless the last run left nothing in it or listed every slot it claimed
+    const bool clear_list = parts && h->table_dirty && h->claimed_prev <= kClaimedCap;
+    const bool clear_table = !parts || (h->table_dirty && !clear_list);
+    const uint32_t nlisted = clear_list ? (uint32_t)h->claimed_prev : 0u;
+    const uint64_t clear_threads = clear_table ? slots : std::max<uint64_t>(nlisted, kMiscWords);
     h->table_dirty = true;   // until a finished run shows otherwise
-    hipLaunchKernelGGL(k_agg_clear, dim3(clear_table ? (uint32_t)((slots + 255) / 256) : 1u), dim3(256), 0, st, G,
-                       clear_table ? 1u : 0u);
+    h->claimed_prev = ~0ull;
+    hipLaunchKernelGGL(k_agg_clear, dim3((uint32_t)((clear_threads + 255) / 256)), dim3(256), 0, st, G,
+                       clear_table ? 1u : 0u, nlisted);
     if (parts) {
         if ((rc = radix_records(h->parts, d_tuples, n, h->rec, h->bits, st)) || (rc = allow_part_lds()))
             return rc;
@@ -523,6 +553,7 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
     PJ_HIP(hipMemcpyAsync(h->host_misc, G.misc, kMiscWords * 8, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));
     h->table_dirty = !parts || h->host_misc[4] != 0;
+    h->claimed_prev = h->host_misc[5];
     *ngroups = h->host_misc[3];
     if (h->host_misc[2])
         return fail(PJ_ERR_CAPACITY, "more distinct groups than the table's %llu slots (max_groups %llu)",

@@ -157,22 +157,30 @@ def test_partitioned_aggregate_equals_oracle(cols, fields, op, sumfield, groups,
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
 
 
-def test_partitioned_aggregate_overflow_and_reuse():
-    """One handle, several runs: partitions with more groups than their LDS
-    table send the rest to the global table (which the next run clears),
-    then a run that fits, then the empty-slot marker key among many groups."""
-    _say("case: partitioned agg overflow / reuse")
+def test_aggregate_handle_reuse_across_paths():
+    """One handle, runs in a row that take both paths: the global table's
+    state carries over (whole-table clear, clear of the listed claims, or
+    none), partitions with more groups than their LDS table send the rest to
+    the global table, more claims than the claim list holds fall back to a
+    whole-table scan, and the empty-slot marker key appears among many."""
+    _say("case: agg handle reuse")
     cols = ["long", "long"]
     agg = P.Aggregate(cols, [0], "sum", 1, max_groups=1 << 16)   # 16 partitions of 8192 LDS slots
     cap = 1 << 18
     out = torch.empty(cap * agg.out_width, dtype=torch.uint8, device="cuda")
     marker = np.uint64(0xA5A5A5A5A5A5A5A5).view(np.int64)
-    for run, (g, n, mark) in enumerate([(120000, 1 << 21, False), (30000, 1 << 20, False),
-                                        (120000, 1 << 21, True), (50000, 1 << 20, False)]):
+    runs = [(120000, 600000, False),     # global table, more claims than listed
+            (120000, 1 << 21, False),    # partitioned, overflowing partitions
+            (30000, 1 << 20, False),     # partitioned, clears the listed claims
+            (120000, 1 << 21, True),     # partitioned, marker key
+            (60000, 600000, False),      # global table, every claim listed
+            (50000, 1 << 20, False),     # partitioned after it
+            (10, 1000, True)]            # small, marker key
+    for run, (g, n, mark) in enumerate(runs):
         rng = np.random.default_rng(100 + run)
         t, rows = _table(cols, n, rng, {0: g})
         if mark:
-            t["c0"][::1000] = marker
+            t["c0"][::97] = marker
         exp = O.aggregate(cols, [0], "sum", 1, rows)
         d = torch.from_numpy(rows.reshape(-1).copy()).cuda()
         ng = agg.run_device(d.data_ptr(), n, out.data_ptr(), cap)

@@ -366,7 +366,10 @@ int pj_agg_out_width(pj_agg_t h, uint32_t* width);
 /* Aggregates n tuples at d_tuples (4-byte aligned) into d_groups (device
  * memory, room for groups_cap output tuples). *ngroups gets the number of
  * groups; PJ_ERR_CAPACITY when they exceed groups_cap or the table's
- * max_groups (twice that many slots). Synchronous. */
+ * twice max_groups slots. With max_groups >= 2^16 and 2^20 <= n < 2^31 the
+ * tuples are radix-partitioned by group first (device memory of about 16
+ * bytes a tuple, kept by the handle). Synchronous; calls on one handle are
+ * serialised by its lock. */
 int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_groups, uint64_t groups_cap,
                       uint64_t* ngroups, void* stream);
 int pj_agg_destroy(pj_agg_t h);

@@ -374,6 +374,24 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
                       uint64_t* ngroups, void* stream);
 int pj_agg_destroy(pj_agg_t h);
 
+/* ------------------------------------------------------------------------
+ * Raw binary tables into HBM: the binary scan source of the reference,
+ * MemMappedTable::load (operators/loaders/table.cpp:170-347), which globs
+ * the pattern and maps every non-empty regular file it names, in glob's
+ * sorted order, as packed tuples with no header. Here the files land back
+ * to back in one device buffer, read through pinned staging buffers by host
+ * threads while earlier chunks are copied to the device.
+ * ---------------------------------------------------------------------- */
+/* Tuples of tuple_width bytes in the files pattern names (for sizing). */
+int pj_binary_tuples(const char* pattern, uint32_t tuple_width, uint64_t* ntuples);
+/* Loads them into d_dst (device memory, room for cap tuples); *ntuples gets
+ * their number. PJ_ERR_CAPACITY when they exceed cap, PJ_ERR_INVALID for a
+ * file whose size is not a whole number of tuples (the reference would read
+ * a partial last tuple). No match is an empty table. The copies are on
+ * stream; returns after they have completed. */
+int pj_load_binary(const char* pattern, uint32_t tuple_width, void* d_dst, uint64_t cap, uint64_t* ntuples,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,9 @@ _SIGS = {
     "pj_batch_run_times": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pj_agg_create": (C.c_int, [C.POINTER(AggDesc), C.POINTER(C.c_void_p)]),
     "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "pj_load_binary": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                 C.c_void_p]),
     "pj_agg_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                     C.POINTER(C.c_uint64), C.c_void_p]),
     "pj_agg_destroy": (C.c_int, [C.c_void_p]),
@@ -437,3 +440,21 @@ class Aggregate:
             self.close()
         except Exception:
             pass
+
+
+def binary_tuples(pattern, width):
+    """Tuples of `width` bytes in the raw binary files `pattern` globs."""
+    n = C.c_uint64()
+    check(lib().pj_binary_tuples(pattern.encode(), width, C.byref(n)))
+    return n.value
+
+
+def load_binary(pattern, width, d_dst, cap, stream=None):
+    """Loads the raw binary files `pattern` globs into device memory at d_dst
+    (room for cap tuples); returns the number of tuples."""
+    n = C.c_uint64()
+    rc = lib().pj_load_binary(pattern.encode(), width, C.c_void_p(d_dst), cap, C.byref(n), C.c_void_p(stream))
+    if rc == PJ_ERR_CAPACITY:
+        raise PJError(rc, "need room for %d tuples" % n.value)
+    check(rc)
+    return n.value

@@ -274,6 +274,55 @@ def cpu_baseline_batch(seed, sizes):
 PCIE_PEAK_GBS = 64.0   # PCIe 5.0 x16, one direction
 
 
+def bench_sortmerge(args, P, torch, desc, db, nb, dp, npr, nmiss, dom, seed, clustered, zipf, stream):
+    """The device sort-merge join on the same resident inputs and output rows
+    as the hash join: both sides radix-sorted by key, then merged. Checked
+    against the same closed form."""
+    j = P.SortMergeJoin(desc)
+    out = torch.empty(npr * j.out_width, dtype=torch.uint8, device=db.device)
+    for _ in range(args.warmup):
+        n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr, stream)
+    torch.cuda.synchronize()
+    ph = {"extract_ms": 0.0, "sort_ms": 0.0, "merge_ms": 0.0, "total_ms": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr, stream)
+        t = j.timings()
+        for k in ph:
+            ph[k] += getattr(t, k) / args.steps
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    s, _ = P.checksum_device(out.data_ptr(), n, j.out_width, stream)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if clustered or zipf:
+        cdf = P.zipf_cdf(nb, args.zipf)
+        es, ec = (O.expected_clustered(P.cluster_table(seed, nb), npr, cdf, threads=16) if clustered else
+                  O.expected_pairs_zipf(seed, nb, npr, cdf, threads=16))
+    else:
+        es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
+    parity = (n, s) == (ec, es)
+    b_alg = (nb + npr) * 16 + n * j.out_width
+    res = {
+        "metric": "join tuples/s (build+probe), %s; device sort-merge join" % args.workload,
+        "value": (nb + npr) / (ms * 1e-3), "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": args.workload, "algorithm": "sortmerge", "build_tuples": nb, "probe_tuples": npr,
+                   "out_tuples": n, "build_key_bits": t.build_key_bits, "probe_key_bits": t.probe_key_bits},
+        "phases_ms": ph,
+        "parity": {"count": n, "expected_count": ec, "checksum_ok": parity},
+        "roofline": {"bound": "hbm", "kernel": "whole sort-merge join", "achieved": b_alg / (ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    j.close()
+    print(json.dumps(res))
+    if not parity:
+        sys.exit(1)
+
+
 def load_from_files(args, P, torch, db, nb, dp, npr, stream):
     """Writes the generated relations to raw binary files (the probe side in
     four files), drops the device copies' contents, and loads them back with
@@ -322,6 +371,9 @@ def main():
     ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key"],
                     help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
                          "per build key (int64-key workloads)")
+    ap.add_argument("--algorithm", default="hash", choices=["hash", "sortmerge"],
+                    help="hash: the radix hash join; sortmerge: the device sort-merge join (pj_smj_*), "
+                         "same inputs and output rows, to compare against")
     ap.add_argument("--from-files", action="store_true",
                     help="write the relations to raw binary files first and load them into HBM with "
                          "pj_load_binary (timed separately; the join's timing is unchanged)")
@@ -385,6 +437,11 @@ def main():
     if zipf and args.aggregate != "none":
         print("--aggregate runs on the int64-key workloads", file=sys.stderr)
         sys.exit(2)
+    if args.algorithm == "sortmerge":
+        if args.aggregate != "none" or args.partitioner != "radix":
+            print("--algorithm sortmerge runs the join alone", file=sys.stderr)
+            sys.exit(2)
+        return bench_sortmerge(args, P, torch, desc, db, nb, dp, npr, nmiss, dom, seed, clustered, zipf, stream)
     j = P.Join(desc)
     cap = npr
     out = torch.empty(cap * j.out_width, dtype=torch.uint8, device=dev)

@@ -375,6 +375,35 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
 int pj_agg_destroy(pj_agg_t h);
 
 /* ------------------------------------------------------------------------
+ * Sort-merge join: the sortmergejoin / mpsmjoin plan nodes (query.cpp:167-184,
+ * SortMergeJoinOp and MPSMJoinOp, operators/join.cpp:905-1791) on the
+ * device. Same descriptor, output rows and limits as pj_join_*, and the same
+ * output multiset; it exists to be timed against the hash join. Unlike the
+ * reference's SortMergeJoinOp it also takes two INTEGER key columns.
+ * ---------------------------------------------------------------------- */
+typedef struct pj_smj_s* pj_smj_t;
+
+typedef struct {
+    float extract_ms;         /* tuples -> (key, payload) arrays, key ranges */
+    float sort_ms;            /* both radix sorts (with one host read of the key ranges) */
+    float merge_ms;           /* run search, scan, output rows */
+    float total_ms;
+    uint32_t build_key_bits;  /* low key bits sorted on each side */
+    uint32_t probe_key_bits;
+} pj_smj_timings;
+
+/* Validates desc as pj_join_create does (SortMergeJoinOp::init, join.cpp:905-1020). */
+int pj_smj_create(const pj_join_desc* desc, pj_smj_t* out);
+int pj_smj_out_width(pj_smj_t h, uint32_t* width);
+/* As pj_join_run_device: device tuples in, output rows into d_out (16-byte
+ * aligned, room for out_cap rows); *out_n gets the row count, and
+ * PJ_ERR_CAPACITY (nothing written) when it exceeds out_cap. Synchronous. */
+int pj_smj_run_device(pj_smj_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
+                      void* d_out, uint64_t out_cap, uint64_t* out_n, void* stream);
+int pj_smj_last_timings(pj_smj_t h, pj_smj_timings* t);
+int pj_smj_destroy(pj_smj_t h);
+
+/* ------------------------------------------------------------------------
  * Raw binary tables into HBM: the binary scan source of the reference,
  * MemMappedTable::load (operators/loaders/table.cpp:170-347), which globs
  * the pattern and maps every non-empty regular file it names, in glob's

@@ -136,6 +136,12 @@ _SIGS = {
     "pj_agg_create": (C.c_int, [C.POINTER(AggDesc), C.POINTER(C.c_void_p)]),
     "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "pj_smj_create": (C.c_int, [C.POINTER(JoinDesc), C.POINTER(C.c_void_p)]),
+    "pj_smj_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_smj_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pj_smj_last_timings": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pj_smj_destroy": (C.c_int, [C.c_void_p]),
     "pj_load_binary": (C.c_int, [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                  C.c_void_p]),
     "pj_agg_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
@@ -458,3 +464,45 @@ def load_binary(pattern, width, d_dst, cap, stream=None):
         raise PJError(rc, "need room for %d tuples" % n.value)
     check(rc)
     return n.value
+
+
+class SmjTimings(C.Structure):
+    _fields_ = [("extract_ms", C.c_float), ("sort_ms", C.c_float), ("merge_ms", C.c_float),
+                ("total_ms", C.c_float), ("build_key_bits", C.c_uint32), ("probe_key_bits", C.c_uint32)]
+
+
+class SortMergeJoin:
+    """Owns a pj_smj_t: the sort-merge join over the same descriptor and
+    output rows as Join."""
+
+    def __init__(self, desc):
+        self.h = C.c_void_p()
+        check(lib().pj_smj_create(C.byref(desc), C.byref(self.h)))
+        w = C.c_uint32()
+        check(lib().pj_smj_out_width(self.h, C.byref(w)))
+        self.out_width = w.value
+
+    def run_device(self, d_build, nbuild, d_probe, nprobe, d_out, out_cap, stream=None):
+        n = C.c_uint64()
+        rc = lib().pj_smj_run_device(self.h, C.c_void_p(d_build), nbuild, C.c_void_p(d_probe), nprobe,
+                                     C.c_void_p(d_out), out_cap, C.byref(n), C.c_void_p(stream))
+        if rc == PJ_ERR_CAPACITY:
+            raise PJError(rc, "need %d output tuples" % n.value)
+        check(rc)
+        return n.value
+
+    def timings(self):
+        t = SmjTimings()
+        check(lib().pj_smj_last_timings(self.h, C.byref(t)))
+        return t
+
+    def close(self):
+        if self.h:
+            lib().pj_smj_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

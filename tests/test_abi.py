@@ -123,3 +123,12 @@ def test_aggregate_output_width():
     a = P.Aggregate(["int", "int", "long"], [0, 1], "count", None)
     assert a.out_width == 16          # two INTEGER groups, LONG count
     a.close()
+
+
+def test_sortmerge_join_validates_like_the_hash_join():
+    d = _desc(projection=["B$0", "B$9"])
+    h = C.c_void_p()
+    assert P.lib().pj_smj_create(C.byref(d), C.byref(h)) == P.PJ_ERR_INVALID
+    j = P.SortMergeJoin(_desc())
+    assert j.out_width == P.Join(_desc()).out_width == 24
+    j.close()

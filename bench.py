@@ -518,19 +518,28 @@ def main():
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": a_bytes / (a_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None}}
         if not args.no_cpu_baseline:
-            # the oracle's aggregation (numpy, one core) over the first 16M rows of the same join output
+            # the reference's own AggregateSum / AggregateCount (oracle/_ref) when it was built, else the
+            # oracle's numpy aggregation, over the first 16M rows of the same join output
             sys.path.insert(0, os.path.join(ROOT, "oracle"))
             import oracle as O
+            import reference_run as RR
             m = min(n, 1 << 24)
             host = out[: m * j.out_width].cpu().numpy().reshape(m, j.out_width)
-            c0 = time.perf_counter()
-            if args.aggregate == "sum":
-                O.aggregate(["long"] * 3, [], "sum", 2, host)
+            fields, op, sf = ([], "sum", 2) if args.aggregate == "sum" else ([0], "count", None)
+            if RR.available():
+                threads = RR.default_threads()
+                buckets = 1 << max(1, (min(nb, m) - 1).bit_length()) if fields else 1024
+                _, cs = RR.run_aggregate(["long"] * 3, fields, op, sf, host, threads=threads, buckets=buckets)
+                kind, cores = "reference", threads
+                how = "reference GenericAggregate built -O3 from its sources, global mode, modulo hash over %d " \
+                      "buckets, %d threads" % (buckets, threads)
             else:
-                O.aggregate(["long"] * 3, [0], "count", None, host)
-            cs = time.perf_counter() - c0
-            agg_res["cpu_baseline"] = {"value": m / cs, "unit": "input tuples/s", "cores": 1, "kind": "port",
-                                       "sample": "first %d rows of the join output" % m}
+                c0 = time.perf_counter()
+                O.aggregate(["long"] * 3, fields, op, sf, host)
+                cs = time.perf_counter() - c0
+                kind, cores, how = "port", 1, "the oracle's numpy aggregation, one thread"
+            agg_res["cpu_baseline"] = {"value": m / cs, "unit": "input tuples/s", "cores": cores, "kind": kind,
+                                       "sample": "first %d rows of the join output; %s" % (m, how)}
         agg.close()
     w_in, w_out = 16, j.out_width
     b_alg = nb * w_in + npr * w_in + n * w_out

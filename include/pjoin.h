@@ -121,6 +121,17 @@ int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild,
 /* Host-page protocol used by the C++ HashJoinOp. */
 /* Stages build tuples (HashJoinOp::buildFromPage, join.cpp:838-884).
  * Thread-safe; call from every build thread. */
+/* aggregate_count grouped by the build key over the join of nbuild x nprobe
+ * device tuples (aggregatecount.cpp:37-52 over HashJoinOp's output,
+ * join.cpp:681-757), without writing the join's rows: one group tuple per
+ * build key with matches, the key column(s) as in the build schema followed by
+ * the count (LONG), the layout pj_agg_run_device gives for that count. The
+ * count of a key is (build tuples with the key) x (probe tuples with it).
+ * Always on radix partitions; the handle's projection is not used. Output
+ * 4-byte aligned, room for groups_cap groups; PJ_ERR_CAPACITY when more.
+ * Synchronous. */
+int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
+                         void* d_groups, uint64_t groups_cap, uint64_t* ngroups, void* stream);
 int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n);
 /* Uploads and partitions the staged build side once the group barrier has
  * been passed (join.cpp:618). Exactly once per scan. */

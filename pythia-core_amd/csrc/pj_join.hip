@@ -434,7 +434,8 @@ struct DenseTab {
 // counts per bucket, a scan turns counts into bucket starts, pass 2 places
 // each record at its bucket's cursor; afterwards dir entry b holds the end of
 // bucket b. Two 16-bit entries share a 32-bit word. bucket_of(r) returns the
-// bucket of record r, or ~0 to skip it; put(j, r) stores r at slot j.
+// bucket of record r, or ~0 to skip it; put(j, r, i) stores record r (index
+// i of R) at slot j.
 template <class BucketOf, class Put>
 __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint32_t r_off, uint32_t r_cnt,
                                               uint32_t* dirw, uint32_t* wsum, BucketOf bucket_of, Put put)
@@ -465,7 +466,7 @@ __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint3
         if (b != ~0u) {
             const uint32_t sh = (b & 1u) * 16u;
             const uint32_t j = (atomicAdd(&dirw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
-            put(j, r);
+            put(j, r, r_off + i);
         }
     }
     __syncthreads();
@@ -621,7 +622,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
                     const uint64_t h = key_mix(rec_key(r));
                     return radix_part(h, bitsR) == I.part ? (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1) : ~0u;
                 },
-                [&](uint32_t j, const uint4& r) {
+                [&](uint32_t j, const uint4& r, uint32_t) {
                     tkey[j] = rec_key(r);
                     tpay[j] = make_uint2(r.z, r.w);
                 });
@@ -668,7 +669,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
                 build_buckets(
                     R, I.r_off, I.r_cnt, dirw, wsum,
                     [&](const uint4& r) -> uint32_t { return code_bucket((uint32_t)(ord_key(rec_key(r)) - base)); },
-                    [&](uint32_t j, const uint4& r) {
+                    [&](uint32_t j, const uint4& r, uint32_t) {
                         tcode[j] = (uint32_t)(ord_key(rec_key(r)) - base);
                         tpay[j] = make_uint2(r.z, r.w);
                     });
@@ -677,6 +678,130 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
             }
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- group-join
+// The count per build key of a radix join's matches (aggregate_count grouped
+// by the join key over the join's output) without writing the join's rows.
+// Slot j of the build table keeps its record's index in R_part (x) and the
+// matches it met in this item (y, an LDS counter); after the item the
+// counts are added to rcount[] in HBM, where an item's probe pieces and
+// table splits add up. A key that an earlier slot of its bucket already
+// holds, or a partition split over several tables, sets *dup: equal keys
+// then have several records, which the caller merges.
+constexpr uint32_t kGjUnroll = 8;
+
+__global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restrict__ R, const uint4* __restrict__ S,
+                                                             PlanView plan, uint32_t* __restrict__ item_ctr,
+                                                             uint32_t* __restrict__ rcount, uint32_t bitsR,
+                                                             uint32_t bkt_shift, uint32_t* __restrict__ dup)
+{
+    extern __shared__ uint64_t lds64[];
+    const uint32_t tid = threadIdx.x;
+    uint32_t* misc = reinterpret_cast<uint32_t*>(lds64);
+    uint32_t* wsum = misc + 40;
+    uint64_t* table = lds64 + kMiscWords / 2;
+    const uint32_t nitems = plan.offs[plan.FR];
+    for (;;) {
+        if (tid == 0)
+            misc[0] = atomicAdd(item_ctr, 1u);
+        __syncthreads();
+        const uint32_t it = misc[0];
+        if (it >= nitems)
+            break;
+        const Item I = plan_item(plan, it);
+        const uint4* Sp = S + I.s_off;
+        uint64_t* tkey = table;                                            // kTableCap
+        uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);          // kTableCap
+        uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);    // kTableBuckets / 2 + 1
+        build_buckets(
+            R, I.r_off, I.r_cnt, dirw, wsum,
+            [&](const uint4& r) -> uint32_t {
+                const uint64_t h = key_mix(rec_key(r));
+                return radix_part(h, bitsR) == I.part ? (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1) : ~0u;
+            },
+            [&](uint32_t j, const uint4& r, uint32_t i) {
+                tkey[j] = rec_key(r);
+                tpay[j] = make_uint2(i, 0u);
+            });
+        const uint16_t* dir = reinterpret_cast<const uint16_t*>(dirw);
+        const RadixTab tab{tkey, tpay, dir, bitsR, bkt_shift, I.part};
+        for (uint32_t b0 = 0; b0 < I.s_cnt; b0 += kJoinThreads * kGjUnroll) {
+            uint4 sr[kGjUnroll];
+#pragma unroll
+            for (uint32_t u = 0; u < kGjUnroll; ++u) {
+                const uint32_t i = b0 + u * kJoinThreads + tid;
+                if (i < I.s_cnt)
+                    sr[u] = ld_stream(Sp + i);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kGjUnroll; ++u) {
+                const uint32_t i = b0 + u * kJoinThreads + tid;
+                if (i >= I.s_cnt)
+                    break;
+                uint32_t nm = 0, first = 0;
+                tab.lookup(sr[u], nm, first);
+                const uint64_t mk = rec_key(sr[u]);
+                for (uint32_t j = first, c = 0; c < nm; ++j)   // the nm matches lie in first's bucket
+                    if (tkey[j] == mk) {
+                        atomicAdd(&tpay[j].y, 1u);
+                        ++c;
+                    }
+            }
+        }
+        __syncthreads();
+        const uint32_t ntab = dir[kTableBuckets - 1];
+        bool seen = plan.rst[I.part + 1] - plan.rst[I.part] > I.r_cnt;   // a split partition
+        for (uint32_t j = tid; j < ntab; j += kJoinThreads) {
+            const uint2 pv = tpay[j];
+            if (pv.y)
+                atomicAdd(&rcount[pv.x], pv.y);
+            const uint64_t k = tkey[j];
+            const uint32_t b = (uint32_t)(key_mix(k) >> bkt_shift) & (kTableBuckets - 1);
+            for (uint32_t m = b ? dir[b - 1] : 0u; m < j; ++m)
+                if (tkey[m] == k)
+                    seen = true;
+        }
+        if (seen)
+            atomicOr(dup, 1u);
+        __syncthreads();
+    }
+}
+
+// One (build key, count) tuple per record of R_part that met matches, at
+// rows taken from *cursor (one atomic a wave); rows past cap are counted,
+// not written. Records [0, rst[FR]); padding records never match.
+__global__ __launch_bounds__(256) void k_groupjoin_emit(const uint4* __restrict__ R, const uint32_t* __restrict__ rst,
+                                                        uint32_t FR, const uint32_t* __restrict__ rcount,
+                                                        uint32_t key_words, uint32_t* __restrict__ out, uint64_t cap,
+                                                        unsigned long long* __restrict__ cursor)
+{
+    const uint32_t n = rst[FR];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t c = i < n ? rcount[i] : 0u;
+        const uint64_t m = __ballot(c != 0);
+        if (!m)
+            continue;
+        unsigned long long base = 0;
+        if (lane == 0)
+            base = atomicAdd(cursor, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (c) {
+            const uint64_t row = base + (uint64_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+            if (row < cap) {
+                const uint64_t k = rec_key(R[i]);
+                uint32_t* o = out + row * (key_words + 2);
+                o[0] = (uint32_t)k;
+                if (key_words == 2)
+                    o[1] = (uint32_t)(k >> 32);
+                o[key_words] = c;   // the count as a LONG
+                o[key_words + 1] = 0u;
+            }
+        }
     }
 }
 
@@ -937,6 +1062,7 @@ struct pj_join_s {
     hipEvent_t ev[8];
     bool ev_ok = false;
     uint32_t* host_ctr = nullptr;   // pinned: the join's counters, read back after each probe
+    DevBuf gj_count, gj_rows;       // group-join: matches a build record, merged rows
 };
 
 constexpr uint32_t kMaxBits = 14;
@@ -1617,6 +1743,113 @@ int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild, const 
     if (rc)
         return rc;
     return probe_device(h, d_probe, nprobe, d_out, out_cap, out_n, st);
+}
+
+int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
+                         void* d_groups, uint64_t groups_cap, uint64_t* ngroups, void* stream)
+{
+    if (!h || !ngroups || (nbuild && !d_build) || (nprobe && !d_probe) || (groups_cap && !d_groups))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if ((uintptr_t)d_groups % 4)
+        return fail(PJ_ERR_INVALID, "the group buffer must be 4-byte aligned");
+    *ngroups = 0;
+    const pj_join_desc& d = h->desc;
+    // the group columns: the build key, one INTEGER / LONG / DATE column or two INTEGER columns
+    pj_schema gs = {};
+    gs.ncols = d.nkeys + 1;
+    for (uint32_t k = 0; k < d.nkeys; ++k)
+        gs.type[k] = d.build.type[d.buildjattr[k]];
+    gs.type[d.nkeys] = PJ_CT_LONG;
+    const uint32_t key_words = d.nkeys == 2 || gs.type[0] != PJ_CT_INTEGER ? 2u : 1u;
+    std::lock_guard<std::mutex> g(h->mu);
+    hipStream_t st = as_stream(stream);
+    int rc;
+    h->model_ms = 0.f;
+    h->raw_build = d_build;
+    h->raw_nbuild = nbuild;
+    h->plan = RangePlan();
+    if ((rc = prepare_radix(h, d_build, nbuild, st)) ||
+        (rc = partition_side(h, d_probe, nprobe, h->lay.probe, radix_fn(h->bS), false, true, h->spart, h->S,
+                             h->sstarts, false, nullptr, st)))
+        return rc;
+    if (nbuild == 0 || nprobe == 0) {
+        PJ_HIP(hipStreamSynchronize(st));
+        return PJ_OK;
+    }
+    PlanView v = {};
+    v.FR = 1u << h->bR;
+    v.shift = h->bR - h->bS;
+    v.rst = (const uint32_t*)h->rstarts.p;
+    v.sst = (const uint32_t*)h->sstarts.p;
+    v.chunk = probe_chunk(nprobe);
+    if (v.FR > kPlanThreads * kPlanPer)
+        return fail(PJ_ERR_UNSUPPORTED, "%u build partitions exceed the planner", v.FR);
+    const uint64_t nrec = h->rpart.cap / 16;   // records R_part can hold, padding included
+    if ((rc = h->planoffs.ensure((size_t)(v.FR + 1) * 4)) || (rc = h->counters.ensure(64)) ||
+        (rc = h->gj_count.ensure(nrec * 4)))
+        return rc;
+    if (!h->host_ctr)
+        PJ_HIP(hipHostMalloc((void**)&h->host_ctr, 64, hipHostMallocDefault));
+    v.offs = (const uint32_t*)h->planoffs.p;
+    uint32_t* counters = (uint32_t*)h->counters.p;
+    uint32_t* rcount = (uint32_t*)h->gj_count.p;
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((nrec + 255) / 256)), dim3(256), 0, st, rcount, (uint32_t)nrec,
+                       0u);
+    hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, st, v, (uint32_t*)h->planoffs.p, counters);
+    const size_t lds = join_lds_bytes(false);
+    if ((rc = allow_lds((const void*)k_groupjoin, lds)))
+        return rc;
+    hipLaunchKernelGGL(k_groupjoin, dim3((uint32_t)device_count_cus()), dim3(kJoinThreads), lds, st,
+                       (const uint4*)h->rpart.p, (const uint4*)h->spart.p, v, counters, rcount, h->bR,
+                       64 - h->bR - 12, counters + 1);
+    PJ_HIP(hipGetLastError());
+    PJ_HIP(hipMemcpyAsync(h->host_ctr, counters, 16, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipStreamSynchronize(st));
+    const bool dup = h->host_ctr[1] != 0;
+    unsigned long long* cursor = (unsigned long long*)(counters + 2);
+    const uint32_t emit_grid = (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 4096);
+    if (!dup) {   // every key on one record: the records' counts are the groups
+        hipLaunchKernelGGL(k_groupjoin_emit, dim3(emit_grid), dim3(256), 0, st, (const uint4*)h->rpart.p,
+                           (const uint32_t*)h->rstarts.p, v.FR, (const uint32_t*)rcount, key_words,
+                           (uint32_t*)d_groups, groups_cap, cursor);
+        PJ_HIP(hipGetLastError());
+        PJ_HIP(hipMemcpyAsync(h->host_ctr, cursor, 8, hipMemcpyDeviceToHost, st));
+        PJ_HIP(hipStreamSynchronize(st));
+        *ngroups = *(const uint64_t*)h->host_ctr;
+        if (*ngroups > groups_cap)
+            return fail(PJ_ERR_CAPACITY, "%llu groups, the output holds %llu", (unsigned long long)*ngroups,
+                        (unsigned long long)groups_cap);
+        return PJ_OK;
+    }
+    // equal keys on several records: one row a record with matches, then the
+    // sum of their counts per key
+    const uint32_t row_bytes = (key_words + 2) * 4;
+    if ((rc = h->gj_rows.ensure(nbuild * row_bytes)))
+        return rc;
+    hipLaunchKernelGGL(k_groupjoin_emit, dim3(emit_grid), dim3(256), 0, st, (const uint4*)h->rpart.p,
+                       (const uint32_t*)h->rstarts.p, v.FR, (const uint32_t*)rcount, key_words,
+                       (uint32_t*)h->gj_rows.p, nbuild, cursor);
+    PJ_HIP(hipGetLastError());
+    PJ_HIP(hipMemcpyAsync(h->host_ctr, cursor, 8, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipStreamSynchronize(st));
+    const uint64_t nrows = *(const uint64_t*)h->host_ctr;
+    if (nrows > nbuild)
+        return fail(PJ_ERR_STATE, "group-join emitted %llu rows for %llu build records", (unsigned long long)nrows,
+                    (unsigned long long)nbuild);
+    pj_agg_desc ad = {};
+    ad.in = gs;
+    ad.op = PJ_AGG_SUM;
+    ad.nfields = d.nkeys;
+    for (uint32_t k = 0; k < d.nkeys; ++k)
+        ad.fields[k] = (uint16_t)k;
+    ad.sumfield = (uint16_t)d.nkeys;
+    ad.max_groups = std::max<uint64_t>(nrows, 1);
+    pj_agg_t a = nullptr;
+    if ((rc = pj_agg_create(&ad, &a)))
+        return rc;
+    rc = pj_agg_run_device(a, h->gj_rows.p, nrows, d_groups, groups_cap, ngroups, stream);
+    pj_agg_destroy(a);
+    return rc;
 }
 
 int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n)

@@ -136,6 +136,8 @@ _SIGS = {
     "pj_agg_create": (C.c_int, [C.POINTER(AggDesc), C.POINTER(C.c_void_p)]),
     "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "pj_join_count_by_key": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
+                                       C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
     "pj_smj_create": (C.c_int, [C.POINTER(JoinDesc), C.POINTER(C.c_void_p)]),
     "pj_smj_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_smj_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
@@ -241,6 +243,17 @@ class Join:
             raise PJError(rc, "need %d output tuples" % n.value)
         check(rc)
         return n.value
+
+    def count_by_key(self, d_build, nbuild, d_probe, nprobe, d_groups, groups_cap, stream=None):
+        """aggregate_count grouped by the build key over the join, without the
+        join's rows; returns the number of groups."""
+        g = C.c_uint64()
+        rc = lib().pj_join_count_by_key(self.h, C.c_void_p(d_build), nbuild, C.c_void_p(d_probe), nprobe,
+                                        C.c_void_p(d_groups), groups_cap, C.byref(g), C.c_void_p(stream))
+        if rc == PJ_ERR_CAPACITY:
+            raise PJError(rc, "need %d groups" % g.value)
+        check(rc)
+        return g.value
 
     def timings(self):
         t = Timings()

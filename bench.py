@@ -274,6 +274,65 @@ def cpu_baseline_batch(seed, sizes):
 PCIE_PEAK_GBS = 64.0   # PCIe 5.0 x16, one direction
 
 
+def bench_groupjoin(args, P, torch, desc, db, nb, dp, npr, nmiss, dom, seed, clustered, stream):
+    """The count per build key of the join, computed inside the join
+    (pj_join_count_by_key: no join rows written), against the same count
+    from the join's rows in HBM (pj_join_run_device then pj_agg_run_device),
+    which also checks it."""
+    j = P.Join(desc)
+    groups = torch.empty(nb * 16, dtype=torch.uint8, device=db.device)
+    for _ in range(args.warmup):
+        g = j.count_by_key(db.data_ptr(), nb, dp.data_ptr(), npr, groups.data_ptr(), nb, stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = j.count_by_key(db.data_ptr(), nb, dp.data_ptr(), npr, groups.data_ptr(), nb, stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the unfused pipeline on the same inputs: its time (one run after a warm-up) and its groups
+    rows = torch.empty(npr * j.out_width, dtype=torch.uint8, device=db.device)
+    agg = P.Aggregate(["long"] * 3, [0], "count", None, max_groups=nb)
+    ref = torch.empty(nb * 16, dtype=torch.uint8, device=db.device)
+    for k in range(2):
+        torch.cuda.synchronize()
+        u0 = time.perf_counter()
+        n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, rows.data_ptr(), npr, stream)
+        g2 = agg.run_device(rows.data_ptr(), n, ref.data_ptr(), nb, stream)
+        torch.cuda.synchronize()
+        unfused_ms = (time.perf_counter() - u0) * 1e3
+    agg.close()
+    a = groups[: g * 16].view(torch.int64).view(-1, 2)
+    b = ref[: g2 * 16].view(torch.int64).view(-1, 2)
+    same = g == g2 and torch.equal(a[torch.argsort(a[:, 0])], b[torch.argsort(b[:, 0])])
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if clustered:
+        es, ec = O.expected_clustered(P.cluster_table(seed, nb), npr, P.zipf_cdf(nb, args.zipf), threads=16)
+    else:
+        es, ec = O.expected_pkfk(seed, dom, nb, npr, nmiss, threads=16)
+    parity = same and int(a[:, 1].sum()) == ec == n
+    b_alg = (nb + npr) * 16 + g * 16
+    res = {
+        "metric": "join + count per build key, tuples/s (build+probe), %s; counted inside the join" % args.workload,
+        "value": (nb + npr) / (ms * 1e-3), "unit": "tuples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64", "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": args.workload, "aggregate": "count_by_key_fused", "build_tuples": nb,
+                   "probe_tuples": npr, "groups": g, "join_rows_counted": int(a[:, 1].sum())},
+        "join_then_aggregate_ms": unfused_ms,
+        "parity": {"groups": g, "groups_from_join_rows": g2, "same_groups": bool(same), "expected_rows": ec,
+                   "checksum_ok": bool(parity)},
+        "roofline": {"bound": "hbm", "kernel": "whole step (partition passes, k_groupjoin, emit)",
+                     "achieved": b_alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": None,
+    }
+    j.close()
+    print(json.dumps(res))
+    if not parity:
+        sys.exit(1)
+
+
 def bench_sortmerge(args, P, torch, desc, db, nb, dp, npr, nmiss, dom, seed, clustered, zipf, stream):
     """The device sort-merge join on the same resident inputs and output rows
     as the hash join: both sides radix-sorted by key, then merged. Checked
@@ -368,9 +427,10 @@ def main():
                     sorted(BATCH_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
-    ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key"],
+    ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key", "count_by_key_fused"],
                     help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
-                         "per build key (int64-key workloads)")
+                         "per build key (int64-key workloads); count_by_key_fused: the count per build key "
+                         "computed inside the join (pj_join_count_by_key), no join rows written")
     ap.add_argument("--algorithm", default="hash", choices=["hash", "sortmerge"],
                     help="hash: the radix hash join; sortmerge: the device sort-merge join (pj_smj_*), "
                          "same inputs and output rows, to compare against")
@@ -434,6 +494,11 @@ def main():
     load_res = None
     if args.from_files:
         load_res = load_from_files(args, P, torch, db, nb, dp, npr, stream)
+    if args.aggregate == "count_by_key_fused":
+        if zipf or args.partitioner != "radix":
+            print("--aggregate count_by_key_fused runs on the int64-key workloads, radix", file=sys.stderr)
+            sys.exit(2)
+        return bench_groupjoin(args, P, torch, desc, db, nb, dp, npr, nmiss, dom, seed, clustered, stream)
     if zipf and args.aggregate != "none":
         print("--aggregate runs on the int64-key workloads", file=sys.stderr)
         sys.exit(2)

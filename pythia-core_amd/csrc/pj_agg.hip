@@ -335,6 +335,23 @@ __global__ __launch_bounds__(kPartThreads) void k_agg_parts(const uint4* __restr
         if (used_global)
             G.misc[4] = 1;
         __syncthreads();
+        // the partition's groups: counted, their rows reserved with one atomic, then written
+        uint32_t mine = 0;
+        for (uint32_t s = tid; s < kPartSlots; s += kPartThreads)
+            mine += lkey[s] != kEmpty;
+        for (int d = 32; d > 0; d >>= 1)
+            mine += __shfl_xor(mine, d, 64);
+        if (lane == 0)
+            wcnt[wave] = mine;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < kPartThreads / 64; ++w)
+                tot += wcnt[w];
+            cbase = tot ? atomicAdd(&G.misc[3], (unsigned long long)tot) : 0ull;
+        }
+        __syncthreads();
+        uint64_t next = cbase;
         for (uint32_t c = 0; c < kPartSlots; c += kPartThreads) {
             const uint32_t s = c + tid;
             const uint64_t k = lkey[s];
@@ -344,22 +361,18 @@ __global__ __launch_bounds__(kPartThreads) void k_agg_parts(const uint4* __restr
             if (lane == 0)
                 wcnt[wave] = (uint32_t)__popcll(m);
             __syncthreads();
-            if (tid == 0) {
-                uint32_t tot = 0;
-                for (uint32_t w = 0; w < kPartThreads / 64; ++w) {
-                    const uint32_t x = wcnt[w];
-                    wcnt[w] = tot;
-                    tot += x;
-                }
-                cbase = tot ? atomicAdd(&G.misc[3], (unsigned long long)tot) : 0ull;
+            uint32_t before = 0, total = 0;
+            for (uint32_t w = 0; w < kPartThreads / 64; ++w) {
+                before += w < wave ? wcnt[w] : 0u;
+                total += wcnt[w];
             }
             __syncthreads();
             if (has) {
-                const uint64_t pos = cbase + wcnt[wave] + rank;
+                const uint64_t pos = next + before + rank;
                 if (pos < cap)
                     write_group(out + pos * ow, L, k, lval[s]);
             }
-            __syncthreads();
+            next += total;
         }
     }
 }

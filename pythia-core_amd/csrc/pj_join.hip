@@ -769,29 +769,57 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
     }
 }
 
-// One (build key, count) tuple per record of R_part that met matches, at
-// rows taken from *cursor (one atomic a wave); rows past cap are counted,
-// not written. Records [0, rst[FR]); padding records never match.
-__global__ __launch_bounds__(256) void k_groupjoin_emit(const uint4* __restrict__ R, const uint32_t* __restrict__ rst,
-                                                        uint32_t FR, const uint32_t* __restrict__ rcount,
-                                                        uint32_t key_words, uint32_t* __restrict__ out, uint64_t cap,
-                                                        unsigned long long* __restrict__ cursor)
+// One (build key, count) tuple per record of R_part that met matches
+// (records [0, rst[FR]); padding records never match). Each block takes one
+// contiguous range: it counts its groups, reserves their rows with one
+// atomic on *cursor, then writes them in order. Rows past cap are counted,
+// not written.
+constexpr uint32_t kEmitThreads = 256;
+
+__global__ __launch_bounds__(kEmitThreads) void k_groupjoin_emit(const uint4* __restrict__ R,
+                                                                  const uint32_t* __restrict__ rst, uint32_t FR,
+                                                                  const uint32_t* __restrict__ rcount,
+                                                                  uint32_t key_words, uint32_t* __restrict__ out,
+                                                                  uint64_t cap, unsigned long long* __restrict__ cursor)
 {
+    __shared__ uint32_t wcnt[kEmitThreads / 64];
+    __shared__ unsigned long long base_s;
     const uint32_t n = rst[FR];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t per = ((n + gridDim.x - 1) / gridDim.x + kEmitThreads - 1) / kEmitThreads * kEmitThreads;
+    const uint32_t lo = blockIdx.x * per < n ? blockIdx.x * per : n;
+    const uint32_t hi = n - lo > per ? lo + per : n;
+    uint32_t mine = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kEmitThreads)
+        mine += rcount[i] != 0;
+    for (int d = 32; d > 0; d >>= 1)
+        mine += __shfl_xor(mine, d, 64);
+    if (lane == 0)
+        wcnt[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kEmitThreads / 64; ++w)
+            t += wcnt[w];
+        base_s = t ? atomicAdd(cursor, (unsigned long long)t) : 0ull;
+    }
+    __syncthreads();
+    uint64_t next = base_s;   // the block's next row
+    for (uint32_t i0 = lo; i0 < hi; i0 += kEmitThreads) {
         const uint32_t i = i0 + threadIdx.x;
-        const uint32_t c = i < n ? rcount[i] : 0u;
+        const uint32_t c = i < hi ? rcount[i] : 0u;
         const uint64_t m = __ballot(c != 0);
-        if (!m)
-            continue;
-        unsigned long long base = 0;
         if (lane == 0)
-            base = atomicAdd(cursor, (unsigned long long)__popcll(m));
-        base = __shfl(base, 0, 64);
+            wcnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < kEmitThreads / 64; ++w) {
+            before += w < wave ? wcnt[w] : 0u;
+            total += wcnt[w];
+        }
+        __syncthreads();
         if (c) {
-            const uint64_t row = base + (uint64_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
+            const uint64_t row = next + before + (uint64_t)__popcll(m & (lane ? (~0ull >> (64 - lane)) : 0ull));
             if (row < cap) {
                 const uint64_t k = rec_key(R[i]);
                 uint32_t* o = out + row * (key_words + 2);
@@ -802,6 +830,7 @@ __global__ __launch_bounds__(256) void k_groupjoin_emit(const uint4* __restrict_
                 o[key_words + 1] = 0u;
             }
         }
+        next += total;
     }
 }
 
@@ -1807,7 +1836,7 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
     PJ_HIP(hipStreamSynchronize(st));
     const bool dup = h->host_ctr[1] != 0;
     unsigned long long* cursor = (unsigned long long*)(counters + 2);
-    const uint32_t emit_grid = (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 4096);
+    const uint32_t emit_grid = (uint32_t)std::min<uint64_t>((nrec + 4095) / 4096, 4096);
     if (!dup) {   // every key on one record: the records' counts are the groups
         hipLaunchKernelGGL(k_groupjoin_emit, dim3(emit_grid), dim3(256), 0, st, (const uint4*)h->rpart.p,
                            (const uint32_t*)h->rstarts.p, v.FR, (const uint32_t*)rcount, key_words,

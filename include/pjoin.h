@@ -383,6 +383,11 @@ int pj_agg_out_width(pj_agg_t h, uint32_t* width);
  * serialised by its lock. */
 int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_groups, uint64_t groups_cap,
                       uint64_t* ngroups, void* stream);
+/* As pj_agg_run_device with host buffers: tuples are copied to the device,
+ * aggregated there, and the groups copied back (the path of the plan
+ * operators, whose pages live in host memory). Synchronous. */
+int pj_agg_run_host(pj_agg_t h, const void* tuples, uint64_t n, void* groups, uint64_t groups_cap,
+                    uint64_t* ngroups);
 int pj_agg_destroy(pj_agg_t h);
 
 /* ------------------------------------------------------------------------

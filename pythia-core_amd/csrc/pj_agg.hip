@@ -395,6 +395,8 @@ struct pj_agg_s {
     RadixRecords parts;
     uint64_t* host_misc = nullptr;   // pinned readback of the table's counters
     std::mutex mu;
+    std::mutex host_mu;              // guards the host path's device copies
+    DevBuf host_in, host_out;
 };
 
 namespace {
@@ -574,6 +576,35 @@ int pj_agg_run_device(pj_agg_t h, const void* d_tuples, uint64_t n, void* d_grou
     if (*ngroups > groups_cap)
         return fail(PJ_ERR_CAPACITY, "%llu groups, the output holds %llu", (unsigned long long)*ngroups,
                     (unsigned long long)groups_cap);
+    return PJ_OK;
+}
+
+int pj_agg_run_host(pj_agg_t h, const void* tuples, uint64_t n, void* groups, uint64_t groups_cap,
+                    uint64_t* ngroups)
+{
+    if (!h || !ngroups || (n && !tuples) || (groups_cap && !groups))
+        return fail(PJ_ERR_INVALID, "null argument");
+    *ngroups = 0;
+    std::lock_guard<std::mutex> g(h->host_mu);
+    int in_size = 0, off[PJ_MAX_COLS];
+    int rc = schema_offsets(h->desc.in, off, &in_size);
+    if (rc)
+        return rc;
+    const uint64_t ow = (uint64_t)h->out_words * 4;
+    // room on the device for every group the input can have
+    const uint64_t dcap = std::max<uint64_t>(std::min<uint64_t>(n, std::max<uint64_t>(groups_cap, 1)), 1);
+    if ((rc = h->host_in.ensure(std::max<uint64_t>(n * (uint64_t)in_size, 16))) || (rc = h->host_out.ensure(dcap * ow)))
+        return rc;
+    if (n)
+        PJ_HIP(hipMemcpy(h->host_in.p, tuples, n * (uint64_t)in_size, hipMemcpyHostToDevice));
+    rc = pj_agg_run_device(h, h->host_in.p, n, h->host_out.p, dcap, ngroups, nullptr);
+    if (rc == PJ_ERR_CAPACITY && *ngroups > dcap && *ngroups > groups_cap)
+        return fail(PJ_ERR_CAPACITY, "%llu groups, the output holds %llu", (unsigned long long)*ngroups,
+                    (unsigned long long)groups_cap);
+    if (rc)
+        return rc;
+    if (*ngroups)
+        PJ_HIP(hipMemcpy(groups, h->host_out.p, *ngroups * ow, hipMemcpyDeviceToHost));
     return PJ_OK;
 }
 

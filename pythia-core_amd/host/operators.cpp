@@ -566,6 +566,183 @@ Operator::ResultCode PartitionOp::scanStop(unsigned short tid)
     return Ready;
 }
 
+// ---------------------------------------------------------------- AggregateOp
+
+// GenericAggregate::init (operators/genericaggregate.cpp:51-180) with
+// AggregateSum::foldinit (aggregatesum.cpp:37-51) or AggregateCount::foldinit
+// (aggregatecount.cpp:37-42): the grouping columns ("field" or "fields"),
+// then the sum (the summed column's type) or the count (LONG).
+void AggregateOp::init(const Setting& root, const Setting& node)
+{
+    Operator::init(root, node);
+    Schema& in = nextOp->getOutSchema();
+    std::vector<unsigned> fields;
+    if (node.exists("fields")) {
+        const Setting& f = node["fields"];
+        if (!f.isAggregate())
+            throw InvalidParameter("fields must be a list of column numbers");
+        for (int i = 0; i < f.getLength(); ++i)
+            fields.push_back((unsigned)f[i].asInt());
+    } else if (node.exists("field")) {
+        fields.push_back((unsigned)node["field"].asInt());
+    } else {
+        throw MissingParameterException("field");
+    }
+    for (unsigned f : fields)
+        if (f >= in.columns())
+            throw InvalidParameter("aggregation field out of range");
+    if (fields.size() > 2)
+        throw IllegalSchemaDeclarationException("at most two grouping columns on the device");
+    unsigned buckets = 1024;
+    if (!fields.empty()) {
+        const HashSpec hs = make_hashspec(node["hash"]);   // TupleHasher::create's checks
+        buckets = std::max(hs.rounded, 1u);
+    }
+    unsigned sumfield = 0;
+    if (!count) {
+        sumfield = (unsigned)node["sumfield"].asInt();
+        if (sumfield >= in.columns())
+            throw InvalidParameter("sumfield out of range");
+        const ColumnType t = in.getColumnType(sumfield);
+        if (!(t == CT_INTEGER || t == CT_LONG || t == CT_DECIMAL))
+            throw InvalidParameter("sumfield must be a numeric column");
+    }
+    for (unsigned f : fields)
+        schema.add(in.getColumnType(f), in.getColumnWidth(f));
+    schema.add(count ? CT_LONG : in.getColumnType(sumfield));
+    global = node.exists("global");
+    if (global) {
+        const int64_t t = node["threads"].asInt();
+        if (t <= 0 || t > MAX_THREADS)
+            throw InvalidParameter("threads");
+        threads = (unsigned)t;
+        barrier.reset(new Barrier(threads));
+    }
+    const uint64_t slots = node.exists("slotperbucket") ? (uint64_t)node["slotperbucket"].asInt() : 6;
+    in.to_c(&desc.in);
+    desc.op = count ? PJ_AGG_COUNT : PJ_AGG_SUM;
+    desc.nfields = (uint32_t)fields.size();
+    for (size_t i = 0; i < fields.size(); ++i)
+        desc.fields[i] = (uint16_t)fields[i];
+    desc.sumfield = (uint16_t)sumfield;
+    desc.max_groups = std::max<uint64_t>((uint64_t)buckets * std::max<uint64_t>(slots, 1), 1024);
+    const int rc = pj_agg_create(&desc, &handle);
+    if (rc == PJ_ERR_UNSUPPORTED)
+        throw IllegalSchemaDeclarationException(pj_last_error());
+    if (rc)
+        throw InvalidParameter(pj_last_error());
+    staged.assign(MAX_THREADS, {});
+    groups.assign(MAX_THREADS, {});
+    outputloc.assign(MAX_THREADS, 0);
+    outend.assign(MAX_THREADS, 0);
+    overlay.assign(MAX_THREADS, Page());
+}
+
+void AggregateOp::threadInit(unsigned short tid)
+{
+    if (global && tid >= threads)
+        throw InvalidParameter("a global aggregation runs on its own `threads` threads");
+}
+
+// The device aggregation of `in`; the handle's table grows to the input's
+// size when the plan's buckets were too few (the reference's table chains
+// instead).
+int AggregateOp::aggregate(const std::vector<char>& in, std::vector<char>& out, uint64_t* ngroups)
+{
+    std::lock_guard<std::mutex> g(mu);
+    const uint64_t w = nextOp->getOutSchema().getTupleSize(), ow = schema.getTupleSize();
+    const uint64_t n = w ? in.size() / w : 0, cap = std::max<uint64_t>(n, 1);
+    out.resize(cap * ow);
+    int rc = pj_agg_run_host(handle, in.data(), n, out.data(), cap, ngroups);
+    if (rc == PJ_ERR_CAPACITY && desc.max_groups < cap) {
+        pj_agg_destroy(handle);
+        handle = nullptr;
+        desc.max_groups = std::min<uint64_t>(cap, 1ull << 30);
+        if ((rc = pj_agg_create(&desc, &handle)) == PJ_OK)
+            rc = pj_agg_run_host(handle, in.data(), n, out.data(), cap, ngroups);
+    }
+    out.resize(rc == PJ_OK ? *ngroups * ow : 0);
+    return rc;
+}
+
+// GenericAggregate::scanStart (genericaggregate.cpp:291-424): drain the
+// input, then aggregate it (global mode: all threads' input, once).
+Operator::ResultCode AggregateOp::scanStart(unsigned short tid)
+{
+    ResultCode rc = Ready;
+    if (nextOp->scanStart(tid) == Error)
+        rc = Error;
+    GetNextResultT r(Ready, nullptr);
+    while (rc != Error && r.first == Ready) {
+        r = nextOp->getNext(tid);
+        if (r.first == Error)
+            rc = Error;
+        else if (r.second && r.second->getUsedSpace())
+            staged[tid].insert(staged[tid].end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
+    }
+    if (nextOp->scanStop(tid) == Error)
+        rc = Error;
+    outputloc[tid] = outend[tid] = 0;
+    if (!global) {
+        uint64_t ng = 0;
+        const int arc = rc == Error ? PJ_OK : aggregate(staged[tid], groups[tid], &ng);
+        std::vector<char>().swap(staged[tid]);
+        outend[tid] = ng;
+        return rc == Error || arc != PJ_OK ? Error : Ready;
+    }
+    barrier->arrive();
+    if (tid == 0) {
+        std::vector<char> all;
+        for (unsigned t = 0; t < threads; ++t) {
+            all.insert(all.end(), staged[t].begin(), staged[t].end());
+            std::vector<char>().swap(staged[t]);
+        }
+        uint64_t ng = 0;
+        agg_rc = aggregate(all, groups[0], &ng);
+        for (unsigned t = 0; t < threads; ++t) {   // an even share of the groups a thread
+            outputloc[t] = ng * t / threads;
+            outend[t] = ng * (t + 1) / threads;
+        }
+    }
+    barrier->arrive();
+    return rc == Error || agg_rc != PJ_OK ? Error : Ready;
+}
+
+// GenericAggregate::getNext (genericaggregate.cpp:426-495): this thread's
+// groups in buffsize slices, Finished with the last one.
+Operator::GetNextResultT AggregateOp::getNext(unsigned short tid)
+{
+    std::vector<char>& g = global ? groups[0] : groups[tid];
+    const unsigned ow = schema.getTupleSize();
+    const uint64_t maxout = std::max<uint64_t>(buffsize / ow, 1);
+    const uint64_t remaining = outend[tid] - outputloc[tid];
+    char* start = g.data() + outputloc[tid] * ow;
+    if (remaining <= maxout) {
+        overlay[tid] = Page(start, remaining * ow, ow, true);
+        outputloc[tid] += remaining;
+        return {Finished, &overlay[tid]};
+    }
+    overlay[tid] = Page(start, maxout * ow, ow, true);
+    outputloc[tid] += maxout;
+    return {Ready, &overlay[tid]};
+}
+
+Operator::ResultCode AggregateOp::scanStop(unsigned short tid)
+{
+    if (global)
+        barrier->arrive();   // no thread frees the groups while another serves them
+    else
+        std::vector<char>().swap(groups[tid]);
+    return Ready;
+}
+
+void AggregateOp::destroy()
+{
+    if (handle)
+        pj_agg_destroy(handle);
+    handle = nullptr;
+}
+
 // ---------------------------------------------------------------- MergeOp
 
 void MergeOp::init(const Setting& root, const Setting& node)
@@ -595,6 +772,10 @@ bool MergeOp::run(const std::function<void(const Page&)>& consume)
 {
     std::vector<Operator*> sub;
     collect(nextOp, sub);
+    for (Operator* op : sub)   // a global aggregation's barrier waits for exactly its `threads`
+        if (auto* a = dynamic_cast<AggregateOp*>(op))
+            if (a->globalThreads() && a->globalThreads() != threads)
+                throw InvalidParameter("a global aggregation's threads must equal the merge's");
     std::mutex mu;
     std::atomic<bool> failed(false);
     std::vector<std::string> errors;
@@ -672,6 +853,12 @@ Operator* Query::build(const Setting& sub)
         ops.emplace_back(p);
         p->nextOp = build(sub["input"]);
         op = p;
+    } else if (type == "aggregate_sum" || type == "aggregate_count") {
+        auto* a = new AggregateOp();
+        ops.emplace_back(a);
+        a->count = type == "aggregate_count";
+        a->nextOp = build(sub["input"]);
+        op = a;
     } else if (type == "merge") {
         auto* m = new MergeOp();
         ops.emplace_back(m);

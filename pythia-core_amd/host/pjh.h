@@ -14,9 +14,11 @@
 #include <pthread.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -273,6 +275,45 @@ class PartitionOp : public Operator {
     std::vector<uint64_t> outputloc;
     std::vector<Page> overlay;
     int part_rc = 0;
+};
+
+// aggregate_sum / aggregate_count (GenericAggregate,
+// operators/genericaggregate.cpp:51-515, with AggregateSum's or
+// AggregateCount's fold) on the device. Global mode (a "global" key): every
+// thread stages its input, the last one to arrive has the device aggregate
+// all of it, and each of the `threads` threads serves an even share of the
+// groups. Thread-local mode: each thread aggregates its own input and serves
+// its own groups (a key can then appear once per thread, as in the
+// reference).
+class AggregateOp : public Operator {
+  public:
+    Operator* nextOp = nullptr;
+    bool count = false;   // aggregate_count; else aggregate_sum
+
+    void init(const Setting& root, const Setting& node) override;
+    void threadInit(unsigned short tid) override;
+    ResultCode scanStart(unsigned short tid) override;
+    GetNextResultT getNext(unsigned short tid) override;
+    ResultCode scanStop(unsigned short tid) override;
+    void destroy() override;
+    std::vector<Operator*> children() override { return {nextOp}; }
+    unsigned globalThreads() const { return global ? threads : 0u; }
+
+  private:
+    int aggregate(const std::vector<char>& in, std::vector<char>& out, uint64_t* ngroups);
+
+    pj_agg_desc desc = {};
+    pj_agg_t handle = nullptr;
+    std::mutex mu;                 // one device aggregation at a time
+    bool global = false;
+    unsigned threads = 1;
+    std::unique_ptr<Barrier> barrier;
+    std::vector<std::vector<char>> staged;    // per thread
+    std::vector<std::vector<char>> groups;    // per thread (global mode: all in [0])
+    std::vector<uint64_t> outputloc, outend;
+    std::vector<Page> overlay;
+    int agg_rc = 0;
+    std::atomic<unsigned> arrived{0};
 };
 
 // Runs its subtree on T threads and hands their pages to the single

@@ -416,6 +416,10 @@ int pj_smj_out_width(pj_smj_t h, uint32_t* width);
  * PJ_ERR_CAPACITY (nothing written) when it exceeds out_cap. Synchronous. */
 int pj_smj_run_device(pj_smj_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
                       void* d_out, uint64_t out_cap, uint64_t* out_n, void* stream);
+/* As pj_smj_run_device with host buffers (copied to the device and back),
+ * for the plan operators. Uses the null stream; synchronous. */
+int pj_smj_run_host(pj_smj_t h, const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe, void* out,
+                    uint64_t out_cap, uint64_t* out_n);
 int pj_smj_last_timings(pj_smj_t h, pj_smj_timings* t);
 int pj_smj_destroy(pj_smj_t h);
 

@@ -174,6 +174,7 @@ using namespace pj;
 
 struct pj_smj_s {
     Layout lay;
+    pj_schema desc_build, desc_probe;
     std::mutex mu;
     DevBuf keys[2][2], vals[2][2];   // [side][unsorted, sorted]
     DevBuf lo, cnt, off, temp, minmax;
@@ -181,6 +182,8 @@ struct pj_smj_s {
     pj_smj_timings last = {};
     hipEvent_t ev[6];
     bool ev_ok = false;
+    DevBuf host_b, host_p, host_o;   // the host path's device copies
+    std::mutex host_mu;              // guards them
 };
 
 namespace {
@@ -244,6 +247,8 @@ int pj_smj_create(const pj_join_desc* desc, pj_smj_t* out)
     if (!h)
         return fail(PJ_ERR_INVALID, "out of host memory");
     h->lay = L;
+    h->desc_build = desc->build;
+    h->desc_probe = desc->probe;
     *out = h;
     return PJ_OK;
 }
@@ -354,6 +359,33 @@ int pj_smj_run_device(pj_smj_t h, const void* d_build, uint64_t nbuild, const vo
         t.total_ms = ms;
     t.build_key_bits = rbits;
     t.probe_key_bits = sbits;
+    return PJ_OK;
+}
+
+int pj_smj_run_host(pj_smj_t h, const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe, void* out,
+                    uint64_t out_cap, uint64_t* out_n)
+{
+    if (!h || !out_n || (nbuild && !build) || (nprobe && !probe) || (out_cap && !out))
+        return fail(PJ_ERR_INVALID, "null argument");
+    *out_n = 0;
+    std::lock_guard<std::mutex> g(h->host_mu);
+    int boff[PJ_MAX_COLS], poff[PJ_MAX_COLS], bsize = 0, psize = 0;
+    int rc;
+    if ((rc = schema_offsets(h->desc_build, boff, &bsize)) || (rc = schema_offsets(h->desc_probe, poff, &psize)))
+        return rc;
+    const uint64_t ow = h->lay.out_width;
+    if ((rc = h->host_b.ensure(std::max<uint64_t>(nbuild * (uint64_t)bsize, 16))) ||
+        (rc = h->host_p.ensure(std::max<uint64_t>(nprobe * (uint64_t)psize, 16))) ||
+        (rc = h->host_o.ensure(std::max<uint64_t>(out_cap * ow, 16))))
+        return rc;
+    if (nbuild)
+        PJ_HIP(hipMemcpy(h->host_b.p, build, nbuild * (uint64_t)bsize, hipMemcpyHostToDevice));
+    if (nprobe)
+        PJ_HIP(hipMemcpy(h->host_p.p, probe, nprobe * (uint64_t)psize, hipMemcpyHostToDevice));
+    if ((rc = pj_smj_run_device(h, h->host_b.p, nbuild, h->host_p.p, nprobe, h->host_o.p, out_cap, out_n, nullptr)))
+        return rc;
+    if (*out_n)
+        PJ_HIP(hipMemcpy(out, h->host_o.p, *out_n * ow, hipMemcpyDeviceToHost));
     return PJ_OK;
 }
 

@@ -566,6 +566,179 @@ Operator::ResultCode PartitionOp::scanStop(unsigned short tid)
     return Ready;
 }
 
+// ---------------------------------------------------------------- SortMergeJoinOp
+
+// JoinOp::init + SortMergeJoinOp::init (operators/join.cpp:224-326,
+// 905-1020): key columns, projection, thread groups, and the staging sizes
+// the reference requires (maxbuildtuples / maxprobetuples, or the ...inM
+// forms); its presorted / prepartitioned hints change nothing here.
+void SortMergeJoinOp::init(const Setting& root, const Setting& node)
+{
+    Operator::init(root, node);
+    pj_join_desc d;
+    memset(&d, 0, sizeof(d));
+    Schema& bs = buildOp->getOutSchema();
+    Schema& ps = probeOp->getOutSchema();
+    bs.to_c(&d.build);
+    ps.to_c(&d.probe);
+    std::vector<unsigned short> bj = join_attrs(node["buildjattr"]);
+    std::vector<unsigned short> pj = join_attrs(node["probejattr"]);
+    if (bj.size() != pj.size() || bj.empty() || bj.size() > PJ_MAX_KEYS)
+        throw InvalidParameter("buildjattr and probejattr differ in length");
+    d.nkeys = (uint32_t)bj.size();
+    for (size_t i = 0; i < bj.size(); ++i) {
+        if (bj[i] >= bs.columns() || pj[i] >= ps.columns())
+            throw InvalidParameter("join attribute out of range");
+        d.buildjattr[i] = bj[i];
+        d.probejattr[i] = pj[i];
+    }
+    const Setting& proj = node["projection"];
+    if (!proj.isAggregate() || proj.getLength() == 0 || proj.getLength() > PJ_MAX_COLS)
+        throw InvalidParameter("projection");
+    d.nproj = (uint32_t)proj.getLength();
+    for (int i = 0; i < proj.getLength(); ++i) {
+        auto pr = parse_projection(proj[i].asString());
+        Schema& s = pr.first == PJ_SIDE_BUILD ? bs : ps;
+        if (pr.second >= s.columns())
+            throw InvalidParameter("projection " + proj[i].asString() + " out of range");
+        d.proj_side[i] = (uint8_t)pr.first;
+        d.proj_attr[i] = (uint16_t)pr.second;
+        schema.add(s.getColumnType(pr.second), s.getColumnWidth(pr.second));
+    }
+    if (!node.exists("maxbuildtuples") && !node.exists("maxbuildtuplesinM"))
+        throw MissingParameterException("maxbuildtuples");
+    if (!node.exists("maxprobetuples") && !node.exists("maxprobetuplesinM"))
+        throw MissingParameterException("maxprobetuples");
+    group_of.assign(MAX_THREADS, -1);
+    const Setting& tg = node["threadgroups"];
+    for (int g = 0; g < tg.getLength(); ++g) {
+        const Setting& grp = tg[g];
+        if (grp.getLength() == 0)
+            throw InvalidParameter("empty thread group");
+        members.emplace_back();
+        for (int j = 0; j < grp.getLength(); ++j) {
+            const int64_t t = grp[j].asInt();
+            if (t < 0 || t >= MAX_THREADS || group_of[t] != -1)
+                throw InvalidParameter("thread " + std::to_string(t) + " in two groups");
+            group_of[t] = g;
+            members.back().push_back((unsigned short)t);
+        }
+        leader_of_group.push_back((int)grp[0].asInt());
+        barriers.emplace_back(new Barrier((unsigned)grp.getLength()));
+        pj_smj_t h = nullptr;
+        const int rc = pj_smj_create(&d, &h);
+        if (rc == PJ_ERR_UNSUPPORTED)
+            throw IllegalSchemaDeclarationException(last_pj_error());
+        if (rc)
+            throw InvalidParameter(last_pj_error());
+        handles.push_back(h);
+    }
+    build_staged.assign(MAX_THREADS, {});
+    probe_staged.assign(MAX_THREADS, {});
+    rows.assign(leader_of_group.size(), {});
+    join_rc.assign(leader_of_group.size(), PJ_OK);
+    outputloc.assign(MAX_THREADS, 0);
+    outend.assign(MAX_THREADS, 0);
+    overlay.assign(MAX_THREADS, Page());
+}
+
+void SortMergeJoinOp::threadInit(unsigned short tid)
+{
+    if (group_of.at(tid) < 0)
+        throw InvalidParameter("thread " + std::to_string(tid) + " is in no thread group");
+}
+
+static Operator::ResultCode drain(Operator* op, unsigned short tid, std::vector<char>& to)
+{
+    Operator::ResultCode rc = Operator::Ready;
+    if (op->scanStart(tid) == Operator::Error)
+        rc = Operator::Error;
+    Operator::GetNextResultT r(Operator::Ready, nullptr);
+    while (rc != Operator::Error && r.first == Operator::Ready) {
+        r = op->getNext(tid);
+        if (r.first == Operator::Error)
+            rc = Operator::Error;
+        else if (r.second && r.second->getUsedSpace())
+            to.insert(to.end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
+    }
+    if (op->scanStop(tid) == Operator::Error)
+        rc = Operator::Error;
+    return rc;
+}
+
+// SortMergeJoinOp::scanStart (join.cpp:1234-1292, BufferAndSort
+// :1181-1232): every thread stages both inputs; the group's leader joins
+// them on the device; each thread then serves its share of the rows.
+Operator::ResultCode SortMergeJoinOp::scanStart(unsigned short tid)
+{
+    const int g = group_of.at(tid);
+    ResultCode rc = drain(buildOp, tid, build_staged[tid]);
+    if (drain(probeOp, tid, probe_staged[tid]) == Error)
+        rc = Error;
+    barriers[g]->arrive();
+    if (tid == leader_of_group[g]) {
+        std::vector<char> b, p;
+        for (unsigned short t : members[g]) {
+            b.insert(b.end(), build_staged[t].begin(), build_staged[t].end());
+            p.insert(p.end(), probe_staged[t].begin(), probe_staged[t].end());
+            std::vector<char>().swap(build_staged[t]);
+            std::vector<char>().swap(probe_staged[t]);
+        }
+        const uint64_t bw = buildOp->getOutSchema().getTupleSize(), pw = probeOp->getOutSchema().getTupleSize();
+        const uint64_t ow = schema.getTupleSize(), nb = bw ? b.size() / bw : 0, np = pw ? p.size() / pw : 0;
+        uint64_t cap = std::max<uint64_t>(np, 1), n = 0;
+        rows[g].resize(cap * ow);
+        int jrc = pj_smj_run_host(handles[g], b.data(), nb, p.data(), np, rows[g].data(), cap, &n);
+        if (jrc == PJ_ERR_CAPACITY && n > cap) {   // many-to-many: room for every row, once more
+            cap = n;
+            rows[g].resize(cap * ow);
+            jrc = pj_smj_run_host(handles[g], b.data(), nb, p.data(), np, rows[g].data(), cap, &n);
+        }
+        rows[g].resize(jrc == PJ_OK ? n * ow : 0);
+        join_rc[g] = jrc;
+        const uint64_t k = members[g].size();
+        for (uint64_t i = 0; i < k; ++i) {
+            const unsigned short t = members[g][i];
+            outputloc[t] = (jrc == PJ_OK ? n : 0) * i / k;
+            outend[t] = (jrc == PJ_OK ? n : 0) * (i + 1) / k;
+        }
+    }
+    barriers[g]->arrive();
+    return rc == Error || join_rc[g] != PJ_OK ? Error : Ready;
+}
+
+// SortMergeJoinOp::getNext (join.cpp:1294-1411): this thread's rows in
+// buffsize slices, Finished with the last one.
+Operator::GetNextResultT SortMergeJoinOp::getNext(unsigned short tid)
+{
+    std::vector<char>& r = rows[group_of.at(tid)];
+    const unsigned ow = schema.getTupleSize();
+    const uint64_t maxout = std::max<uint64_t>(buffsize / ow, 1);
+    const uint64_t remaining = outend[tid] - outputloc[tid];
+    char* start = r.data() + outputloc[tid] * ow;
+    if (remaining <= maxout) {
+        overlay[tid] = Page(start, remaining * ow, ow, true);
+        outputloc[tid] += remaining;
+        return {Finished, &overlay[tid]};
+    }
+    overlay[tid] = Page(start, maxout * ow, ow, true);
+    outputloc[tid] += maxout;
+    return {Ready, &overlay[tid]};
+}
+
+Operator::ResultCode SortMergeJoinOp::scanStop(unsigned short tid)
+{
+    barriers[group_of.at(tid)]->arrive();   // no thread frees the rows while another serves them
+    return Ready;
+}
+
+void SortMergeJoinOp::destroy()
+{
+    for (pj_smj_t h : handles)
+        pj_smj_destroy(h);
+    handles.clear();
+}
+
 // ---------------------------------------------------------------- AggregateOp
 
 // GenericAggregate::init (operators/genericaggregate.cpp:51-180) with
@@ -853,6 +1026,12 @@ Operator* Query::build(const Setting& sub)
         ops.emplace_back(p);
         p->nextOp = build(sub["input"]);
         op = p;
+    } else if (type == "sortmergejoin" || type == "mpsmjoin" || type == "newmpsmjoin") {
+        auto* j = new SortMergeJoinOp();
+        ops.emplace_back(j);
+        j->buildOp = build(sub["build"]);
+        j->probeOp = build(sub["probe"]);
+        op = j;
     } else if (type == "aggregate_sum" || type == "aggregate_count") {
         auto* a = new AggregateOp();
         ops.emplace_back(a);

@@ -277,6 +277,35 @@ class PartitionOp : public Operator {
     int part_rc = 0;
 };
 
+// sortmergejoin / mpsmjoin (SortMergeJoinOp, MPSMJoinOp,
+// operators/join.cpp:905-1791) on the device: the threads of a group stage
+// their build and probe pages, the group's leader joins them with the device
+// sort-merge join, and each thread of the group serves an even share of the
+// rows. Same rows as a hashjoin node with the same projection.
+class SortMergeJoinOp : public Operator {
+  public:
+    Operator* buildOp = nullptr;
+    Operator* probeOp = nullptr;
+
+    void init(const Setting& root, const Setting& node) override;
+    void threadInit(unsigned short tid) override;
+    ResultCode scanStart(unsigned short tid) override;
+    GetNextResultT getNext(unsigned short tid) override;
+    ResultCode scanStop(unsigned short tid) override;
+    void destroy() override;
+    std::vector<Operator*> children() override { return {buildOp, probeOp}; }
+
+  private:
+    std::vector<pj_smj_t> handles;   // one a thread group
+    std::vector<int> group_of, leader_of_group;
+    std::vector<std::vector<unsigned short>> members;
+    std::vector<std::unique_ptr<Barrier>> barriers;
+    std::vector<std::vector<char>> build_staged, probe_staged, rows;   // per thread / per group
+    std::vector<int> join_rc;
+    std::vector<uint64_t> outputloc, outend;
+    std::vector<Page> overlay;
+};
+
 // aggregate_sum / aggregate_count (GenericAggregate,
 // operators/genericaggregate.cpp:51-515, with AggregateSum's or
 // AggregateCount's fold) on the device. Global mode (a "global" key): every

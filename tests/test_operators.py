@@ -178,3 +178,57 @@ treeroot: { name = "m"; input: { name = "j"; build: { name = "b"; }; probe: { na
     rows = P.run_plan(plan, {}, 100000 * 24)
     s, c = O.expected_pkfk(32, 1 << 40, 20000, 100000, 0)
     assert rows.shape[0] == c and O.checksum(rows, 24)[0] == s
+
+
+SMJ_PLAN = """
+buffsize = 65536;
+build: { type = "memscan"; schema = ( %(bschema)s ); };
+probe: { type = "memscan"; schema = ( %(pschema)s ); };
+join: { type = "%(type)s"; threadgroups = ( [0, 1, 2, 3] ); maxbuildtuples = %(nb)d; maxprobetuples = %(np)d;
+        buildjattr = 0; probejattr = 0; projection = ( %(proj)s ); };
+merge: { type = "merge"; threads = 4; };
+treeroot: { name = "merge"; input: { name = "join"; build: { name = "build"; }; probe: { name = "probe"; }; }; };
+"""
+
+
+def smj_plan(bcols, pcols, proj, nb, npr, kind="sortmergejoin"):
+    q = lambda cols: ", ".join('"%s"' % c for c in cols)
+    return SMJ_PLAN % dict(bschema=q(bcols), pschema=q(pcols), type=kind, nb=nb, np=npr,
+                           proj=", ".join('"%s"' % p for p in proj))
+
+
+def test_sortmerge_plan_needs_its_staging_sizes():
+    # SortMergeJoinOp::init reads maxbuildtuples / maxprobetuples (join.cpp:934-956)
+    p = smj_plan(["long", "long"], ["long", "long"], ["B$0", "P$1"], 10, 10).replace("maxbuildtuples = 10;", "")
+    with pytest.raises(P.PJError) as e:
+        P.run_plan(p, {"build": (np.zeros(16, np.uint8), 1), "probe": (np.zeros(16, np.uint8), 1)})
+    assert "maxbuildtuples" in str(e.value)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["sortmergejoin", "mpsmjoin"])
+def test_sortmerge_plan_equals_oracle(kind):
+    b, p = O.gen_pkfk(34, 1 << 40, 30000, 120000, 12000)
+    proj = ["B$0", "B$1", "P$1"]
+    got = P.run_plan(smj_plan(["long", "long"], ["long", "long"], proj, 30000, 120000, kind),
+                     {"build": (b.view(np.uint8), 30000), "probe": (p.view(np.uint8), 120000)})
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], proj, buckets=1 << 15)
+    exp, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
+
+
+@pytest.mark.gpu
+def test_sortmerge_plan_many_to_many():
+    rng = np.random.default_rng(35)
+    nb, npr = 2000, 9000
+    build = pack(["int", "long"], [(int(k), int(v)) for k, v in zip(rng.integers(-200, 200, nb),
+                                                                    rng.integers(0, 1 << 62, nb))])
+    probe = pack(["long", "int"], [(int(k), int(v)) for k, v in zip(rng.integers(-200, 200, npr),
+                                                                    rng.integers(0, 1 << 31, npr))])
+    proj = ["P$1", "B$1", "B$0"]
+    got = P.run_plan(smj_plan(["int", "long"], ["long", "int"], proj, nb, npr),
+                     {"build": (build, nb), "probe": (probe, npr)})
+    d = O.join_desc(["int", "long"], ["long", "int"], [0], [0], proj, buckets=1 << 10)
+    exp, _ = O.hashjoin(d, build, probe)
+    assert got.shape[0] > npr
+    assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))

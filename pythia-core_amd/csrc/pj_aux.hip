@@ -634,6 +634,9 @@ int pj_partition_exactrange_device(const pj_schema* schema, uint32_t attr, int64
         part_sizes[p] = e - starts[p];
         total += part_sizes[p];
     }
+    if (total != n)
+        return fail(PJ_ERR_STATE, "partition sizes add up to %llu of %llu tuples", (unsigned long long)total,
+                    (unsigned long long)n);
     return PJ_OK;
 }
 

@@ -48,8 +48,8 @@ def lib():
 
 
 def default_threads():
-    """One thread per core the process may use, at most 16 (a GPU box's
-    host share)."""
+    """One thread per core the process may use, at most 16 (the host
+    cores that come with one GPU)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

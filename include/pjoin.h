@@ -132,6 +132,10 @@ int pj_join_run_device(pj_join_t h, const void* d_build, uint64_t nbuild,
  * Synchronous. */
 int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe, uint64_t nprobe,
                          void* d_groups, uint64_t groups_cap, uint64_t* ngroups, void* stream);
+/* As pj_join_count_by_key with host buffers (copied to the device and
+ * back), for the plan operators. Uses the null stream; synchronous. */
+int pj_join_count_by_key_host(pj_join_t h, const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe,
+                              void* groups, uint64_t groups_cap, uint64_t* ngroups);
 int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n);
 /* Uploads and partitions the staged build side once the group barrier has
  * been passed (join.cpp:618). Exactly once per scan. */

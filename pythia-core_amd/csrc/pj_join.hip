@@ -1881,6 +1881,35 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
     return rc;
 }
 
+int pj_join_count_by_key_host(pj_join_t h, const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe,
+                              void* groups, uint64_t groups_cap, uint64_t* ngroups)
+{
+    if (!h || !ngroups || (nbuild && !build) || (nprobe && !probe) || (groups_cap && !groups))
+        return fail(PJ_ERR_INVALID, "null argument");
+    *ngroups = 0;
+    int boff[PJ_MAX_COLS], poff[PJ_MAX_COLS], bsize = 0, psize = 0;
+    int rc;
+    if ((rc = schema_offsets(h->desc.build, boff, &bsize)) || (rc = schema_offsets(h->desc.probe, poff, &psize)))
+        return rc;
+    const uint32_t key_words = h->desc.nkeys == 2 || h->desc.build.type[h->desc.buildjattr[0]] != PJ_CT_INTEGER ? 2u
+                                                                                                                : 1u;
+    const uint64_t gw = (key_words + 2) * 4;
+    DevBuf bdev, pdev, gdev;   // this call's own device copies
+    if ((rc = bdev.ensure(std::max<uint64_t>(nbuild * (uint64_t)bsize, 16))) ||
+        (rc = pdev.ensure(std::max<uint64_t>(nprobe * (uint64_t)psize, 16))) ||
+        (rc = gdev.ensure(std::max<uint64_t>(groups_cap * gw, 16))))
+        return rc;
+    if (nbuild)
+        PJ_HIP(hipMemcpy(bdev.p, build, nbuild * (uint64_t)bsize, hipMemcpyHostToDevice));
+    if (nprobe)
+        PJ_HIP(hipMemcpy(pdev.p, probe, nprobe * (uint64_t)psize, hipMemcpyHostToDevice));
+    if ((rc = pj_join_count_by_key(h, bdev.p, nbuild, pdev.p, nprobe, gdev.p, groups_cap, ngroups, nullptr)))
+        return rc;
+    if (*ngroups)
+        PJ_HIP(hipMemcpy(groups, gdev.p, *ngroups * gw, hipMemcpyDeviceToHost));
+    return PJ_OK;
+}
+
 int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n)
 {
     if (!h || (n && !tuples))

@@ -277,6 +277,7 @@ void HashJoinOp::init(const Setting& root, const Setting& node)
     if (node.exists("probebatch"))
         probe_batch_tuples = (uint64_t)node["probebatch"].asInt();
 
+    desc_ = d;
     // one device hash table per thread group, as the reference keeps one
     // HashTable per group
     for (size_t g = 0; g < leader_of_group.size(); ++g) {
@@ -290,6 +291,16 @@ void HashJoinOp::init(const Setting& root, const Setting& node)
     }
     state.resize(MAX_THREADS);
     build_rc.assign(leader_of_group.size(), PJ_OK);
+}
+
+bool HashJoinOp::oneGroupOf(unsigned n) const
+{
+    if (groupsize.size() != 1 || (unsigned)groupsize[0] != n || !leftthreads.empty())
+        return false;
+    for (unsigned t = 0; t < n; ++t)
+        if (group_of[t] != 0)
+            return false;
+    return true;
 }
 
 // HashJoinOp::threadInit (join.cpp:520-543): per-thread state and output page.
@@ -804,6 +815,15 @@ void AggregateOp::init(const Setting& root, const Setting& node)
         throw IllegalSchemaDeclarationException(pj_last_error());
     if (rc)
         throw InvalidParameter(pj_last_error());
+    // fusable: a global count whose groups are the join's build key columns, in order
+    if (auto* j = dynamic_cast<HashJoinOp*>(nextOp)) {
+        const pj_join_desc& jd = j->descriptor();
+        bool same = count && global && !fields.empty() && fields.size() == jd.nkeys && j->oneGroupOf(threads);
+        for (size_t k = 0; same && k < fields.size(); ++k)
+            same = jd.proj_side[fields[k]] == PJ_SIDE_BUILD && jd.proj_attr[fields[k]] == jd.buildjattr[k];
+        if (same)
+            fused = j;
+    }
     staged.assign(MAX_THREADS, {});
     groups.assign(MAX_THREADS, {});
     outputloc.assign(MAX_THREADS, 0);
@@ -838,10 +858,50 @@ int AggregateOp::aggregate(const std::vector<char>& in, std::vector<char>& out, 
     return rc;
 }
 
+// The fused count: every thread stages its share of the join's two inputs,
+// thread 0 counts per build key inside the join, and each thread serves an
+// even share of the groups.
+Operator::ResultCode AggregateOp::scanStartFused(unsigned short tid)
+{
+    std::vector<char> b, p;
+    ResultCode rc = drain(fused->buildOp, tid, staged[tid]);
+    if (drain(fused->probeOp, tid, groups[tid]) == Error)   // groups[tid] holds the probe pages until the count
+        rc = Error;
+    outputloc[tid] = outend[tid] = 0;
+    barrier->arrive();
+    if (tid == 0) {
+        for (unsigned t = 0; t < threads; ++t) {
+            b.insert(b.end(), staged[t].begin(), staged[t].end());
+            p.insert(p.end(), groups[t].begin(), groups[t].end());
+            std::vector<char>().swap(staged[t]);
+            std::vector<char>().swap(groups[t]);
+        }
+        const uint64_t bw = fused->buildOp->getOutSchema().getTupleSize();
+        const uint64_t pw = fused->probeOp->getOutSchema().getTupleSize();
+        const uint64_t nb = bw ? b.size() / bw : 0, np = pw ? p.size() / pw : 0;
+        const uint64_t ow = schema.getTupleSize(), cap = std::max<uint64_t>(nb, 1);
+        uint64_t ng = 0;
+        groups[0].resize(cap * ow);
+        std::lock_guard<std::mutex> g(mu);
+        agg_rc = pj_join_count_by_key_host(fused->handleOf(0), b.data(), nb, p.data(), np, groups[0].data(), cap, &ng);
+        groups[0].resize(agg_rc == PJ_OK ? ng * ow : 0);
+        if (agg_rc != PJ_OK)
+            ng = 0;
+        for (unsigned t = 0; t < threads; ++t) {
+            outputloc[t] = ng * t / threads;
+            outend[t] = ng * (t + 1) / threads;
+        }
+    }
+    barrier->arrive();
+    return rc == Error || agg_rc != PJ_OK ? Error : Ready;
+}
+
 // GenericAggregate::scanStart (genericaggregate.cpp:291-424): drain the
 // input, then aggregate it (global mode: all threads' input, once).
 Operator::ResultCode AggregateOp::scanStart(unsigned short tid)
 {
+    if (fused)
+        return scanStartFused(tid);
     ResultCode rc = Ready;
     if (nextOp->scanStart(tid) == Error)
         rc = Error;

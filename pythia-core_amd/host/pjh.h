@@ -231,6 +231,11 @@ class HashJoinOp : public Operator {
     // Probe tuples gathered per device call.
     uint64_t probe_batch_tuples = 1 << 22;
 
+    const pj_join_desc& descriptor() const { return desc_; }
+    pj_join_t handleOf(unsigned g) const { return handles.at(g); }
+    // Whether threads 0 .. n-1 form the only thread group and all of them build.
+    bool oneGroupOf(unsigned n) const;
+
   private:
     struct ThreadState {
         pj_cursor cur = {nullptr, 0};
@@ -242,6 +247,7 @@ class HashJoinOp : public Operator {
     ResultCode fillPage(unsigned short tid, ThreadState& st);
 
     std::vector<pj_join_t> handles;   // one device table per thread group
+    pj_join_desc desc_ = {};
     std::vector<int> build_rc;
     HashSpec hash;
     unsigned tuplesperbucket = 0;
@@ -330,6 +336,12 @@ class AggregateOp : public Operator {
 
   private:
     int aggregate(const std::vector<char>& in, std::vector<char>& out, uint64_t* ngroups);
+    ResultCode scanStartFused(unsigned short tid);
+
+    // A count grouped by the build key over a hashjoin child that runs as one
+    // thread group: counted inside the join (pj_join_count_by_key), without
+    // the join's rows.
+    HashJoinOp* fused = nullptr;
 
     pj_agg_desc desc = {};
     pj_agg_t handle = nullptr;

@@ -93,10 +93,15 @@ def test_thread_local_aggregation_partials_add_up():
 
 
 @pytest.mark.gpu
-def test_count_per_key_over_a_join_plan():
+@pytest.mark.parametrize("field", [0, 1])
+def test_count_per_key_over_a_join_plan(field):
     """aggregate_count (global) over a hashjoin, the shape of a grouped join
-    query, against the oracle's aggregate over the oracle's join."""
+    query, against the oracle's aggregate over the oracle's join. Grouped by
+    the build key (field 0) the count is taken inside the join; grouped by
+    the probe payload (field 1) over the join's rows."""
     b, p = O.gen_pkfk(33, 1 << 40, 20000, 100000, 10000)
+    if field == 1:   # few distinct payloads, so that groups gather many rows
+        p.reshape(-1, 2)[:, 1] %= 97
     text = """
 buffsize = 65536;
 build: { type = "memscan"; schema = ( "long", "long" ); };
@@ -104,14 +109,14 @@ probe: { type = "memscan"; schema = ( "long", "long" ); };
 join: { type = "hashjoin"; hash: { fn = "modulo"; buckets = 32768; }; threadgroups = ( [0, 1, 2] );
         tuplesperbucket = 2; allocpolicy = "local"; buildjattr = 0; probejattr = 0;
         projection = ( "B$0", "P$1" ); };
-agg: { type = "aggregate_count"; field = 0; hash: { fn = "modulo"; buckets = 32768; field = 0; };
+agg: { type = "aggregate_count"; field = %d; hash: { fn = "modulo"; buckets = 32768; field = %d; };
        global = "yes"; threads = 3; };
 merge: { type = "merge"; threads = 3; };
 treeroot: { name = "merge"; input: { name = "agg"; input: { name = "join"; build: { name = "build"; };
             probe: { name = "probe"; }; }; }; };
-"""
+""" % (field, field)
     got = P.run_plan(text, {"build": (b.view(np.uint8), 20000), "probe": (p.view(np.uint8), 100000)})
     d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "P$1"], buckets=1 << 15)
     jrows, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
-    exp = O.aggregate(["long", "long"], [0], "count", None, jrows)
+    exp = O.aggregate(["long", "long"], [field], "count", None, jrows)
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))

@@ -68,30 +68,6 @@ __device__ __forceinline__ uint32_t range_part(const uint64_t* bnd, uint32_t F, 
     return pos < F - 1 ? pos : F - 1;
 }
 
-// range_part for K keys at once: the searches advance level by level
-// together, so K independent LDS loads are in flight at every step.
-template <int K>
-__device__ __forceinline__ void range_part_multi(const uint64_t* bnd, uint32_t F, uint32_t p2, const uint64_t* u,
-                                                 uint32_t* pos)
-{
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        pos[k] = 0;
-    for (uint32_t step = p2 >> 1; step; step >>= 1) {
-        uint64_t b[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            b[k] = bnd[pos[k] + step - 1];
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (b[k] <= u[k])
-                pos[k] += step;
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        pos[k] = pos[k] < F - 1 ? pos[k] : F - 1;
-}
-
 // Partition of a key: RANGE selects the ordered-key ranges in lds_bnd,
 // otherwise the top pf.bits bits of the key mix.
 template <bool RANGE>

@@ -41,6 +41,70 @@ namespace pj {
 
 constexpr int kHistThreads = 1024;
 constexpr int kHistItems = 8;
+constexpr uint32_t kDirBits = 12;   // range directory: 4096 equal slices of the boundaries' key range
+
+// A directory over the F - 1 ascending range boundaries bnd[0 .. F-2]: the
+// boundaries' key range [bnd[0], bnd[F-2]] cut into 2^kDirBits slices of
+// 2^shift keys, dir[d] = the boundaries below slice d's first key. The
+// partition of key u (the boundaries <= u, as range_part counts them) is
+// then dir[d] plus the few boundaries inside u's slice.
+struct RangeDir {
+    const uint64_t* bnd;
+    const uint32_t* dir;
+    uint64_t umin, umax;
+    uint32_t shift, nb;   // nb = F - 1 boundaries
+    __device__ __forceinline__ uint32_t part(uint64_t u) const
+    {
+        if (nb == 0 || u < umin)
+            return 0;
+        if (u >= umax)
+            return nb;
+        const uint32_t d = (uint32_t)((u - umin) >> shift);
+        uint32_t lo = dir[d], hi = dir[d + 1];   // the boundaries in slice d lie in [lo, hi)
+        while (lo < hi) {                        // first of them > u
+            const uint32_t mid = (lo + hi) >> 1;
+            if (bnd[mid] <= u)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+};
+
+// Fills dir (2^kDirBits + 1 entries) for the F - 1 boundaries in bnd (LDS,
+// already loaded); the caller synchronises before the first lookup.
+__device__ __forceinline__ RangeDir build_range_dir(const uint64_t* bnd, uint32_t F, uint32_t* dir)
+{
+    RangeDir R;
+    R.bnd = bnd;
+    R.dir = dir;
+    R.nb = F - 1;
+    R.umin = F > 1 ? bnd[0] : 0;
+    R.umax = F > 1 ? bnd[F - 2] : 0;
+    uint32_t sh = 0;
+    while (sh < 64 && ((R.umax - R.umin) >> sh) >= (1ull << kDirBits))
+        ++sh;
+    R.shift = sh;
+    for (uint32_t d = threadIdx.x; d <= (1u << kDirBits); d += blockDim.x) {
+        uint32_t c = R.nb;   // a slice that starts past umax: every boundary lies below it
+        const uint64_t off = (uint64_t)d << sh;
+        if (R.nb && (off >> sh) == d && off <= R.umax - R.umin) {
+            const uint64_t start = R.umin + off;
+            uint32_t lo = 0, hi = R.nb;          // boundaries < start
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bnd[mid] < start)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            c = lo;
+        }
+        dir[d] = c;
+    }
+    return R;
+}
 
 // hist[p * nblk + blk] = tuples of block blk's chunk in partition p, rounded
 // up to a multiple of pad. With pmin/pmax (range partitions of a build side)
@@ -62,6 +126,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     unsigned long long* lmin = reinterpret_cast<unsigned long long*>(bnd + (RANGE ? pf.p2 - 1 : 0));
     unsigned long long* lmax = lmin + (mm ? F : 0);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(lmax + (mm ? F : 0));
+    uint32_t* dirw = cnt + F;   // RANGE: 2^kDirBits + 1 directory entries
     load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < F; p += kHistThreads) {
         cnt[p] = 0;
@@ -71,6 +136,11 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
         }
     }
     __syncthreads();
+    RangeDir rdir = {};
+    if (RANGE) {
+        rdir = build_range_dir(bnd, F, dirw);
+        __syncthreads();
+    }
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     for (uint64_t base = beg; base < end; base += (uint64_t)kHistThreads * kHistItems) {
@@ -83,12 +153,10 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
                 r[k] = load_rec<IDENT>(in, i, L);
         }
         uint32_t part[kHistItems];
-        if (RANGE) {   // all records' searches advance together
-            uint64_t u[kHistItems];
+        if (RANGE) {
 #pragma unroll
             for (int k = 0; k < kHistItems; ++k)
-                u[k] = ord_key(rec_key(r[k]));   // records past the end search harmlessly
-            range_part_multi<kHistItems>(bnd, F, pf.p2, u, part);
+                part[k] = rdir.part(ord_key(rec_key(r[k])));   // records past the end search harmlessly
         }
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
@@ -1236,7 +1304,8 @@ int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
                 uint32_t nblk, uint32_t pad, uint32_t* hist, unsigned long long* pmin, unsigned long long* pmax,
                 uint16_t* pid, hipStream_t st)
 {
-    const size_t lds = (size_t)pf.F * 4 + bounds_lds_bytes(pf, RANGE) + (pmin ? (size_t)pf.F * 16 : 0);
+    const size_t lds = (size_t)pf.F * 4 + bounds_lds_bytes(pf, RANGE) + (pmin ? (size_t)pf.F * 16 : 0) +
+                       (RANGE ? ((1u << kDirBits) + 1) * 4 : 0);
     if (lds > kLdsBytes)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the histogram's LDS budget", pf.F);
     const void* fn = ident ? (const void*)k_hist<true, RANGE> : (const void*)k_hist<false, RANGE>;

@@ -389,3 +389,31 @@ def test_full_64m_x_512m_clustered(partitioner):
     assert (cnt, s) == (ec, es)
     if partitioner == "learned":
         assert used == 1
+
+
+def _fin64(x):
+    m = (1 << 64) - 1
+    x ^= x >> 30
+    x = (x * 0xbf58476d1ce4e5b9) & m
+    x ^= x >> 27
+    x = (x * 0x94d049bb133111eb) & m
+    return x ^ (x >> 31)
+
+
+@pytest.mark.parametrize("bits", [0, 4, 11, 14])
+def test_keys_equal_to_padding_and_extremes(bits):
+    """Real tuples whose keys equal the partition pass's padding keys (0, and
+    the first key whose mix lands outside partition 0), -1, INT64_MIN and
+    INT64_MAX, on both sides and repeated, among ordinary keys: every
+    partition count, padding included, still joins exactly."""
+    _say("case: padding / extreme keys bits=%d" % bits)
+    pk = next(k for k in range(1, 1 << 20) if _fin64(k) >> 63)
+    special = np.array([0, pk, -1, -(1 << 63), (1 << 63) - 1], np.int64)
+    rng = np.random.default_rng(90 + bits)
+    bkeys = np.concatenate([special, special[:2], rng.integers(-(1 << 40), 1 << 40, 50000)])
+    pkeys = np.concatenate([np.repeat(special, 300), rng.choice(bkeys, 200000), rng.integers(0, 1 << 40, 1000)])
+    rng.shuffle(pkeys)
+    build = np.stack([bkeys, np.arange(bkeys.size, dtype=np.int64)], 1).reshape(-1)
+    probe = np.stack([pkeys, np.arange(pkeys.size, dtype=np.int64)], 1).reshape(-1)
+    got, _ = compare(HEADLINE, build.view(np.uint8), probe.view(np.uint8), bkeys.size, pkeys.size, radix_bits=bits)
+    assert got.shape[0] >= 300 * 5

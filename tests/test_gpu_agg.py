@@ -212,3 +212,22 @@ def test_join_then_partitioned_count_by_key():
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))
     j.close()
     agg.close()
+
+
+def test_partitioned_aggregate_padding_and_extreme_keys():
+    """Group keys equal to the partition pass's padding keys (0, and the
+    first key whose mix lands outside partition 0), -1, INT64_MIN and
+    INT64_MAX among many groups: the padding is skipped, the real groups
+    counted."""
+    _say("case: partitioned agg padding / extreme keys")
+    from test_gpu_join import _fin64
+    pk = next(k for k in range(1, 1 << 20) if _fin64(k) >> 63)
+    rng = np.random.default_rng(12)
+    cols = ["long", "long"]
+    t, rows = _table(cols, 1 << 21, rng, {0: 150000})
+    special = np.array([0, pk, -1, -(1 << 63), (1 << 63) - 1], np.int64)
+    t["c0"][: 5 * 1000] = np.repeat(special, 1000)
+    for op in ("sum", "count"):
+        exp = O.aggregate(cols, [0], op, 1, rows)
+        got = _device_agg(cols, [0], op, 1, rows, max_groups=1 << 18)
+        assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp))

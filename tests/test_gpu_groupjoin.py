@@ -149,3 +149,20 @@ def test_full_16m_x_256m_equals_join_then_aggregate():
     b = b[torch.argsort(b[:, 0])]
     assert torch.equal(a, b)
     assert int(a[:, 1].sum()) == n
+
+
+def test_padding_and_extreme_keys():
+    _say("case: count_by_key padding / extreme keys")
+    from test_gpu_join import _fin64
+    pk = next(k for k in range(1, 1 << 20) if _fin64(k) >> 63)
+    special = np.array([0, pk, -1, -(1 << 63), (1 << 63) - 1], np.int64)
+    rng = np.random.default_rng(61)
+    bkeys = np.concatenate([special, rng.choice(1 << 40, 60000, replace=False)])
+    pkeys = np.concatenate([np.repeat(special, 400), rng.choice(bkeys, 300000)])
+    rng.shuffle(pkeys)
+    build = np.stack([bkeys, np.arange(bkeys.size, dtype=np.int64)], 1).reshape(-1)
+    probe = np.stack([pkeys, np.arange(pkeys.size, dtype=np.int64)], 1).reshape(-1)
+    cols = ["long", "long"]
+    for bits in (-1, 11, 14):
+        check(cols, cols, [0], [0], build.view(np.uint8), probe.view(np.uint8), bkeys.size, pkeys.size,
+              radix_bits=bits)

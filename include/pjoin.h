@@ -428,6 +428,32 @@ int pj_smj_last_timings(pj_smj_t h, pj_smj_timings* t);
 int pj_smj_destroy(pj_smj_t h);
 
 /* ------------------------------------------------------------------------
+ * TupleHasher on the device: the bucket id of every tuple under one of the
+ * reference's hash functions (TupleHasher::create, hash.cpp:101-248, and
+ * the hashers of hash.h:104-487), the ids a PartitionOp or a hash exchange
+ * exposes. The join itself partitions on its own mix.
+ * ---------------------------------------------------------------------- */
+enum { PJ_HASH_MODULO = 0, PJ_HASH_PARAMMODULO = 1, PJ_HASH_KNUTH = 2, PJ_HASH_RANGE = 3, PJ_HASH_EXACTRANGE = 4,
+       PJ_HASH_BYTES = 5, PJ_HASH_TPCHORDERKEY = 6, PJ_HASH_WILLIS = 7, PJ_HASH_ALWAYSZERO = 8,
+       PJ_HASH_TPCHQ1MAGIC = 9 };
+
+typedef struct {
+    uint32_t fn;                   /* PJ_HASH_* (a plan's hash.fn) */
+    uint32_t buckets;              /* hash.buckets; rounded up to a power of two except by exactrange */
+    int64_t range_min, range_max;  /* hash.range, for range and exactrange */
+    uint32_t offset, skipbits;     /* hash.offset, hash.skipbits (parammodulo, knuth) */
+} pj_hash_spec;
+
+/* HashFunction::buckets() of the function spec builds. */
+int pj_hash_buckets(const pj_hash_spec* spec, uint32_t* buckets);
+/* d_buckets[i] = the bucket of tuple i (n tuples of tuple_width bytes at
+ * d_tuples, device memory). Value hashers read the key_size (4: INTEGER,
+ * sign-extended; 8: LONG or DATE) bytes at key_offset; the byte hasher
+ * folds key_size bytes there; tpchq1magic reads 4. Asynchronous on stream. */
+int pj_tuple_hash_device(const pj_hash_spec* spec, const void* d_tuples, uint64_t n, uint32_t tuple_width,
+                         uint32_t key_offset, uint32_t key_size, uint32_t* d_buckets, void* stream);
+
+/* ------------------------------------------------------------------------
  * Raw binary tables into HBM: the binary scan source of the reference,
  * MemMappedTable::load (operators/loaders/table.cpp:170-347), which globs
  * the pattern and maps every non-empty regular file it names, in glob's

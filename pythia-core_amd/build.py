@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip", "pj_agg.hip", "pj_load.hip", "pj_smj.hip"]
+KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip", "pj_agg.hip", "pj_load.hip", "pj_smj.hip", "pj_hash.hip"]
 HOST_SOURCES = ["schema.cpp", "operators.cpp"]
 
 

@@ -136,6 +136,9 @@ _SIGS = {
     "pj_agg_create": (C.c_int, [C.POINTER(AggDesc), C.POINTER(C.c_void_p)]),
     "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "pj_hash_buckets": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_tuple_hash_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_void_p, C.c_void_p]),
     "pj_join_count_by_key": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
                                        C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
     "pj_smj_create": (C.c_int, [C.POINTER(JoinDesc), C.POINTER(C.c_void_p)]),
@@ -170,6 +173,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libpjoin.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # torch ships its own HIP runtime under another file name. Loaded
+        # first, it is the runtime libpjoin binds to, so the process has one
+        # and torch's device memory and streams are valid in the library;
+        # loaded after libpjoin, a second runtime would come up beside it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in list(_SIGS.items()) + list(_OPTIONAL_SIGS.items()):
             if not hasattr(L, name):
@@ -519,3 +530,28 @@ class SortMergeJoin:
             self.close()
         except Exception:
             pass
+
+
+HASH_FNS = {"modulo": 0, "parammodulo": 1, "knuth": 2, "range": 3, "exactrange": 4, "bytes": 5,
+            "tpchorderkey": 6, "willis": 7, "alwayszero": 8, "tpchq1magic": 9}
+
+
+class HashSpec(C.Structure):
+    _fields_ = [("fn", C.c_uint32), ("buckets", C.c_uint32), ("range_min", C.c_int64), ("range_max", C.c_int64),
+                ("offset", C.c_uint32), ("skipbits", C.c_uint32)]
+
+
+def hash_spec(fn, buckets, range_min=0, range_max=0, offset=0, skipbits=0):
+    return HashSpec(HASH_FNS[fn], buckets, range_min, range_max, offset, skipbits)
+
+
+def hash_buckets(spec):
+    b = C.c_uint32()
+    check(lib().pj_hash_buckets(C.byref(spec), C.byref(b)))
+    return b.value
+
+
+def tuple_hash_device(spec, d_tuples, n, width, key_offset, key_size, d_buckets, stream=None):
+    """Bucket ids of n device tuples under a reference hash function."""
+    check(lib().pj_tuple_hash_device(C.byref(spec), C.c_void_p(d_tuples), n, width, key_offset, key_size,
+                                     C.c_void_p(d_buckets), C.c_void_p(stream)))

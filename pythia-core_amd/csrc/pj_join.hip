@@ -795,36 +795,57 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
             });
         const uint16_t* dir = reinterpret_cast<const uint16_t*>(dirw);
         const RadixTab tab{tkey, tpay, dir, bitsR, bkt_shift, I.part};
-        for (uint32_t b0 = 0; b0 < I.s_cnt; b0 += kJoinThreads * kGjUnroll) {
-            uint4 sr[kGjUnroll];
+        // Each wave probes a slice of kGjUnroll * 64 records a round; a
+        // record's register is refilled with the next round's record as soon
+        // as its lookup is done, so loads stay in flight while LDS work runs.
+        constexpr uint32_t kSlice = 64 * kGjUnroll, kRound = (kJoinThreads / 64) * kSlice;
+        const uint32_t lane = tid & 63, wave = tid >> 6;
+        uint4 nxt[kGjUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kGjUnroll; ++u) {
+            const uint32_t i = wave * kSlice + u * 64 + lane;
+            if (i < I.s_cnt)
+                nxt[u] = ld_stream(Sp + i);
+        }
+        for (uint32_t r0 = 0; r0 < I.s_cnt; r0 += kRound) {
+            const uint32_t t0 = r0 + wave * kSlice;
 #pragma unroll
             for (uint32_t u = 0; u < kGjUnroll; ++u) {
-                const uint32_t i = b0 + u * kJoinThreads + tid;
-                if (i < I.s_cnt)
-                    sr[u] = ld_stream(Sp + i);
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kGjUnroll; ++u) {
-                const uint32_t i = b0 + u * kJoinThreads + tid;
-                if (i >= I.s_cnt)
-                    break;
+                const uint32_t i = t0 + u * 64 + lane;
+                const uint4 s = nxt[u];
                 uint32_t nm = 0, first = 0;
-                tab.lookup(sr[u], nm, first);
-                const uint64_t mk = rec_key(sr[u]);
-                for (uint32_t j = first, c = 0; c < nm; ++j)   // the nm matches lie in first's bucket
-                    if (tkey[j] == mk) {
-                        atomicAdd(&tpay[j].y, 1u);
-                        ++c;
-                    }
+                if (i < I.s_cnt)
+                    tab.lookup(s, nm, first);
+                if (i + kRound < I.s_cnt)
+                    nxt[u] = ld_stream(Sp + i + kRound);
+                if (nm == 1) {
+                    atomicAdd(&tpay[first].y, 1u);
+                } else if (nm) {
+                    const uint64_t mk = rec_key(s);
+                    for (uint32_t j = first, c = 0; c < nm; ++j)   // the nm matches lie in first's bucket
+                        if (tkey[j] == mk) {
+                            atomicAdd(&tpay[j].y, 1u);
+                            ++c;
+                        }
+                }
             }
         }
         __syncthreads();
         const uint32_t ntab = dir[kTableBuckets - 1];
-        bool seen = plan.rst[I.part + 1] - plan.rst[I.part] > I.r_cnt;   // a split partition
+        const bool split = plan.rst[I.part + 1] - plan.rst[I.part] > I.r_cnt;
+        // An item that is its partition's only one (one table, one probe
+        // piece) holds the final counts and stores them; otherwise items add.
+        const uint32_t ps = plan_probe_part(plan, I.part);
+        const bool sole = !split && probe_pieces(plan.sst[ps + 1] - plan.sst[ps], plan.chunk) == 1;
+        bool seen = split;
         for (uint32_t j = tid; j < ntab; j += kJoinThreads) {
             const uint2 pv = tpay[j];
-            if (pv.y)
-                atomicAdd(&rcount[pv.x], pv.y);
+            if (pv.y) {
+                if (sole)
+                    rcount[pv.x] = pv.y;
+                else
+                    atomicAdd(&rcount[pv.x], pv.y);
+            }
             const uint64_t k = tkey[j];
             const uint32_t b = (uint32_t)(key_mix(k) >> bkt_shift) & (kTableBuckets - 1);
             for (uint32_t m = b ? dir[b - 1] : 0u; m < j; ++m)

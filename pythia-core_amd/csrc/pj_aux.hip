@@ -180,7 +180,7 @@ struct RangeFn {
             v = (int32_t)t[off_words];
         else
             v = (int64_t)((uint64_t)t[off_words] | ((uint64_t)t[off_words + 1] << 32));
-        return (uint32_t)((v - min) / bucketrange);
+        return (uint32_t)((int64_t)((uint64_t)v - (uint64_t)min) / bucketrange);   // wrapping difference
     }
 };
 
@@ -563,6 +563,9 @@ int pj_partition_exactrange_device(const pj_schema* schema, uint32_t attr, int64
         return fail(PJ_ERR_UNSUPPORTED, "device partitioner needs 4-byte multiples");
     if (range_max < range_min)
         return fail(PJ_ERR_INVALID, "empty range");
+    if ((uint64_t)range_max - (uint64_t)range_min > (uint64_t)INT64_MAX - buckets)
+        return fail(PJ_ERR_INVALID, "range [%lld, %lld] over %u buckets overflows a LONG", (long long)range_min,
+                    (long long)range_max, buckets);
     if (n >= (1ull << 32))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^32 tuples or more");
     RangeFn fn;

@@ -71,7 +71,8 @@ int hash_params(const pj_hash_spec* s, HashParams* p)
         return PJ_OK;
     case PJ_HASH_RANGE:
     case PJ_HASH_BYTES:        // the bit count itself
-        if (s->fn == PJ_HASH_RANGE && (s->range_max < s->range_min || s->range_max - s->range_min + 1 <= 0))
+        if (s->fn == PJ_HASH_RANGE && (s->range_max < s->range_min ||
+                                       (uint64_t)s->range_max - (uint64_t)s->range_min >= (uint64_t)INT64_MAX))
             return fail(PJ_ERR_INVALID, "range [%lld, %lld] is empty or spans every key", (long long)s->range_min,
                         (long long)s->range_max);
         p->k = lg;
@@ -83,6 +84,9 @@ int hash_params(const pj_hash_spec* s, HashParams* p)
         if (s->range_max < s->range_min)
             return fail(PJ_ERR_INVALID, "range [%lld, %lld] is empty", (long long)s->range_min,
                         (long long)s->range_max);
+        if ((uint64_t)s->range_max - (uint64_t)s->range_min > (uint64_t)INT64_MAX - s->buckets)
+            return fail(PJ_ERR_INVALID, "range [%lld, %lld] over %u buckets overflows a LONG",
+                        (long long)s->range_min, (long long)s->range_max, s->buckets);
         p->k = s->buckets;
         p->min = s->range_min;
         p->max = s->range_max;

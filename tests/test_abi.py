@@ -132,3 +132,19 @@ def test_sortmerge_join_validates_like_the_hash_join():
     j = P.SortMergeJoin(_desc())
     assert j.out_width == P.Join(_desc()).out_width == 24
     j.close()
+
+
+def test_exactrange_partition_arguments_are_checked_before_the_device():
+    """PartitionOp's exactrange bucket width (max - min + B) / B (hash.h:455-460)
+    is LONG arithmetic: spans that would overflow it are refused, as are
+    empty ranges and bucket counts PartitionOp::init refuses (partition.cpp:44-105)."""
+    sch = P.make_schema(["long", "long"])
+    sizes = (C.c_uint64 * 8)()
+    lo, hi = -(1 << 63), (1 << 63) - 1
+    for rmin, rmax, b in ((lo, hi, 8), (0, hi - 7, 8), (5, 4, 8), (0, 10, 0), (0, 10, 128)):
+        rc = P.lib().pj_partition_exactrange_device(C.byref(sch), 0, rmin, rmax, b, None, 0, None, sizes, None)
+        assert rc == P.PJ_ERR_INVALID, (rmin, rmax, b)
+    # the widest span that fits, on an empty relation: no device work, all sizes zero
+    sizes[0] = 99
+    assert P.lib().pj_partition_exactrange_device(C.byref(sch), 0, 0, hi - 8, 8, None, 0, None, sizes, None) == P.PJ_OK
+    assert list(sizes) == [0] * 8

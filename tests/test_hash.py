@@ -33,11 +33,21 @@ def test_bucket_counts_equal_the_oracle(fn, buckets, kw):
 
 
 def test_bad_specs_are_rejected():
+    lo, hi = -(1 << 63), (1 << 63) - 1
     for spec in (P.hash_spec("modulo", 0), P.hash_spec("range", 8, range_min=5, range_max=4),
-                 P.hash_spec("exactrange", 8, range_min=5, range_max=4), P.HashSpec(99, 8, 0, 0, 0, 0)):
+                 P.hash_spec("exactrange", 8, range_min=5, range_max=4), P.HashSpec(99, 8, 0, 0, 0, 0),
+                 # spans whose LONG arithmetic (max - min + 1, max - min + buckets) would overflow
+                 P.hash_spec("range", 8, range_min=lo, range_max=hi),
+                 P.hash_spec("range", 8, range_min=-1, range_max=hi),
+                 P.hash_spec("exactrange", 8, range_min=lo, range_max=hi),
+                 P.hash_spec("exactrange", 8, range_min=0, range_max=hi - 7)):
         with pytest.raises(P.PJError) as e:
             P.hash_buckets(spec)
         assert e.value.code == P.PJ_ERR_INVALID
+    # the widest spans that still fit
+    for fn, kw in (("range", dict(range_min=0, range_max=hi - 1)), ("exactrange", dict(range_min=0, range_max=hi - 8))):
+        h = _oracle_hasher(fn, 8, kw)
+        assert P.hash_buckets(P.hash_spec(fn, 8, **kw)) == O.lib().or_hasher_buckets(C.byref(h)) == 8
 
 
 def _keys(fn, kw, n, rng):

@@ -270,3 +270,28 @@ def test_batch_output_overflow_reported(mode):
         bt.results()
     assert e.value.code == P.PJ_ERR_CAPACITY
     bt.close()
+
+
+def test_full_batch_512_joins_tuned_graph():
+    """The full 512-join batch (build sizes log-uniform in [1K, 4M], seed
+    505, probe 4x build): tuned radix bits, the four streams captured into
+    one graph and replayed twice, then launched directly; every join of every
+    run against the generator's closed form (count and checksum)."""
+    _say("case: full batch of 512 joins")
+    seed = 505
+    sizes = [int(x) for x in P.batch_sizes(seed, 512, 1 << 10, 1 << 22)]
+    joins, keep = _make(seed, sizes)
+    _say("generated %d joins, %d build tuples" % (len(sizes), sum(sizes)))
+    bt = P.Batch(P.make_desc(**DESC))
+    bt.tune(joins, trial=4)
+    bt.prepare(joins)
+    for mode in (P.Batch.CONCURRENT_GRAPH, P.Batch.CONCURRENT_GRAPH, P.Batch.CONCURRENT):
+        for j in range(len(joins)):
+            keep[3 * j + 2].fill_(0xAB)
+        torch.cuda.synchronize()
+        bt.run(mode)
+        cnt, _ = bt.results()
+        _check(seed, joins, cnt)
+        _say("mode %d: 512 joins match, %d output tuples" % (mode, int(cnt.sum())))
+    bt.close()
+    del keep

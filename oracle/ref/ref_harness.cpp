@@ -101,7 +101,7 @@ int ref_hashjoin_run(int threads, int nbcols, const int* btypes, const int* bwid
                      const int* ptypes, const int* pwidths, int nkeys, const int* bjattr,
                      const int* pjattr, int nproj, const char* const* proj, const char* hashfn,
                      int buckets, int tuplesperbucket, int buffsize, int range_min, int range_max,
-                     const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe,
+                     int prefetch, const void* build, uint64_t nbuild, const void* probe, uint64_t nprobe,
                      void* out, uint64_t out_cap, uint64_t* out_n, double* t_build,
                      double* t_total)
 {
@@ -131,6 +131,10 @@ int ref_hashjoin_run(int threads, int nbcols, const int* btypes, const int* bwid
             g0.add(Setting::TypeInt) = i;
         jn.add("tuplesperbucket", Setting::TypeInt) = tuplesperbucket;
         jn.add("allocpolicy", Setting::TypeString) = "local";
+        if (prefetch >= 0) {   // HashJoinOp::init, join.cpp:386-404 (absent: no prefetch)
+            jn.add("probeprefetchstep", Setting::TypeInt) = prefetch;
+            jn.add("buildprefetchstep", Setting::TypeInt) = prefetch;
+        }
         if (nkeys == 1) {
             jn.add("buildjattr", Setting::TypeInt) = bjattr[0];
             jn.add("probejattr", Setting::TypeInt) = pjattr[0];

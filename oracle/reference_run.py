@@ -30,7 +30,7 @@ def lib():
         L.ref_hashjoin_run.argtypes = [
             C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
             C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
-            C.POINTER(C.c_char_p), C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+            C.POINTER(C.c_char_p), C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
             C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
             C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ref_tuple_hash.restype = C.c_uint
@@ -69,10 +69,11 @@ def _schema(cols):
 
 def run_join(build_cols, probe_cols, buildjattr, probejattr, projection, build, probe, threads=4,
              fn="modulo", buckets=1024, tuplesperbucket=2, buffsize=1 << 20, range_min=0, range_max=0,
-             out_cap=None):
+             out_cap=None, prefetch=-1):
     """Runs the reference HashJoinOp under MergeOp(threads). build/probe are
-    packed uint8 arrays. Returns (rows uint8 [n, w], build seconds, total
-    seconds)."""
+    packed uint8 arrays; prefetch >= 0 sets its probeprefetchstep and
+    buildprefetchstep (-1: none, the reference's default). Returns (rows
+    uint8 [n, w], build seconds, total seconds)."""
     bt, bw, bsz = _schema(build_cols)
     pt, pw, psz = _schema(probe_cols)
     ow = 0
@@ -92,7 +93,7 @@ def run_join(build_cols, probe_cols, buildjattr, probejattr, projection, build, 
         proj = (C.c_char_p * len(projection))(*[p.encode() for p in projection])
         rc = lib().ref_hashjoin_run(threads, len(bt), _ints(bt), _ints(bw), len(pt), _ints(pt), _ints(pw),
                                     len(buildjattr), _ints(buildjattr), _ints(probejattr), len(projection), proj,
-                                    fn.encode(), buckets, tuplesperbucket, buffsize, range_min, range_max,
+                                    fn.encode(), buckets, tuplesperbucket, buffsize, range_min, range_max, prefetch,
                                     build.ctypes.data_as(C.c_void_p) if build.size else None, nb,
                                     probe.ctypes.data_as(C.c_void_p) if probe.size else None, npr,
                                     out.ctypes.data_as(C.c_void_p), cap, C.byref(n), C.byref(tb), C.byref(tt))
@@ -104,7 +105,7 @@ def run_join(build_cols, probe_cols, buildjattr, probejattr, projection, build, 
         return out[: n.value * ow].reshape(-1, ow), tb.value, tt.value
 
 
-def run_pkfk(build, probe, threads):
+def run_pkfk(build, probe, threads, prefetch=-1):
     """The headline join (int64 key, int64 payload) x (int64 fk, int64
     payload), projection [B$0, B$1, P$1], modulo hash over next_pow2(|R|)
     buckets of 2 tuples. Returns seconds and the output count."""
@@ -112,7 +113,7 @@ def run_pkfk(build, probe, threads):
     buckets = 1 << max(1, (nb - 1).bit_length())
     rows, tb, tt = run_join(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
                             build.view(np.uint8), probe.view(np.uint8), threads=threads, fn="modulo",
-                            buckets=buckets, tuplesperbucket=2, buffsize=1 << 20)
+                            buckets=buckets, tuplesperbucket=2, buffsize=1 << 20, prefetch=prefetch)
     return {"seconds": tt, "build_seconds": tb, "out_tuples": rows.shape[0], "rows": rows}
 
 

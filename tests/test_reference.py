@@ -96,3 +96,14 @@ def test_reference_pkfk_matches_expected_checksum():
     s, c = O.expected_pkfk(77, 1 << 40, 50000, 200000, 20000)
     assert r["out_tuples"] == c
     assert O.checksum(r["rows"], 24)[0] == s
+
+
+def test_reference_prefetch_steps_change_no_rows():
+    # probeprefetchstep / buildprefetchstep (join.cpp:386-404) only prefetch:
+    # the CPU baseline sweeps them, so each step must give the same rows
+    b, p = O.gen_pkfk(11, 1 << 40, 1 << 12, 1 << 15, 1 << 12)
+    base = RR.run_pkfk(b, p, 4)["rows"]
+    for pf in (0, 8, 32):
+        rows = RR.run_pkfk(b, p, 4, prefetch=pf)["rows"]
+        assert np.array_equal(O.sorted_rows(rows), O.sorted_rows(base)), pf
+    assert base.shape[0] == (1 << 15) - (1 << 12)

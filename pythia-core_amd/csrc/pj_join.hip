@@ -325,11 +325,21 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         }
         __syncthreads();
     }
-    // partial buffers leave padded to a whole chunk
+    // partial buffers leave padded to a whole chunk. The buffer slot is read
+    // whatever c is (gl < B keeps it inside the buffer) and the padding record
+    // chosen word by word: a select between an LDS record and an argument
+    // record compiles to a flat load from either address, with the argument
+    // copied to scratch memory.
     for (uint32_t q = grp; q < pn; q += ngrp) {
         const uint32_t c = cnt[q];
-        if (c > 0)
-            out[goff[q] + gl] = gl < c ? buf[q * B + gl] : (p0 + q == 0 ? pad_zero : pad_other);
+        if (c > 0) {
+            const uint4 r = buf[q * B + gl];
+            const bool keep = gl < c, z = p0 + q == 0;
+            const uint4 pad = make_uint4(z ? pad_zero.x : pad_other.x, z ? pad_zero.y : pad_other.y,
+                                         z ? pad_zero.z : pad_other.z, z ? pad_zero.w : pad_other.w);
+            out[goff[q] + gl] = make_uint4(keep ? r.x : pad.x, keep ? r.y : pad.y, keep ? r.z : pad.z,
+                                           keep ? r.w : pad.w);
+        }
     }
 }
 
@@ -651,6 +661,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanView v, uint32_t* __r
 // hold 32-bit key codes; a dense item whose keys turn out not to be distinct
 // is rebuilt as a code table, or, when it is too large for one, counted in
 // *fail_ctr (the caller then runs the join on radix partitions).
+// Probe records in flight a thread in the range kernel, which holds both
+// table formats' probe loops: with the radix kernel's 8 it spills registers
+// to scratch inside the probe loop.
+constexpr int kRangeProbeItems = 6;
+
 template <bool RANGE>
 __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
                                                         PlanView plan, uint32_t* __restrict__ item_ctr,
@@ -729,7 +744,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
             }
             if (dense) {
                 const DenseTab tab{tpay, I.lo, I.hi, I.base, I.r_cnt};
-                probe_rounds<kJoinThreads>(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
+                probe_rounds<kJoinThreads, DenseTab, kRangeProbeItems>(tab, Sp, I.s_cnt, out, out_cap, out_cursor,
+                                                                       fmt, kbp, wsum, misc + 2);
             } else {
                 uint32_t* tcode = reinterpret_cast<uint32_t*>(tpay + kCodeCap);   // kCodeCap
                 uint32_t* dirw = tcode + kCodeCap;                                 // kTableBuckets / 2 + 1
@@ -742,7 +758,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
                         tpay[j] = make_uint2(r.z, r.w);
                     });
                 const CodeTab tab{tcode, tpay, reinterpret_cast<const uint16_t*>(dirw), I.lo, I.hi, I.base};
-                probe_rounds<kJoinThreads>(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
+                probe_rounds<kJoinThreads, CodeTab, kRangeProbeItems>(tab, Sp, I.s_cnt, out, out_cap, out_cursor,
+                                                                      fmt, kbp, wsum, misc + 2);
             }
         }
         __syncthreads();

@@ -63,13 +63,13 @@ __device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t r
 }
 
 // Streams the probe records [0, s_cnt) of Sp past a built table (NT
-// threads a block). The probe runs in rounds of NT * 8 records: every wave
-// probes a 512-record slice, the
+// threads a block). The probe runs in rounds of NT * PI records (PI = 8 by
+// default): every wave probes a 64 * PI-record slice, the
 // block reserves the round's output rows with one atomic, and the lanes of
 // each wave write adjacent output rows. Loads are software-pipelined: a
 // record's slot is refilled with the next round's record as soon as its
 // lookup is done.
-template <int NT, class T>
+template <int NT, class T, int PI = kProbeItems>
 __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restrict__ Sp, uint32_t s_cnt,
                                              uint32_t* __restrict__ out, uint64_t out_cap,
                                              unsigned long long* __restrict__ out_cursor, const OutFormat& fmt,
@@ -77,22 +77,22 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    constexpr uint32_t kSlice = 64 * kProbeItems;
+    constexpr uint32_t kSlice = 64 * PI;
     constexpr uint32_t kRound = (NT / 64) * kSlice;
-    uint4 nxt[kProbeItems];
+    uint4 nxt[PI];
 #pragma unroll
-    for (int k = 0; k < kProbeItems; ++k) {
+    for (int k = 0; k < PI; ++k) {
         const uint32_t i = wave * kSlice + k * 64 + lane;
         if (i < s_cnt)
             nxt[k] = ld_stream(Sp + i);
     }
     for (uint32_t r0 = 0; r0 < s_cnt; r0 += kRound) {
         const uint32_t t0 = r0 + wave * kSlice;
-        uint4 s[kProbeItems];
-        uint32_t nm[kProbeItems], first[kProbeItems];
+        uint4 s[PI];
+        uint32_t nm[PI], first[PI];
         bool multi = false;
 #pragma unroll
-        for (int k = 0; k < kProbeItems; ++k) {
+        for (int k = 0; k < PI; ++k) {
             const uint32_t i = t0 + k * 64 + lane;
             s[k] = nxt[k];
             nm[k] = 0;
@@ -104,17 +104,17 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
                 nxt[k] = ld_stream(Sp + i + kRound);
         }
         const bool wmulti = __any(multi);
-        uint64_t m[kProbeItems];
+        uint64_t m[PI];
         uint32_t mine = 0, pre = 0, wtotal = 0;
         if (!wmulti) {
 #pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
+            for (int k = 0; k < PI; ++k) {
                 m[k] = __ballot(nm[k] != 0);
                 wtotal += (uint32_t)__popcll(m[k]);
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < kProbeItems; ++k)
+            for (int k = 0; k < PI; ++k)
                 mine += nm[k];
             uint32_t x = mine;
 #pragma unroll
@@ -157,7 +157,7 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
             // many-to-many: a lane writes all matches of its probes
             uint64_t row = wbase + pre;
 #pragma unroll
-            for (int k = 0; k < kProbeItems; ++k) {
+            for (int k = 0; k < PI; ++k) {
                 const uint64_t mk = tab.mkey(s[k]);
                 uint32_t left = nm[k];
                 for (uint32_t j = first[k]; left; j = tab.next(j))
@@ -173,7 +173,7 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
         // at most one match a probe: lanes of a wave write adjacent rows
         uint64_t row0 = wbase;
 #pragma unroll
-        for (int k = 0; k < kProbeItems; ++k) {
+        for (int k = 0; k < PI; ++k) {
             if (nm[k]) {
                 const uint64_t row = row0 + (uint64_t)__popcll(m[k] & below);
                 if (row < out_cap)

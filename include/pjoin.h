@@ -70,6 +70,10 @@ typedef struct {
 } pj_join_desc;
 
 #define PJ_FLAG_TIMING 1u               /* record per-phase HIP event times */
+#define PJ_FLAG_NARROW_ROW_HASH 2u      /* schemas joined in the row-id form (keys other than one INTEGER /
+                                           LONG / DATE column or two INTEGER columns, wider projections):
+                                           keep 10 bits of the key hash, so that unequal keys share hashes
+                                           and the key check on the tuples decides (a test of that check) */
 
 typedef struct pj_join_s* pj_join_t;
 

@@ -7,6 +7,7 @@ runs on a machine without a GPU.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -21,7 +22,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 
-KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip", "pj_agg.hip", "pj_load.hip", "pj_smj.hip", "pj_hash.hip"]
+KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip", "pj_agg.hip", "pj_load.hip", "pj_smj.hip",
+                  "pj_hash.hip", "pj_rows.hip"]
 HOST_SOURCES = ["schema.cpp", "operators.cpp"]
 
 
@@ -37,22 +39,22 @@ def _run(cmd):
 
 def build(verbose=False):
     os.makedirs(LIB, exist_ok=True)
-    objs = []
+    jobs = []   # (command, object), compiled in parallel
     for s in KERNEL_SOURCES:
         src = os.path.join(CSRC, s)
-        if not os.path.exists(src):
-            continue
-        obj = os.path.join(LIB, s.replace(".hip", ".o"))
-        _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
-        objs.append(obj)
+        if os.path.exists(src):
+            obj = os.path.join(LIB, s.replace(".hip", ".o"))
+            jobs.append(([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj], obj))
     for s in HOST_SOURCES:
         src = os.path.join(HOST, s)
-        if not os.path.exists(src):
-            continue
-        obj = os.path.join(LIB, "host_" + s.replace(".cpp", ".o"))
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include"),
-              "-c", src, "-o", obj])
-        objs.append(obj)
+        if os.path.exists(src):
+            obj = os.path.join(LIB, "host_" + s.replace(".cpp", ".o"))
+            jobs.append((["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+                          "-c", src, "-o", obj], obj))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(lambda j: _run(j[0]), jobs))
+    objs = [o for _, o in jobs]
     out = os.path.join(LIB, "libpjoin.so")
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])

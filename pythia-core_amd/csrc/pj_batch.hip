@@ -938,6 +938,10 @@ int pj_batch_create(const pj_join_desc* desc, pj_batch_t* out)
     int rc = derive_layout(desc, &L);
     if (rc)
         return rc;
+    if (L.rowid)
+        return fail(PJ_ERR_UNSUPPORTED,
+                    "batched joins take one INTEGER / LONG / DATE key or two INTEGER keys, at most 8 payload bytes "
+                    "a side and output tuples of at most 32 bytes without CHAR columns");
     pj_batch_s* h = new (std::nothrow) pj_batch_s();
     if (!h)
         return fail(PJ_ERR_INVALID, "out of host memory");

@@ -38,11 +38,44 @@ __device__ __forceinline__ uint32_t radix_part(uint64_t h, uint32_t bits)
     return bits ? (uint32_t)(h >> (64 - bits)) : 0u;
 }
 
-template <bool IDENT>
+// Loads at any byte address (packed tuples put columns at any offset): one
+// 32-bit load when the address is 4-byte aligned, else four byte loads.
+__device__ __forceinline__ uint32_t ld32u(const uint8_t* p)
+{
+    if (((uintptr_t)p & 3u) == 0)
+        return *reinterpret_cast<const uint32_t*>(p);
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+__device__ __forceinline__ uint64_t ld64u(const uint8_t* p)
+{
+    return (uint64_t)ld32u(p) | ((uint64_t)ld32u(p + 4) << 32);
+}
+
+// Hash of a tuple's key columns, each normalised to int64 (an INTEGER
+// column sign-extended), so that equal keys of either width hash alike.
+__device__ __forceinline__ uint64_t row_key_hash(const uint8_t* t, const SideLayout& L)
+{
+    uint64_t h = 0x9e3779b97f4a7c15ull * (L.nk + 1);
+    for (uint32_t k = 0; k < L.nk; ++k) {
+        const uint8_t* p = t + L.koff[k];
+        const uint64_t v = L.kint[k] ? (uint64_t)(int64_t)(int32_t)ld32u(p) : ld64u(p);
+        h = pj_fin64(h ^ v) + 0xd6e8feb86659fd93ull;
+    }
+    return pj_fin64(h);
+}
+
+// Record modes of the partition passes: the tuple's own words (REC_WORDS),
+// the tuple is the record (REC_IDENT), or the row-id form (REC_HASH).
+enum { REC_WORDS = 0, REC_IDENT = 1, REC_HASH = 2 };
+
+template <int RM>
 __device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint64_t i, const SideLayout& L)
 {
-    if (IDENT) {
+    if (RM == REC_IDENT) {
         return ld_stream(reinterpret_cast<const uint4*>(in) + i);
+    } else if (RM == REC_HASH) {
+        const uint64_t h = row_key_hash(reinterpret_cast<const uint8_t*>(in) + i * L.tbytes, L) & L.hmask;
+        return make_uint4((uint32_t)h, (uint32_t)(h >> 32), (uint32_t)i, 0u);
     } else {
         const uint32_t* t = in + i * L.tw;
         uint4 r;

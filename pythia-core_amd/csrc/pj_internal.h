@@ -31,6 +31,14 @@ int fail(int code, const char* fmt, ...);
 // INTEGER/LONG keys compare as the reference's IntLongEqual does; two
 // INTEGER key columns are packed lo/hi), and up to two words of the
 // side's projected non-key columns.
+//
+// Schemas the record cannot hold (keys of two LONG columns or of three or
+// more columns, more than 8 payload bytes a side, CHAR or wide projections,
+// columns off 4-byte boundaries) take the row-id form instead: the record is
+// {h_lo, h_hi, row, 0}, h a 64-bit hash of the key columns normalised to
+// int64 and row the tuple's index in its relation. The join then pairs rows
+// whose hashes are equal, and the materialising pass checks every key column
+// on the tuples themselves and copies the projection (pj_rows.hip).
 struct SideLayout {
     uint32_t tw;   // tuple width in 32-bit words
     int32_t k0;    // word index of key_lo
@@ -38,6 +46,13 @@ struct SideLayout {
     int32_t p0;    // word index of pay0, or -1: zero
     int32_t p1;    // word index of pay1, or -1: zero
     uint32_t ident;  // 1 when the tuple is exactly the record (tw 4, 0,1,2,3)
+    uint32_t tbytes = 0;   // tuple width in bytes (both forms)
+    // row-id form (hashed = 1): the key columns
+    uint32_t hashed = 0;
+    uint32_t nk = 0;
+    uint16_t koff[PJ_MAX_KEYS] = {};   // byte offset of key column k
+    uint8_t kint[PJ_MAX_KEYS] = {};    // 1: INTEGER (4 bytes, sign-extended), 0: LONG or DATE
+    uint64_t hmask = ~0ull;            // bits of the key hash kept (all but under PJ_FLAG_NARROW_ROW_HASH)
 };
 
 // Output tuple assembly: word w of an output tuple is word src[w] of the
@@ -48,10 +63,33 @@ struct OutFormat {
     uint8_t src[8];
 };
 
+// The row-id form's output: key columns compared on the tuples (the
+// reference's conjunction of IntInt / IntLong / LongInt / LongLong
+// equalities, rawcompfns.cpp:48-180) and the projection copied byte for byte
+// (Schema::writeData of each projected column, schema.inl:47-93).
+struct RowKey {
+    uint16_t boff, poff;    // byte offsets in the build / probe tuple
+    uint8_t bint, pint;     // 1: INTEGER, 0: LONG or DATE
+};
+struct RowCopy {
+    uint8_t side;           // PJ_SIDE_BUILD or PJ_SIDE_PROBE
+    uint8_t chr;            // 1: a CHAR column, copied as strncpy copies it (zeros after the first NUL)
+    uint16_t src, dst, width;   // byte offset in the tuple, in the output row; bytes
+};
+struct RowSpec {
+    uint32_t nk, ncopy;
+    uint32_t bw, pw, ow;    // build, probe and output tuple widths in bytes
+    uint32_t words;         // 1: every width and offset is a multiple of 4 (copies by words)
+    RowKey key[PJ_MAX_KEYS];
+    RowCopy copy[PJ_MAX_COLS];
+};
+
 struct Layout {
     SideLayout build, probe;
     OutFormat out;
     uint32_t out_width;   // bytes
+    uint32_t rowid = 0;   // 1: the row-id form (out is then the (build row, probe row) pair)
+    RowSpec rows = {};
 };
 
 // Validates a descriptor and derives the layouts (see pj_join.hip).
@@ -170,6 +208,17 @@ uint32_t choose_build_bits(uint64_t nr, int32_t forced);
 // grow (which waits for the device) when a join needs more than any before.
 int join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void* d_probe, uint64_t np, void* d_out,
                  uint64_t out_cap, uint32_t* slot, hipStream_t st);
+
+// The row-id form's second pass (pj_rows.hip): pairs[0 .. min(*ncand,
+// pair_cap)) are (build row, probe row) pairs whose key hashes are equal.
+// Pair i whose key columns are equal becomes output row i; a pair whose keys
+// differ adds one to *nfail and leaves row i unwritten. compact = true
+// instead writes the rows whose keys are equal densely, reserving them on
+// *rows (the pass the host runs again when *nfail is not zero). Rows at or
+// past out_cap are counted, not written.
+int rows_materialize(const uint2* pairs, const unsigned long long* ncand, uint64_t pair_cap, const void* d_build,
+                     const void* d_probe, const RowSpec& spec, void* d_out, uint64_t out_cap,
+                     unsigned long long* rows, uint32_t* nfail, bool compact, hipStream_t st);
 
 // Device state of the learned partitioner (pj_learned.hip).
 struct LearnedCtx {

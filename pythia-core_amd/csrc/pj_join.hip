@@ -111,7 +111,7 @@ __device__ __forceinline__ RangeDir build_range_dir(const uint64_t* bnd, uint32_
 // it also lowers pmin[p] / raises pmax[p] to the partition's ordered keys;
 // with pid (range partitions) it stores every tuple's partition, so the
 // scatter need not search the boundaries again.
-template <bool IDENT, bool RANGE>
+template <int RM, bool RANGE>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ in, uint64_t n,
                                                         SideLayout L, PartFn pf, uint64_t chunk,
                                                         uint32_t nblk, uint32_t pad, uint32_t* __restrict__ hist,
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
             uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
             r[k] = make_uint4(0, 0, 0, 0);
             if (i < end)
-                r[k] = load_rec<IDENT>(in, i, L);
+                r[k] = load_rec<RM>(in, i, L);
         }
         uint32_t part[kHistItems];
         if (RANGE) {
@@ -221,7 +221,7 @@ constexpr int kWcThreads = 1024;
 // writes only partitions [h * FL, (h + 1) * FL). On MI355X a 2048-way scatter
 // of 256M 16-byte records takes 2.5 ms with 64-byte chunks and 1.8-1.9 ms
 // split with 128-byte chunks.
-template <bool IDENT, int B, bool RANGE, int SPLIT>
+template <int RM, int B, bool RANGE, int SPLIT>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
                                                             SideLayout L, PartFn pf, uint64_t chunk,
                                                             uint32_t nblk, const uint32_t* __restrict__ offs,
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     for (int k = 0; k < IT; ++k) {
         const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
         if (i < end) {
-            nxt[k] = load_rec<IDENT>(in, i, L);
+            nxt[k] = load_rec<RM>(in, i, L);
             if (have_pid)
                 nxp[k] = pid[i];
         }
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             }
             const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
             if (i2 < end) {
-                nxt[k] = load_rec<IDENT>(in, i2, L);
+                nxt[k] = load_rec<RM>(in, i2, L);
                 if (have_pid)
                     nxp[k] = pid[i2];
             }
@@ -350,7 +350,7 @@ constexpr int kScatterItems = 16;
 // written so that lanes of a wave store consecutive records of a partition.
 // Range partitions keep each tile record's partition in LDS (tpart) and take
 // it from the histogram pass (pid) when given.
-template <bool IDENT, bool RANGE>
+template <int RM, bool RANGE>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
                                                               SideLayout L, PartFn pf, uint64_t chunk,
                                                               uint32_t nblk, const uint32_t* __restrict__ offs,
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
             if (i < tn) {
-                r[k] = load_rec<IDENT>(in, t0 + i, L);
+                r[k] = load_rec<RM>(in, t0 + i, L);
                 if (RANGE && pid)
                     part[k] = pid[t0 + i];
             }
@@ -615,8 +615,8 @@ constexpr uint32_t kPlanThreads = 1024;
 constexpr uint32_t kPlanPer = 16;   // build partitions a thread counts: up to 16384
 
 // One block: offs[p] = first item of build partition p, offs[FR] = items;
-// also clears the join's counters and records the item count and the number
-// of split tables in them (words 6 and 7).
+// also clears the join's counters (words 0-5 and 8-11) and records the item
+// count and the number of split tables in them (words 6 and 7).
 __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanView v, uint32_t* __restrict__ offs,
                                                         uint32_t* __restrict__ counters)
 {
@@ -647,7 +647,7 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanView v, uint32_t* __r
     block_excl_scan<kPlanThreads>(spilled, wsum, &stotal);
     if (threadIdx.x == 0) {
         offs[v.FR] = total;
-        for (int w = 0; w < 6; ++w)
+        for (int w = 0; w < 12; ++w)
             counters[w] = 0;
         counters[6] = total;
         counters[7] = stotal;
@@ -1010,31 +1010,14 @@ int schema_offsets(const pj_schema& s, int* off, int* size)
 
 static bool int_key(uint32_t t) { return t == PJ_CT_INTEGER || t == PJ_CT_LONG || t == PJ_CT_DATE; }
 
-// Mirrors the checks and schema construction of JoinOp::init and
-// HashJoinOp::init (operators/join.cpp:224-326, 367-461), then maps the
-// key, payload and output columns onto 32-bit words.
-int derive_layout(const pj_join_desc* d, Layout* L)
+// The 16-byte record form: one INTEGER/LONG/DATE key column a side or two
+// INTEGER columns, at most 8 payload bytes a side, output tuples of at most
+// 32 bytes without CHAR columns, everything on 4-byte boundaries. Returns
+// false when the descriptor needs the row-id form.
+static bool record_layout(const pj_join_desc* d, const int* boff, const int* poff, int bsize, int psize, Layout* L)
 {
-    int boff[PJ_MAX_COLS], poff[PJ_MAX_COLS], bsize, psize;
-    int rc;
-    if ((rc = schema_offsets(d->build, boff, &bsize)) || (rc = schema_offsets(d->probe, poff, &psize)))
-        return rc;
-    if (d->nkeys == 0 || d->nkeys > PJ_MAX_KEYS)
-        return fail(PJ_ERR_INVALID, "buildjattr/probejattr must name 1..%d columns", PJ_MAX_KEYS);
-    for (uint32_t k = 0; k < d->nkeys; ++k)
-        if (d->buildjattr[k] >= d->build.ncols || d->probejattr[k] >= d->probe.ncols)
-            return fail(PJ_ERR_INVALID, "join attribute %u out of range", k);
-    if (d->nproj == 0 || d->nproj > PJ_MAX_COLS)
-        return fail(PJ_ERR_INVALID, "projection must have 1..%d entries", PJ_MAX_COLS);
-    for (uint32_t j = 0; j < d->nproj; ++j) {
-        const pj_schema& s = d->proj_side[j] == PJ_SIDE_BUILD ? d->build : d->probe;
-        if (d->proj_side[j] > PJ_SIDE_PROBE || d->proj_attr[j] >= s.ncols)
-            return fail(PJ_ERR_INVALID, "projection entry %u out of range", j);
-    }
     if (bsize % 4 || psize % 4)
-        return fail(PJ_ERR_UNSUPPORTED, "device path needs tuple widths that are multiples of 4 bytes");
-
-    // keys: one INTEGER/LONG/DATE column a side, or two INTEGER columns
+        return false;
     SideLayout* sl[2] = {&L->build, &L->probe};
     const pj_schema* sc[2] = {&d->build, &d->probe};
     const int* so[2] = {boff, poff};
@@ -1042,34 +1025,26 @@ int derive_layout(const pj_join_desc* d, Layout* L)
     for (int side = 0; side < 2; ++side) {
         SideLayout& s = *sl[side];
         s.tw = (uint32_t)((side ? psize : bsize) / 4);
+        s.tbytes = (uint32_t)(side ? psize : bsize);
         s.p0 = s.p1 = -1;
         if (d->nkeys == 1) {
-            uint32_t c = ja[side][0];
-            if (!int_key(sc[side]->type[c]))
-                return fail(PJ_ERR_UNSUPPORTED, "device path joins INTEGER, LONG or DATE keys");
+            const uint32_t c = ja[side][0];
             if (so[side][c] % 4)
-                return fail(PJ_ERR_UNSUPPORTED, "key column not 4-byte aligned");
+                return false;
             s.k0 = so[side][c] / 4;
             s.k1 = sc[side]->type[c] == PJ_CT_INTEGER ? -1 : s.k0 + 1;
         } else if (d->nkeys == 2) {
-            uint32_t a = ja[side][0], b = ja[side][1];
+            const uint32_t a = ja[side][0], b = ja[side][1];
             if (sc[side]->type[a] != PJ_CT_INTEGER || sc[side]->type[b] != PJ_CT_INTEGER)
-                return fail(PJ_ERR_UNSUPPORTED, "device path joins composite keys of two INTEGER columns");
+                return false;
             if (so[side][a] % 4 || so[side][b] % 4)
-                return fail(PJ_ERR_UNSUPPORTED, "key column not 4-byte aligned");
+                return false;
             s.k0 = so[side][a] / 4;
             s.k1 = so[side][b] / 4;
         } else {
-            return fail(PJ_ERR_UNSUPPORTED, "device path joins on at most two key columns");
+            return false;
         }
     }
-    if (d->nkeys == 1) {
-        bool bl = d->build.type[d->buildjattr[0]] == PJ_CT_DATE, pl = d->probe.type[d->probejattr[0]] == PJ_CT_DATE;
-        if (bl != pl)
-            return fail(PJ_ERR_UNSUPPORTED, "DATE keys join only DATE keys");
-    }
-
-    // output words
     OutFormat& f = L->out;
     memset(&f, 0, sizeof(f));
     uint32_t words = 0;
@@ -1079,10 +1054,10 @@ int derive_layout(const pj_join_desc* d, Layout* L)
         const uint32_t c = d->proj_attr[j];
         const int w = col_width(s, c);
         if (s.type[c] == PJ_CT_CHAR)
-            return fail(PJ_ERR_UNSUPPORTED, "device path does not project CHAR columns");
+            return false;
         const int nw = w / 4;
         if (words + nw > 8)
-            return fail(PJ_ERR_UNSUPPORTED, "device path emits output tuples of at most 32 bytes");
+            return false;
         int keyidx = -1;
         for (uint32_t k = 0; k < d->nkeys; ++k)
             if (ja[side][k] == c)
@@ -1101,7 +1076,7 @@ int derive_layout(const pj_join_desc* d, Layout* L)
         // a non-key column travels in the record's payload words
         SideLayout& sdl = *sl[side];
         if (so[side][c] % 4)
-            return fail(PJ_ERR_UNSUPPORTED, "payload column not 4-byte aligned");
+            return false;
         for (int q = 0; q < nw; ++q) {
             const int32_t word = so[side][c] / 4 + q;
             int slot;
@@ -1110,7 +1085,7 @@ int derive_layout(const pj_join_desc* d, Layout* L)
             else if (sdl.p0 < 0) { sdl.p0 = word; slot = 0; }
             else if (sdl.p1 < 0) { sdl.p1 = word; slot = 1; }
             else
-                return fail(PJ_ERR_UNSUPPORTED, "device path carries at most 8 payload bytes a side");
+                return false;
             f.src[words++] = (uint8_t)(pbase + slot);
         }
     }
@@ -1120,7 +1095,111 @@ int derive_layout(const pj_join_desc* d, Layout* L)
         SideLayout& s = *sl[side];
         s.ident = (s.tw == 4 && s.k0 == 0 && s.k1 == 1 && s.p0 == 2 && s.p1 == 3) ? 1u : 0u;
     }
+    return true;
+}
+
+// The row-id form: records {key hash, row}, the join's pairs checked and
+// projected on the tuples by the materialising pass.
+static int rowid_layout(const pj_join_desc* d, const int* boff, const int* poff, int bsize, int psize, Layout* L)
+{
+    if (bsize > 65535 || psize > 65535)
+        return fail(PJ_ERR_UNSUPPORTED, "device path takes tuples of at most 65535 bytes");
+    const pj_schema* sc[2] = {&d->build, &d->probe};
+    const int* so[2] = {boff, poff};
+    const uint16_t* ja[2] = {d->buildjattr, d->probejattr};
+    SideLayout* sl[2] = {&L->build, &L->probe};
+    for (int side = 0; side < 2; ++side) {
+        SideLayout& s = *sl[side];
+        s = SideLayout();
+        s.tw = 0;
+        s.k0 = s.k1 = s.p0 = s.p1 = -1;
+        s.ident = 0;
+        s.hashed = 1;
+        s.tbytes = (uint32_t)(side ? psize : bsize);
+        s.nk = d->nkeys;
+        s.hmask = (d->flags & PJ_FLAG_NARROW_ROW_HASH) ? 0x3FFull : ~0ull;
+        for (uint32_t k = 0; k < d->nkeys; ++k) {
+            s.koff[k] = (uint16_t)so[side][ja[side][k]];
+            s.kint[k] = sc[side]->type[ja[side][k]] == PJ_CT_INTEGER ? 1 : 0;
+        }
+    }
+    // the join's rows are (build row, probe row) pairs
+    memset(&L->out, 0, sizeof(L->out));
+    L->out.words = 2;
+    L->out.src[0] = SRC_BP0;
+    L->out.src[1] = SRC_PP0;
+    RowSpec& r = L->rows;
+    r = RowSpec();
+    r.nk = d->nkeys;
+    r.bw = (uint32_t)bsize;
+    r.pw = (uint32_t)psize;
+    bool words = bsize % 4 == 0 && psize % 4 == 0;
+    for (uint32_t k = 0; k < d->nkeys; ++k) {
+        r.key[k].boff = L->build.koff[k];
+        r.key[k].poff = L->probe.koff[k];
+        r.key[k].bint = L->build.kint[k];
+        r.key[k].pint = L->probe.kint[k];
+    }
+    uint32_t dst = 0;
+    for (uint32_t j = 0; j < d->nproj; ++j) {
+        const int side = d->proj_side[j];
+        const uint32_t c = d->proj_attr[j];
+        const int w = col_width(*sc[side], c);
+        RowCopy& q = r.copy[j];
+        q.side = (uint8_t)side;
+        q.chr = sc[side]->type[c] == PJ_CT_CHAR ? 1 : 0;
+        q.src = (uint16_t)so[side][c];
+        q.dst = (uint16_t)dst;
+        q.width = (uint16_t)w;
+        words = words && q.src % 4 == 0 && w % 4 == 0;
+        dst += (uint32_t)w;
+    }
+    if (dst > 65535)
+        return fail(PJ_ERR_UNSUPPORTED, "device path emits output tuples of at most 65535 bytes");
+    r.ncopy = d->nproj;
+    r.ow = dst;
+    r.words = words && dst % 4 == 0 ? 1u : 0u;
+    L->out_width = dst;
+    L->rowid = 1;
     return PJ_OK;
+}
+
+// Mirrors the checks and schema construction of JoinOp::init and
+// HashJoinOp::init (operators/join.cpp:224-326, 367-461): the key columns
+// compare as the reference's ConjunctionEqualsEvaluator does
+// (conjunctionevaluator.cpp:83-88 over rawcompfns.cpp:48-180, INTEGER and
+// LONG by value, DATE only with DATE), then picks the record form or the
+// row-id form.
+int derive_layout(const pj_join_desc* d, Layout* L)
+{
+    int boff[PJ_MAX_COLS], poff[PJ_MAX_COLS], bsize, psize;
+    int rc;
+    if ((rc = schema_offsets(d->build, boff, &bsize)) || (rc = schema_offsets(d->probe, poff, &psize)))
+        return rc;
+    if (d->nkeys == 0 || d->nkeys > PJ_MAX_KEYS)
+        return fail(PJ_ERR_INVALID, "buildjattr/probejattr must name 1..%d columns", PJ_MAX_KEYS);
+    for (uint32_t k = 0; k < d->nkeys; ++k)
+        if (d->buildjattr[k] >= d->build.ncols || d->probejattr[k] >= d->probe.ncols)
+            return fail(PJ_ERR_INVALID, "join attribute %u out of range", k);
+    if (d->nproj == 0 || d->nproj > PJ_MAX_COLS)
+        return fail(PJ_ERR_INVALID, "projection must have 1..%d entries", PJ_MAX_COLS);
+    for (uint32_t j = 0; j < d->nproj; ++j) {
+        const pj_schema& s = d->proj_side[j] == PJ_SIDE_BUILD ? d->build : d->probe;
+        if (d->proj_side[j] > PJ_SIDE_PROBE || d->proj_attr[j] >= s.ncols)
+            return fail(PJ_ERR_INVALID, "projection entry %u out of range", j);
+    }
+    for (uint32_t k = 0; k < d->nkeys; ++k) {
+        const uint32_t bt = d->build.type[d->buildjattr[k]], pt = d->probe.type[d->probejattr[k]];
+        if (!int_key(bt) || !int_key(pt))
+            return fail(PJ_ERR_UNSUPPORTED, "device path joins INTEGER, LONG or DATE keys");
+        if ((bt == PJ_CT_DATE) != (pt == PJ_CT_DATE))
+            return fail(PJ_ERR_UNSUPPORTED, "DATE keys join only DATE keys");
+    }
+    *L = Layout();
+    if (record_layout(d, boff, poff, bsize, psize, L))
+        return PJ_OK;
+    *L = Layout();
+    return rowid_layout(d, boff, poff, bsize, psize, L);
 }
 
 int DevBuf::ensure(size_t bytes)
@@ -1198,6 +1277,9 @@ struct pj_join_s {
     bool ev_ok = false;
     uint32_t* host_ctr = nullptr;   // pinned: the join's counters, read back after each probe
     DevBuf gj_count, gj_rows;       // group-join: matches a build record, merged rows
+    DevBuf pairs;                   // row-id form: the join's (build row, probe row) pairs
+    uint64_t pair_cap = 0;          // pairs the last probe phase could hold
+    uint64_t pair_want = 0;         // pairs a previous run needed
 };
 
 constexpr uint32_t kMaxBits = 14;
@@ -1291,8 +1373,21 @@ int allow_lds(const void* fn, size_t bytes)
     return PJ_OK;
 }
 
+template <int RM, int B, bool RANGE, int SPLIT>
+int launch_wc_rm(const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+                 uint32_t nblk, const uint32_t* offs, uint4* out, uint4 pad_other, uint4 pad_zero, const uint16_t* pid,
+                 size_t lds, hipStream_t st)
+{
+    if (int rc = allow_lds((const void*)k_scatter_wc<RM, B, RANGE, SPLIT>, lds))
+        return rc;
+    hipLaunchKernelGGL((k_scatter_wc<RM, B, RANGE, SPLIT>), dim3(nblk * SPLIT), dim3(kWcThreads), lds, st, in, n, L,
+                       pf, chunk, nblk, offs, out, pad_other, pad_zero, pid);
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
+}
+
 template <int B, bool RANGE, int SPLIT>
-int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
                 uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
 {
     const uint32_t FL = SPLIT == 1 ? pf.F : (pf.F + 1) / 2;
@@ -1309,36 +1404,43 @@ int launch_wc_t(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
         pad_other = make_uint4(0, 0, 0, 0);
         pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
     }
-    const dim3 grid(nblk * SPLIT);
-    if (ident) {
-        if (int rc = allow_lds((const void*)k_scatter_wc<true, B, RANGE, SPLIT>, lds))
-            return rc;
-        hipLaunchKernelGGL((k_scatter_wc<true, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
-                           chunk, nblk, offs, out, pad_other, pad_zero, pid);
-    } else {
-        if (int rc = allow_lds((const void*)k_scatter_wc<false, B, RANGE, SPLIT>, lds))
-            return rc;
-        hipLaunchKernelGGL((k_scatter_wc<false, B, RANGE, SPLIT>), grid, dim3(kWcThreads), lds, st, in, n, L, pf,
-                           chunk, nblk, offs, out, pad_other, pad_zero, pid);
-    }
-    PJ_HIP(hipGetLastError());
-    return PJ_OK;
+    if (rm == REC_IDENT)
+        return launch_wc_rm<REC_IDENT, B, RANGE, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
+                                                        lds, st);
+    if (RANGE || rm == REC_WORDS)
+        return launch_wc_rm<REC_WORDS, B, RANGE, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
+                                                        lds, st);
+    return launch_wc_rm<REC_HASH, B, false, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
+                                                   lds, st);
 }
 
 template <int B>
-int launch_wc(bool ident, bool range, uint32_t split, const uint32_t* in, uint64_t n, const SideLayout& L,
+int launch_wc(int rm, bool range, uint32_t split, const uint32_t* in, uint64_t n, const SideLayout& L,
               const PartFn& pf, uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
               hipStream_t st)
 {
     if (split > 1)
-        return range ? launch_wc_t<B, true, 2>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
-                     : launch_wc_t<B, false, 2>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
-    return range ? launch_wc_t<B, true, 1>(ident, in, n, L, pf, chunk, nblk, offs, out, pid, st)
-                 : launch_wc_t<B, false, 1>(ident, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+        return range ? launch_wc_t<B, true, 2>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, st)
+                     : launch_wc_t<B, false, 2>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+    return range ? launch_wc_t<B, true, 1>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, st)
+                 : launch_wc_t<B, false, 1>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+}
+
+template <int RM, bool RANGE>
+int launch_hist_rm(const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+                   uint32_t nblk, uint32_t pad, uint32_t* hist, unsigned long long* pmin, unsigned long long* pmax,
+                   uint16_t* pid, size_t lds, hipStream_t st)
+{
+    if (int rc = allow_lds((const void*)k_hist<RM, RANGE>, lds))
+        return rc;
+    hipLaunchKernelGGL((k_hist<RM, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk, nblk, pad,
+                       hist, pmin, pmax, pid);
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
 }
 
 template <bool RANGE>
-int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+int launch_hist(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
                 uint32_t nblk, uint32_t pad, uint32_t* hist, unsigned long long* pmin, unsigned long long* pmax,
                 uint16_t* pid, hipStream_t st)
 {
@@ -1346,21 +1448,28 @@ int launch_hist(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L,
                        (RANGE ? ((1u << kDirBits) + 1) * 4 : 0);
     if (lds > kLdsBytes)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the histogram's LDS budget", pf.F);
-    const void* fn = ident ? (const void*)k_hist<true, RANGE> : (const void*)k_hist<false, RANGE>;
-    if (int rc = allow_lds(fn, lds))
+    if (rm == REC_IDENT)
+        return launch_hist_rm<REC_IDENT, RANGE>(in, n, L, pf, chunk, nblk, pad, hist, pmin, pmax, pid, lds, st);
+    if (RANGE || rm == REC_WORDS)
+        return launch_hist_rm<REC_WORDS, RANGE>(in, n, L, pf, chunk, nblk, pad, hist, pmin, pmax, pid, lds, st);
+    return launch_hist_rm<REC_HASH, false>(in, n, L, pf, chunk, nblk, pad, hist, pmin, pmax, pid, lds, st);
+}
+
+template <int RM, bool RANGE>
+int launch_tile_rm(const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
+                   uint32_t nblk, const uint32_t* offs, uint4* out, uint32_t ts, const uint16_t* pid, size_t lds,
+                   hipStream_t st)
+{
+    if (int rc = allow_lds((const void*)k_scatter<RM, RANGE>, lds))
         return rc;
-    if (ident)
-        hipLaunchKernelGGL((k_hist<true, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk, nblk,
-                           pad, hist, pmin, pmax, pid);
-    else
-        hipLaunchKernelGGL((k_hist<false, RANGE>), dim3(nblk), dim3(kHistThreads), lds, st, in, n, L, pf, chunk,
-                           nblk, pad, hist, pmin, pmax, pid);
+    hipLaunchKernelGGL((k_scatter<RM, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf, chunk, nblk,
+                       offs, out, ts, pid);
     PJ_HIP(hipGetLastError());
     return PJ_OK;
 }
 
 template <bool RANGE>
-int launch_tile_scatter(bool ident, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
+int launch_tile_scatter(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf,
                         uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
                         hipStream_t st)
 {
@@ -1369,17 +1478,11 @@ int launch_tile_scatter(bool ident, const uint32_t* in, uint64_t n, const SideLa
     if (ts == 0)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the scatter's LDS budget", pf.F);
     const size_t lds = (size_t)ts * (RANGE ? 18 : 16) + bb + (2 * pf.F + 1) * 4 + 16 * 4;
-    const void* fn = ident ? (const void*)k_scatter<true, RANGE> : (const void*)k_scatter<false, RANGE>;
-    if (int rc = allow_lds(fn, lds))
-        return rc;
-    if (ident)
-        hipLaunchKernelGGL((k_scatter<true, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf, chunk,
-                           nblk, offs, out, ts, pid);
-    else
-        hipLaunchKernelGGL((k_scatter<false, RANGE>), dim3(nblk), dim3(kScatterThreads), lds, st, in, n, L, pf,
-                           chunk, nblk, offs, out, ts, pid);
-    PJ_HIP(hipGetLastError());
-    return PJ_OK;
+    if (rm == REC_IDENT)
+        return launch_tile_rm<REC_IDENT, RANGE>(in, n, L, pf, chunk, nblk, offs, out, ts, pid, lds, st);
+    if (RANGE || rm == REC_WORDS)
+        return launch_tile_rm<REC_WORDS, RANGE>(in, n, L, pf, chunk, nblk, offs, out, ts, pid, lds, st);
+    return launch_tile_rm<REC_HASH, false>(in, n, L, pf, chunk, nblk, offs, out, ts, pid, lds, st);
 }
 
 // Blocks of the partition pass: about 512K tuples each, at least one per CU.
@@ -1432,7 +1535,9 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
     if ((rc = dst.ensure(cap * 16)) || (rc = histb.ensure(m * 4)) || (rc = bsumb.ensure((nsb + 1) * 4)))
         return rc;
     const uint32_t* in = (const uint32_t*)d_in;
-    const bool ident = L.ident && ((uintptr_t)d_in % 16 == 0);
+    if (L.hashed && range)
+        return fail(PJ_ERR_UNSUPPORTED, "row-id records take radix partitions only");
+    const int rm = L.hashed ? REC_HASH : (L.ident && ((uintptr_t)d_in % 16 == 0)) ? REC_IDENT : REC_WORDS;
     uint32_t* hist = (uint32_t*)histb.p;
     uint32_t* bsum = (uint32_t*)bsumb.p;
     unsigned long long* pmin = pminmax;
@@ -1448,8 +1553,8 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
             return rc;
         pid = (uint16_t*)pidsb.p;
     }
-    rc = range ? launch_hist<true>(ident, in, n, L, pf, chunk, nblk, B, hist, pmin, pmax, pid, st)
-               : launch_hist<false>(ident, in, n, L, pf, chunk, nblk, B, hist, nullptr, nullptr, nullptr, st);
+    rc = range ? launch_hist<true>(rm, in, n, L, pf, chunk, nblk, B, hist, pmin, pmax, pid, st)
+               : launch_hist<false>(rm, in, n, L, pf, chunk, nblk, B, hist, nullptr, nullptr, nullptr, st);
     if (rc)
         return rc;
     if (m <= kScanSmall) {   // one launch for small histograms
@@ -1465,13 +1570,13 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
     PJ_HIP(hipGetLastError());
     uint4* out = (uint4*)dst.p;
     switch (B) {
-    case 16: rc = launch_wc<16>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 8: rc = launch_wc<8>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 4: rc = launch_wc<4>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
-    case 2: rc = launch_wc<2>(ident, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 16: rc = launch_wc<16>(rm, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 8: rc = launch_wc<8>(rm, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 4: rc = launch_wc<4>(rm, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
+    case 2: rc = launch_wc<2>(rm, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
     default:
-        rc = range ? launch_tile_scatter<true>(ident, in, n, L, pf, chunk, nblk, hist, out, pid, st)
-                   : launch_tile_scatter<false>(ident, in, n, L, pf, chunk, nblk, hist, out, nullptr, st);
+        rc = range ? launch_tile_scatter<true>(rm, in, n, L, pf, chunk, nblk, hist, out, pid, st)
+                   : launch_tile_scatter<false>(rm, in, n, L, pf, chunk, nblk, hist, out, nullptr, st);
     }
     if (rc)
         return rc;
@@ -1624,6 +1729,8 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
 // build than probe partitions).
 bool want_learned(const pj_join_s* h, uint64_t nb)
 {
+    if (h->lay.rowid)   // hashed keys have no order to cut into ranges
+        return false;
     if (h->desc.partitioner == PJ_PART_LEARNED)
         return true;
     if (h->desc.partitioner != PJ_PART_AUTO || h->desc.radix_bits >= 0)
@@ -1652,8 +1759,10 @@ int prepare_build(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
 // Enqueues one probe phase on st: partition S, count the work items, join,
 // and copy the join's counters into slot (8 words of pinned host memory:
 // [2..3] rows produced, [4] range tables that could not be held, [6] work
-// items, [7] split tables) when the stream reaches that copy. *empty: the
-// join has no work (the slot is zeroed at once).
+// items, [7] split tables; the row-id form copies 12 words, the join's
+// pairs then in [2..3] and the materialising pass's key mismatches in
+// [10]) when the stream reaches that copy. *empty: the join has no work
+// (the slot is zeroed at once).
 int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint64_t out_cap, uint32_t* slot,
                   bool* empty, hipStream_t st)
 {
@@ -1674,7 +1783,7 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     record(h, 3, st);
     if (h->R.n == 0 || np == 0) {
         *empty = true;
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < (h->lay.rowid ? 12 : 8); ++i)
             slot[i] = 0;
         return PJ_OK;
     }
@@ -1702,19 +1811,34 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     if ((rc = allow_lds(kfn, lds)))
         return rc;
     const uint32_t bkt_shift = 64 - h->bR - 12;
+    // the row-id form's join writes (build row, probe row) pairs, checked and
+    // projected afterwards
+    void* jout = d_out;
+    uint64_t jcap = out_cap;
+    if (h->lay.rowid) {
+        const uint64_t want = std::max<uint64_t>(out_cap + out_cap / 64 + 1024, h->pair_want);
+        if ((rc = h->pairs.ensure(want * 8)))
+            return rc;
+        jout = h->pairs.p;
+        jcap = h->pair_cap = h->pairs.cap / 8;
+    }
     record(h, 4, st);
     const uint32_t grid = (uint32_t)device_count_cus();   // blocks without an item exit at once
     if (h->range)
         hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)d_out, out_cap, cursor, h->lay.out, 0u,
+                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, 0u,
                            0u, fail_ctr);
     else
         hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)d_out, out_cap, cursor, h->lay.out, h->bR,
+                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, h->bR,
                            bkt_shift, fail_ctr);
     PJ_HIP(hipGetLastError());
+    if (h->lay.rowid &&
+        (rc = rows_materialize((const uint2*)h->pairs.p, cursor, jcap, h->raw_build, d_probe, h->lay.rows, d_out,
+                               out_cap, (unsigned long long*)(counters + 8), counters + 10, false, st)))
+        return rc;
     record(h, 5, st);
-    PJ_HIP(hipMemcpyAsync(slot, counters, 32, hipMemcpyDeviceToHost, st));
+    PJ_HIP(hipMemcpyAsync(slot, counters, h->lay.rowid ? 48 : 32, hipMemcpyDeviceToHost, st));
     return PJ_OK;
 }
 
@@ -1737,6 +1861,25 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     }
     *out_n = (uint64_t)produced[2] | ((uint64_t)produced[3] << 32);
     *fails = produced[4];
+    if (h->lay.rowid) {
+        if (*out_n > h->pair_cap) {   // pairs were dropped: the caller runs the probe again with room for them
+            h->pair_want = *out_n;
+            return PJ_ERR_CAPACITY;
+        }
+        if (produced[10]) {
+            // some pairs had equal hashes and unequal keys: write the rows
+            // densely instead (the rows counter, words 8-9, is still zero)
+            uint32_t* counters = (uint32_t*)h->counters.p;
+            int rc2 = rows_materialize((const uint2*)h->pairs.p, (const unsigned long long*)(counters + 2),
+                                       h->pair_cap, h->raw_build, d_probe, h->lay.rows, d_out, out_cap,
+                                       (unsigned long long*)(counters + 8), counters + 10, true, st);
+            if (rc2)
+                return rc2;
+            PJ_HIP(hipMemcpyAsync(produced + 8, counters + 8, 8, hipMemcpyDeviceToHost, st));
+            PJ_HIP(hipStreamSynchronize(st));
+            *out_n = (uint64_t)produced[8] | ((uint64_t)produced[9] << 32);
+        }
+    }
     pj_timings& t = h->last;
     t.build_partition_ms = elapsed(h, 0, 1) - h->model_ms;
     t.probe_partition_ms = elapsed(h, 2, 3);
@@ -1759,6 +1902,8 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
 {
     uint32_t fails = 0;
     int rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
+    if (rc == PJ_ERR_CAPACITY && h->lay.rowid)   // the pair buffer was too small: once more with room for all
+        rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
     if (rc == PJ_OK && fails) {
         if (!h->raw_build)
             return fail(PJ_ERR_STATE, "range plan failed and the build tuples are gone");
@@ -1855,7 +2000,7 @@ int pj_join_destroy(pj_join_t h)
         return PJ_OK;
     for (DevBuf* b : {&h->d_raw_build, &h->d_raw_probe, &h->d_out_batch, &h->rpart, &h->spart, &h->hist, &h->bsum,
                       &h->rstarts, &h->sstarts, &h->planoffs, &h->counters, &h->fine_bnd, &h->coarse_bnd,
-                      &h->pminmax, &h->pids, &h->plan_parts, &h->coarse_of})
+                      &h->pminmax, &h->pids, &h->plan_parts, &h->coarse_of, &h->pairs})
         b->release();
     if (h->ev_ok)
         for (int i = 0; i < 8; ++i)
@@ -1890,6 +2035,8 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
         return fail(PJ_ERR_INVALID, "the group buffer must be 4-byte aligned");
     *ngroups = 0;
     const pj_join_desc& d = h->desc;
+    if (h->lay.rowid)
+        return fail(PJ_ERR_UNSUPPORTED, "the count per key takes one INTEGER / LONG / DATE key or two INTEGER keys");
     // the group columns: the build key, one INTEGER / LONG / DATE column or two INTEGER columns
     pj_schema gs = {};
     gs.ncols = d.nkeys + 1;
@@ -2023,7 +2170,7 @@ int pj_join_build_append(pj_join_t h, const void* tuples, uint64_t n)
         return fail(PJ_ERR_INVALID, "null argument");
     if (h->built)
         return fail(PJ_ERR_STATE, "build side already finished");
-    const size_t w = (size_t)h->lay.build.tw * 4;
+    const size_t w = (size_t)h->lay.build.tbytes;
     std::lock_guard<std::mutex> g(h->stage_mu);
     const uint8_t* p = (const uint8_t*)tuples;
     h->staged.insert(h->staged.end(), p, p + n * w);
@@ -2037,7 +2184,7 @@ int pj_join_build_finish(pj_join_t h)
     std::lock_guard<std::mutex> g(h->mu);
     if (h->built)
         return fail(PJ_ERR_STATE, "build side already finished");
-    const size_t w = (size_t)h->lay.build.tw * 4;
+    const size_t w = (size_t)h->lay.build.tbytes;
     const uint64_t n = h->staged.size() / w;
     int rc;
     if ((rc = h->d_raw_build.ensure(std::max<size_t>(h->staged.size(), 16))))
@@ -2072,7 +2219,7 @@ int pj_join_probe(pj_join_t h, const void* tuples, uint64_t n, pj_cursor* cur, v
         ci->next = ci->total = 0;
         if (n) {
             std::lock_guard<std::mutex> g(h->mu);
-            const size_t w = (size_t)h->lay.probe.tw * 4;
+            const size_t w = (size_t)h->lay.probe.tbytes;
             int rc;
             if ((rc = h->d_raw_probe.ensure(n * w)))
                 return rc;

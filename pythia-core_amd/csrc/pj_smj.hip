@@ -243,6 +243,10 @@ int pj_smj_create(const pj_join_desc* desc, pj_smj_t* out)
     int rc = derive_layout(desc, &L);
     if (rc)
         return rc;
+    if (L.rowid)
+        return fail(PJ_ERR_UNSUPPORTED,
+                    "the sort-merge join takes one INTEGER / LONG / DATE key or two INTEGER keys, at most 8 payload "
+                    "bytes a side and output tuples of at most 32 bytes without CHAR columns");
     pj_smj_s* h = new (std::nothrow) pj_smj_s();
     if (!h)
         return fail(PJ_ERR_INVALID, "out of host memory");

@@ -24,6 +24,7 @@ PJ_MAX_COLS = 32
 PJ_MAX_KEYS = 8
 PART = {"radix": 0, "learned": 1, "auto": 2}
 PJ_FLAG_TIMING = 1
+PJ_FLAG_NARROW_ROW_HASH = 2
 
 
 class Schema(C.Structure):

@@ -5,9 +5,19 @@ import numpy as np
 FMT = {"int": "<i4", "long": "<i8", "dec": "<f8", "date": "<i8", "pointer": "<u8"}
 
 
+def fmt(c):
+    """numpy field format of a column: a type name, or ("char", width)."""
+    return "S%d" % c[1] if isinstance(c, tuple) else FMT[c]
+
+
+def dtype_of(cols):
+    """The packed (unaligned) numpy record type of a schema."""
+    return np.dtype([("f%d" % i, fmt(c)) for i, c in enumerate(cols)])
+
+
 def pack(cols, rows):
     """cols: type names; rows: list of tuples. Returns a uint8 array."""
-    dt = np.dtype([("f%d" % i, FMT[c]) for i, c in enumerate(cols)])
+    dt = dtype_of(cols)
     a = np.zeros(len(rows), dtype=dt)
     for i, r in enumerate(rows):
         a[i] = tuple(r)
@@ -15,7 +25,7 @@ def pack(cols, rows):
 
 
 def unpack(cols, buf):
-    dt = np.dtype([("f%d" % i, FMT[c]) for i, c in enumerate(cols)])
+    dt = dtype_of(cols)
     return np.frombuffer(np.ascontiguousarray(buf).reshape(-1).tobytes(), dtype=dt)
 
 

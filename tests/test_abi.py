@@ -57,8 +57,9 @@ def test_out_width_of_the_headline_projection():
     (dict(buildjattr=[3]), P.PJ_ERR_INVALID),
     (dict(build_cols=["dec", "long"]), P.PJ_ERR_UNSUPPORTED),          # DECIMAL key
     (dict(build_cols=[("char", 6), "long"]), P.PJ_ERR_UNSUPPORTED),    # CHAR key, odd width
-    (dict(build_cols=["long", "long", "long"], projection=["B$1", "B$2", "P$1", "B$0"] * 1 + ["P$0"]),
-     P.PJ_ERR_UNSUPPORTED),                                           # > 8 payload bytes a side
+    (dict(build_cols=["date", "long"]), P.PJ_ERR_UNSUPPORTED),         # DATE against LONG
+    (dict(build_cols=["long", "dec"], probe_cols=["long", "dec"], buildjattr=[0, 1], probejattr=[0, 1]),
+     P.PJ_ERR_UNSUPPORTED),                                           # DECIMAL in a composite key
 ])
 def test_bad_descriptors_are_rejected(kw, code):
     h = C.c_void_p()
@@ -66,6 +67,25 @@ def test_bad_descriptors_are_rejected(kw, code):
     rc = P.lib().pj_join_create(C.byref(d), C.byref(h))
     assert rc == code
     assert P.lib().pj_last_error()
+
+
+@pytest.mark.parametrize("kw,width", [
+    # the reference's own composite test: (long, long) keys (queryhashjoincomposite.cpp:156-200)
+    (dict(build_cols=["long", "long", "long"], probe_cols=["long", "long", "dec"], buildjattr=[0, 1],
+          probejattr=[0, 1], projection=["B$2", "P$2"]), 16),
+    # three key columns, INTEGER against LONG
+    (dict(build_cols=["int", "long", "int", "long"], probe_cols=["long", "int", "long", "long"],
+          buildjattr=[0, 1, 2], probejattr=[0, 1, 2], projection=["B$0", "B$3", "P$3"]), 20),
+    # more than 8 payload bytes a side, a 40-byte row
+    (dict(build_cols=["long", "long", "long"], projection=["B$1", "B$2", "P$1", "B$0", "P$0"]), 40),
+    # CHAR columns and a tuple width that is not a multiple of 4
+    (dict(build_cols=["long", ("char", 7), "int"], probe_cols=["long", ("char", 3)],
+          projection=["B$1", "P$1", "B$2", "B$0"]), 22),
+])
+def test_row_id_descriptors_are_accepted(kw, width):
+    j = P.Join(_desc(**kw))
+    assert j.out_width == width
+    j.close()
 
 
 def test_probe_before_build_finish_is_a_state_error():

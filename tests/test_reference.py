@@ -107,3 +107,56 @@ def test_reference_prefetch_steps_change_no_rows():
         rows = RR.run_pkfk(b, p, 4, prefetch=pf)["rows"]
         assert np.array_equal(O.sorted_rows(rows), O.sorted_rows(base)), pf
     assert base.shape[0] == (1 << 15) - (1 << 12)
+
+
+def test_oracle_row_id_shapes_equal_reference():
+    # the shapes the device joins in its row-id form: (long, long) keys
+    # (queryhashjoincomposite.cpp:156-200), three key columns across widths,
+    # CHAR columns in the projection (writeData's strncpy) and odd widths
+    rng = np.random.default_rng(45)
+    cases = [
+        (["long", "long", "long"], ["long", "long", "dec"], [0, 1], [0, 1], ["B$2", "P$2", "B$0", "P$1"],
+         lambda r, n: [(int(a), int(b), int(c)) for a, b, c in zip(r.integers(0, 300, n), r.integers(-2, 2, n),
+                                                                    r.integers(-(1 << 62), 1 << 62, n))],
+         lambda r, n: [(int(a), int(b), float(c)) for a, b, c in zip(r.integers(0, 330, n), r.integers(-2, 2, n),
+                                                                      r.standard_normal(n))]),
+        (["int", "long", "int"], ["long", "int", "long"], [0, 1, 2], [0, 1, 2], ["B$0", "P$2", "B$2", "P$1"],
+         lambda r, n: [(int(a), int(b), int(c)) for a, b, c in zip(r.integers(-9, 9, n), r.integers(-9, 9, n),
+                                                                    r.integers(-3, 3, n))],
+         lambda r, n: [(int(a), int(b), int(c)) for a, b, c in zip(
+             np.where(r.random(n) < 0.1, r.integers(1 << 32, 1 << 40, n), r.integers(-9, 9, n)),
+             r.integers(-9, 9, n), r.integers(-3, 3, n))]),
+    ]
+    for bc, pc, bj, pj, proj, gb, gp in cases:
+        build = pack(bc, gb(rng, 2000))
+        probe = pack(pc, gp(rng, 6000))
+        for fn in ("modulo", "bytes") if bc == pc[:2] + [bc[2]] else ("modulo",):
+            ref, _, _ = RR.run_join(bc, pc, bj, pj, proj, build, probe, threads=3, fn=fn, buckets=64)
+            d = O.join_desc(bc, pc, bj, pj, proj, fn=fn, buckets=64)
+            ora, _ = O.hashjoin(d, build, probe)
+            assert ref.shape[0] > 1000
+            assert np.array_equal(O.sorted_rows(ref), O.sorted_rows(ora))
+
+
+def test_oracle_char_projection_equals_reference():
+    from _util import dtype_of
+    rng = np.random.default_rng(46)
+    bc, pc = [("char", 5), "long", "int"], ["int", ("char", 3)]
+    n = 3000
+    b = np.zeros(n, dtype=dtype_of(bc))
+    raw = rng.integers(0, 256, (n, 5), dtype=np.uint8)
+    raw[rng.random((n, 5)) < 0.25] = 0
+    b["f0"] = raw.view("S5").reshape(n)
+    b["f1"] = rng.integers(-400, 400, n)
+    b["f2"] = rng.integers(-(1 << 31), 1 << 31, n)
+    p = np.zeros(3 * n, dtype=dtype_of(pc))
+    p["f0"] = rng.integers(-450, 450, 3 * n)
+    raw = rng.integers(0, 256, (3 * n, 3), dtype=np.uint8)
+    raw[rng.random((3 * n, 3)) < 0.25] = 0
+    p["f1"] = raw.view("S3").reshape(3 * n)
+    build, probe = b.view(np.uint8).reshape(-1).copy(), p.view(np.uint8).reshape(-1).copy()
+    proj = ["B$0", "P$1", "B$2", "B$1", "P$0"]
+    ref, _, _ = RR.run_join(bc, pc, [1], [0], proj, build, probe, threads=2, fn="modulo", buckets=128)
+    ora, _ = O.hashjoin(O.join_desc(bc, pc, [1], [0], proj, fn="modulo", buckets=128), build, probe)
+    assert ref.shape[0] > 5000 and ref.shape[1] == 24
+    assert np.array_equal(O.sorted_rows(ref), O.sorted_rows(ora))

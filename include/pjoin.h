@@ -158,6 +158,26 @@ int pj_join_reset(pj_join_t h);
 
 int pj_join_last_timings(pj_join_t h, pj_timings* t);
 
+/* Bytes of the fixed device scratch pj_join_run_device_noalloc works in. */
+#define PJ_NOALLOC_SCRATCH_BYTES (128u << 20)
+
+/* pj_join_run_device without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, hosts that forbid hipMalloc).
+ * The partitioned copies, histograms, plan and counters the join needs come
+ * from one fixed array of PJ_NOALLOC_SCRATCH_BYTES in device memory, which
+ * each device the library runs on holds until the process ends; the caller
+ * still passes the inputs and the output. Always radix partitions (the same
+ * rows as pj_join_run_device). A join whose partitioned copies do not fit
+ * the scratch (about 1M x 5M 16-byte tuples) is refused with
+ * PJ_ERR_UNSUPPORTED before any device work, as is a handle holding a staged
+ * build side (PJ_ERR_STATE). Calls on one device share its scratch and are
+ * serialised inside the process; overlapping use of one device's scratch
+ * from elsewhere races. Synchronous; *out_n and PJ_ERR_CAPACITY as
+ * pj_join_run_device. */
+int pj_join_run_device_noalloc(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe,
+                               uint64_t nprobe, void* d_out, uint64_t out_cap, uint64_t* out_n, void* stream);
+
 /* PartitionOp (operators/partition.cpp:44-404) on device-resident tuples:
  * the exactrange hash on column attr (an INTEGER, LONG or DATE column) over
  * [range_min, range_max] into `buckets` partitions (hash.h:452-487).

@@ -165,9 +165,20 @@ inline PartFn range_fn(uint32_t F, const uint64_t* bnd)
 }
 
 // ---------------------------------------------------------------- host
+// A fixed device region that buffers are carved from in order (the
+// no-allocation entry point's scratch): nothing is freed, and a buffer that
+// must grow takes a new piece.
+struct Arena {
+    uint8_t* base;
+    size_t size, used;
+};
+
+// A device buffer that grows on demand: from hipMalloc, or, when arena is
+// set, from that region (release then only forgets it).
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;
     int ensure(size_t bytes);
     void release();
     ~DevBuf() { release(); }

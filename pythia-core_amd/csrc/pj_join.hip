@@ -1206,11 +1206,21 @@ int DevBuf::ensure(size_t bytes)
 {
     if (bytes <= cap)
         return PJ_OK;
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (arena) {
+        const size_t at = (arena->used + 255) / 256 * 256;
+        if (at > arena->size || want > arena->size - at)
+            return fail(PJ_ERR_UNSUPPORTED, "the fixed scratch of %zu bytes cannot hold %zu more bytes", arena->size,
+                        want);
+        p = arena->base + at;
+        cap = want;
+        arena->used = at + want;
+        return PJ_OK;
+    }
     if (p)
         (void)hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = bytes < 256 ? 256 : bytes;
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
@@ -1222,7 +1232,7 @@ int DevBuf::ensure(size_t bytes)
 
 void DevBuf::release()
 {
-    if (p)
+    if (p && !arena)
         (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -1500,6 +1510,32 @@ uint32_t part_blocks(uint64_t n)
 // ordered keys, build side only) then receives each partition's key range,
 // and a write-combining scatter takes each tuple's partition from the
 // histogram pass through h->pids instead of searching again.
+// The shape of one partition pass over n tuples: blocks an input chunk
+// (split), records a write-combining chunk (B; 1 selects the tile-sorted
+// scatter), input chunks (nblk), the histogram's counters (m) and scan
+// segments (nsb), and the records the partitioned copy can hold (cap).
+struct PartGeom {
+    uint32_t split, B, nblk, nsb;
+    uint64_t chunk, m, cap;
+};
+
+PartGeom part_geom(uint64_t n, const PartFn& pf, bool range, bool write_combine)
+{
+    PartGeom g;
+    const uint32_t F = pf.F;
+    const uint32_t bb = bounds_lds_bytes(pf, range);
+    g.split = write_combine ? wc_split(F, bb) : 1;
+    g.B = write_combine ? wc_chunk(g.split > 1 ? (F + 1) / 2 : F, bb) : 1;
+    g.nblk = g.B > 1 ? part_blocks(n) : hist_blocks(n);
+    if (g.split > 1)   // input chunks pair up blocks b and b + 8 (one XCD)
+        g.nblk = (g.nblk + 7) / 8 * 8;
+    g.chunk = (n + g.nblk - 1) / g.nblk;
+    g.m = (uint64_t)F * g.nblk;
+    g.nsb = (uint32_t)((g.m + kScanSeg - 1) / kScanSeg);
+    g.cap = n + (uint64_t)F * g.nblk * (g.B - 1);
+    return g;
+}
+
 int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in, uint64_t n, const SideLayout& L,
                    const PartFn& pf, bool range, bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts,
                    bool host_copy, unsigned long long* pminmax, hipStream_t st)
@@ -1520,16 +1556,9 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
     }
     if (n >= (1ull << 31))
         return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
-    const uint32_t bb = bounds_lds_bytes(pf, range);
-    const uint32_t split = write_combine ? wc_split(F, bb) : 1;
-    const uint32_t B = write_combine ? wc_chunk(split > 1 ? (F + 1) / 2 : F, bb) : 1;
-    uint32_t nblk = B > 1 ? part_blocks(n) : hist_blocks(n);
-    if (split > 1)   // input chunks pair up blocks b and b + 8 (one XCD)
-        nblk = (nblk + 7) / 8 * 8;
-    const uint64_t chunk = (n + nblk - 1) / nblk;
-    const uint64_t m = (uint64_t)F * nblk;
-    const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
-    const uint64_t cap = n + (uint64_t)F * nblk * (B - 1);
+    const PartGeom g = part_geom(n, pf, range, write_combine);
+    const uint32_t split = g.split, B = g.B, nblk = g.nblk, nsb = g.nsb;
+    const uint64_t chunk = g.chunk, m = g.m, cap = g.cap;
     if (cap >= (1ull << 32))
         return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");
     if ((rc = dst.ensure(cap * 16)) || (rc = histb.ensure(m * 4)) || (rc = bsumb.ensure((nsb + 1) * 4)))
@@ -1816,7 +1845,9 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     void* jout = d_out;
     uint64_t jcap = out_cap;
     if (h->lay.rowid) {
-        const uint64_t want = std::max<uint64_t>(out_cap + out_cap / 64 + 1024, h->pair_want);
+        uint64_t want = std::max<uint64_t>(out_cap + out_cap / 64 + 1024, h->pair_want);
+        if (h->pairs.arena && h->pairs.cap)   // fixed scratch: the pairs it was given
+            want = std::min<uint64_t>(want, h->pairs.cap / 8);
         if ((rc = h->pairs.ensure(want * 8)))
             return rc;
         jout = h->pairs.p;
@@ -1902,8 +1933,13 @@ int probe_device(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, ui
 {
     uint32_t fails = 0;
     int rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
-    if (rc == PJ_ERR_CAPACITY && h->lay.rowid)   // the pair buffer was too small: once more with room for all
+    if (rc == PJ_ERR_CAPACITY && h->lay.rowid) {
+        if (h->pairs.arena)   // a fixed scratch cannot grow
+            return fail(PJ_ERR_UNSUPPORTED, "the join's %llu row pairs exceed the %llu the fixed scratch holds",
+                        (unsigned long long)h->pair_want, (unsigned long long)h->pair_cap);
+        // the pair buffer was too small: once more with room for all
         rc = probe_once(h, d_probe, np, d_out, out_cap, out_n, &fails, st);
+    }
     if (rc == PJ_OK && fails) {
         if (!h->raw_build)
             return fail(PJ_ERR_STATE, "range plan failed and the build tuples are gone");
@@ -1939,6 +1975,103 @@ int pj::join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void*
     bool empty;
     return probe_enqueue(h, d_probe, np, d_out, out_cap, slot, &empty, st);
 }
+
+// ---------------------------------------------------------------- scratch
+// The no-allocation entry point's device memory: one fixed array, the same
+// symbol on every device (each device holds its own copy). Host code never
+// passes the symbol itself to a kernel; it takes the array's address on each
+// device once, with hipGetSymbolAddress, and uses that.
+static __device__ __align__(256) uint8_t scratch[PJ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct ScratchSlot {
+    std::mutex mu;        // calls on one device share its scratch: one at a time
+    void* ptr = nullptr;  // the device's address of `scratch`
+};
+ScratchSlot g_scratch[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// The current device and its address of `scratch` (looked up once a device).
+int scratch_slot(ScratchSlot** out)
+{
+    int dev = 0;
+    PJ_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(PJ_ERR_DEVICE, "device %d beyond the %d the scratch table covers", dev, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    ScratchSlot& s = g_scratch[dev];
+    if (!s.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PJ_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                        hipGetErrorString(e));
+        s.ptr = p;
+    }
+    *out = &s;
+    return PJ_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+size_t piece(size_t bytes) { return align256(bytes < 256 ? 256 : bytes); }
+
+// Scratch bytes one radix join of nb x np tuples carves (every buffer the
+// partition passes and the probe phase ensure, each a piece of its own),
+// the row-id form's pairs aside.
+size_t noalloc_need(const pj_join_s* h, uint64_t nb, uint64_t np)
+{
+    const uint32_t bR = choose_build_bits(nb, h->desc.radix_bits);
+    const uint32_t bS = std::min<uint32_t>(bR, 11);
+    size_t need = piece(((size_t)(1u << bR) + 1) * 4) + piece(((size_t)(1u << bS) + 1) * 4);
+    if (nb) {
+        const PartGeom g = part_geom(nb, radix_fn(bR), false, false);
+        need += piece(g.cap * 16) + piece(g.m * 4) + piece(((size_t)g.nsb + 1) * 4);
+    }
+    if (np) {
+        const PartGeom g = part_geom(np, radix_fn(bS), false, true);
+        need += piece(g.cap * 16) + piece(g.m * 4) + piece(((size_t)g.nsb + 1) * 4);
+    }
+    need += piece(((size_t)(1u << bR) + 1) * 4) + piece(64);   // plan offsets, counters
+    return need;
+}
+
+// For the duration of one call, the handle's device buffers are empty
+// buffers carved from the scratch; the handle's own are put back after.
+struct ScratchSwap {
+    struct Saved {
+        void* p;
+        size_t cap;
+        Arena* arena;
+    };
+    Arena ar;
+    DevBuf* bufs[10];
+    Saved saved[10];
+    ScratchSwap(pj_join_s* h, void* base, size_t size)
+        : ar{(uint8_t*)base, size, 0},
+          bufs{&h->rpart, &h->spart, &h->hist, &h->bsum, &h->pids, &h->rstarts, &h->sstarts, &h->planoffs,
+               &h->counters, &h->pairs}
+    {
+        for (int i = 0; i < 10; ++i) {
+            saved[i] = Saved{bufs[i]->p, bufs[i]->cap, bufs[i]->arena};
+            bufs[i]->p = nullptr;
+            bufs[i]->cap = 0;
+            bufs[i]->arena = &ar;
+        }
+    }
+    ~ScratchSwap()
+    {
+        for (int i = 0; i < 10; ++i) {
+            bufs[i]->p = saved[i].p;
+            bufs[i]->cap = saved[i].cap;
+            bufs[i]->arena = saved[i].arena;
+        }
+    }
+};
+
+}  // namespace
 
 // ======================================================================
 // C ABI
@@ -1982,6 +2115,11 @@ int pj_join_create(const pj_join_desc* desc, pj_join_t* out)
     for (int i = 0; i < 8; ++i)
         if (hipEventCreate(&h->ev[i]) != hipSuccess)
             h->ev_ok = false;
+    // pinned words the probe phase's counters are read back into (without a
+    // device the handle still validates descriptors; the probe phase then
+    // allocates them, and the no-allocation entry point refuses the handle)
+    if (hipHostMalloc((void**)&h->host_ctr, 64, hipHostMallocDefault) != hipSuccess)
+        h->host_ctr = nullptr;
     *out = h;
     return PJ_OK;
 }
@@ -2283,6 +2421,53 @@ int pj_join_last_timings(pj_join_t h, pj_timings* t)
         return fail(PJ_ERR_INVALID, "null argument");
     *t = h->last;
     return PJ_OK;
+}
+
+int pj_join_run_device_noalloc(pj_join_t h, const void* d_build, uint64_t nbuild, const void* d_probe,
+                               uint64_t nprobe, void* d_out, uint64_t out_cap, uint64_t* out_n, void* stream)
+{
+    if (!h || !out_n || (nbuild && !d_build) || (nprobe && !d_probe) || (out_cap && !d_out))
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (out_cap && ((uintptr_t)d_out % 16))
+        return fail(PJ_ERR_INVALID, "the output buffer must be 16-byte aligned");
+    *out_n = 0;
+    if (nbuild >= (1ull << 31) || nprobe >= (1ull << 31))
+        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
+    if (!h->host_ctr)
+        return fail(PJ_ERR_STATE, "the handle has no pinned counter words (created without a device?)");
+    ScratchSlot* slot = nullptr;
+    int rc;
+    if ((rc = scratch_slot(&slot)))
+        return rc;
+    std::lock_guard<std::mutex> g(h->mu);
+    if (h->built)
+        return fail(PJ_ERR_STATE, "the handle holds a staged build side (pj_join_reset first)");
+    const size_t need = noalloc_need(h, nbuild, nprobe);
+    const size_t size = sizeof(scratch);
+    if (need > size)
+        return fail(PJ_ERR_UNSUPPORTED,
+                    "a %llu x %llu join needs %zu bytes of scratch; the fixed scratch holds %zu (use "
+                    "pj_join_run_device)",
+                    (unsigned long long)nbuild, (unsigned long long)nprobe, need, size);
+    std::lock_guard<std::mutex> gs(slot->mu);
+    ScratchSwap sw(h, slot->ptr, size);
+    if (h->lay.rowid) {   // the (build row, probe row) pairs take what the partition passes leave
+        const size_t room = (size - need - 256) / 8 * 8;
+        if (room < 8 * 1024)
+            return fail(PJ_ERR_UNSUPPORTED, "the fixed scratch leaves no room for the join's row pairs");
+        if ((rc = h->pairs.ensure(room)))
+            return rc;
+    }
+    hipStream_t st = as_stream(stream);
+    h->model_ms = 0.f;
+    h->raw_build = d_build;
+    h->raw_nbuild = nbuild;
+    h->plan.ok = false;
+    record(h, 0, st);
+    if ((rc = prepare_radix(h, d_build, nbuild, st)))
+        return rc;
+    record(h, 1, st);
+    return probe_device(h, d_probe, nprobe, d_out, out_cap, out_n, st);
 }
 
 }  // extern "C"

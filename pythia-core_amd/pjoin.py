@@ -97,6 +97,8 @@ _SIGS = {
     "pj_join_destroy": (C.c_int, [C.c_void_p]),
     "pj_join_run_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                      C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pj_join_run_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                     C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.c_void_p]),
     "pj_join_build_append": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "pj_join_build_finish": (C.c_int, [C.c_void_p]),
     "pj_join_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(Cursor), C.c_void_p,
@@ -251,6 +253,17 @@ class Join:
         n = C.c_uint64()
         rc = lib().pj_join_run_device(self.h, C.c_void_p(d_build), nbuild, C.c_void_p(d_probe), nprobe,
                                       C.c_void_p(d_out), out_cap, C.byref(n), C.c_void_p(stream))
+        if rc == PJ_ERR_CAPACITY:
+            raise PJError(rc, "need %d output tuples" % n.value)
+        check(rc)
+        return n.value
+
+    def run_device_noalloc(self, d_build, nbuild, d_probe, nprobe, d_out, out_cap, stream=None):
+        """pj_join_run_device_noalloc: the same join in the library's fixed
+        device scratch (no device allocation)."""
+        n = C.c_uint64()
+        rc = lib().pj_join_run_device_noalloc(self.h, C.c_void_p(d_build), nbuild, C.c_void_p(d_probe), nprobe,
+                                              C.c_void_p(d_out), out_cap, C.byref(n), C.c_void_p(stream))
         if rc == PJ_ERR_CAPACITY:
             raise PJError(rc, "need %d output tuples" % n.value)
         check(rc)

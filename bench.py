@@ -2,15 +2,27 @@
 GB/s on the 16M x 256M uniform int64 join (keys a seeded sample of
 [0, 2^40), 10% of probes miss), plain radix partitioning, inputs resident in
 HBM. One step is one whole join: partition R, partition S, LDS build and
-probe, output written to HBM.
+probe, output written to HBM. The default run also times the clustered
+64M x 512M join under radix and learned partitions (learned_vs_radix) and
+the reference HashJoinOp on the host cores.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--partitioner radix]
 Prints one JSON line.
+
+python bench.py --collect-traffic [--workload W]
+Runs this benchmark twice under rocprofv3 (FETCH_SIZE, then WRITE_SIZE),
+scales the counters by a 1 GiB copy's, and writes the HBM bytes of every
+kernel of one step to profiles/traffic_<workload>.md together with the
+library's hash; the benchmark reports those bytes as `traffic` only while
+that hash is the current library's.
 """
 import argparse
 import ctypes as C
+import glob
+import hashlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -19,10 +31,8 @@ sys.path.insert(0, os.path.join(ROOT, "pythia-core_amd"))
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes one k_join launch moves on each workload, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy
-# (profiles/k_join_uniform_16Mx256M.md).
-JOIN_TRAFFIC_BYTES = {"uniform_16Mx256M": 11.122e9}
+LIB_SO = os.path.join(ROOT, "pythia-core_amd", "lib", "libpjoin.so")
+PROFILES = os.path.join(ROOT, "profiles")
 
 WORKLOADS = {
     # name: (seed, key domain, |R|, |S|, misses)
@@ -427,6 +437,291 @@ def load_from_files(args, P, torch, db, nb, dp, npr, stream):
                     "parity check covers them"}
 
 
+def lib_sha():
+    """First 16 hex digits of the SHA-256 of the built library."""
+    h = hashlib.sha256()
+    with open(LIB_SO, "rb") as f:
+        for block in iter(lambda: f.read(1 << 20), b""):
+            h.update(block)
+    return h.hexdigest()[:16]
+
+
+def host_info():
+    """What the CPU baseline ran on: core counts, model, sockets, NUMA nodes."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "NUMA node(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.lower().replace("(s)", "s").replace(" ", "_")] = v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+# ---------------------------------------------------------------- traffic
+# The HBM bytes of one step's kernels, per workload, from two rocprofv3
+# counter passes; kept in profiles/traffic_<workload>.md as a JSON block.
+
+def traffic_path(workload):
+    return os.path.join(PROFILES, "traffic_%s.md" % workload)
+
+
+def read_traffic(workload):
+    """{kernel label: {"read": B, "write": B}} when the summary was made
+    from the current library, else None."""
+    p = traffic_path(workload)
+    if not os.path.exists(p):
+        return None
+    txt = open(p).read()
+    a = txt.find("```json")
+    if a < 0:
+        return None
+    b = txt.find("```", a + 7)
+    d = json.loads(txt[a + 7:b])
+    if d.get("library_sha256_16") != lib_sha():
+        return None
+    return d["kernels"]
+
+
+def _label_step_kernels(rows):
+    """Dispatches of one join step in launch order -> {label: [counter values]}.
+    The step launches k_hist twice (build, then probe), the build side's
+    scatter, the probe side's write-combining scatter, k_plan, k_join."""
+    out, hist_seen = {}, 0
+    for name, val in rows:
+        if "k_hist" in name:
+            label = "k_hist build" if hist_seen == 0 else "k_hist probe"
+            hist_seen += 1
+        elif "k_scatter_wc" in name:
+            label = "k_scatter_wc probe"
+        elif "k_scatter" in name:
+            label = "k_scatter build"
+        elif "k_join" in name:
+            label = "k_join"
+        elif "k_scan" in name or "k_part_bounds" in name or "k_plan" in name or "k_fill" in name:
+            label = "scans, partition starts, k_plan"
+        else:
+            continue
+        out.setdefault(label, []).append(val)
+    return out
+
+
+def _counter_rows(d, counter):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    import csv
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def collect_traffic(args):
+    """Two rocprofv3 passes over one step of the workload; writes the summary."""
+    base = os.path.join(ROOT, "gpurun_out", "traffic_%s" % args.workload)
+    cmd_tail = ["python3", os.path.abspath(__file__), "--workload", args.workload, "--steps", "1", "--warmup", "0",
+                "--no-cpu-baseline", "--no-compare", "--calibrate-copy"]
+    per = {}
+    cmds = []
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = base + "_" + counter
+        cmd = ["rocprofv3", "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--"] + cmd_tail
+        cmds.append(" ".join(cmd).replace(ROOT, "$R"))
+        _say("traffic: " + " ".join(cmd))
+        subprocess.run(cmd, check=True, cwd="/tmp", timeout=200)
+        rows = _counter_rows(d, counter)
+        copies = [v for k, v in rows if "copyBuffer" in k or "copy_kernel" in k or "elementwise" in k]
+        if not copies:
+            raise RuntimeError("no calibration copy in the %s pass" % counter)
+        per[counter] = (_label_step_kernels(rows), max(copies))
+    gib = 1 << 30
+    fetch_factor = gib / (per["FETCH_SIZE"][1] * 1024.0)
+    write_factor = gib / (per["WRITE_SIZE"][1] * 1024.0)
+    kernels = {}
+    for label, vals in per["FETCH_SIZE"][0].items():
+        wv = per["WRITE_SIZE"][0].get(label, [0.0])
+        kernels[label] = {"read": sum(vals) * 1024.0 * fetch_factor, "write": sum(wv) * 1024.0 * write_factor,
+                          "dispatches": len(vals)}
+    d = {"library_sha256_16": lib_sha(), "workload": args.workload, "fetch_factor": fetch_factor,
+         "write_factor": write_factor, "kernels": kernels}
+    lines = ["# HBM traffic of one %s step, per kernel" % args.workload, "",
+             "rocprofv3 counter passes (one counter a pass), each scaled by what the counter reads for the "
+             "run's 1 GiB device copy (FETCH x %.2f, WRITE x %.2f). bench.py reports these bytes as `traffic` "
+             "while the library's hash below is the built library's." % (fetch_factor, write_factor), "",
+             "Commands (from /tmp, repository at $R), written by `python bench.py --collect-traffic "
+             "--workload %s`:" % args.workload, ""]
+    lines += ["    " + c for c in cmds]
+    lines += ["", "| kernel | dispatches | HBM read (GB) | HBM write (GB) |", "|---|---|---|---|"]
+    for label, v in sorted(kernels.items()):
+        lines.append("| %s | %d | %.3f | %.3f |" % (label, v["dispatches"], v["read"] / 1e9, v["write"] / 1e9))
+    lines += ["", "```json", json.dumps(d, indent=1), "```", ""]
+    for path in (traffic_path(args.workload), os.path.join(ROOT, "gpurun_out", os.path.basename(
+            traffic_path(args.workload)))):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            f.write("\n".join(lines))
+    print(json.dumps(d))
+
+
+# ---------------------------------------------------------------- CPU baseline
+
+class _Relations:
+    """Host copies of two device relations in files (in /dev/shm where it
+    exists) for reference runs in processes of their own; removed on exit."""
+
+    def __init__(self, db, dp):
+        import tempfile
+        import numpy as np
+        d = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
+        self.paths = [os.path.join(d, "pj_cpu_%d_%s.npy" % (os.getpid(), s)) for s in ("build", "probe")]
+        for t, path in zip((db, dp), self.paths):
+            np.save(path, t.cpu().numpy().view(np.int64))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        for path in self.paths:
+            if os.path.exists(path):
+                os.remove(path)
+
+    def run(self, threads, prefetch):
+        """One reference run in a new process (a fresh reference allocator)."""
+        out = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "reference_run.py"), "consume",
+                              self.paths[0], self.paths[1], str(threads), str(prefetch)],
+                             stdout=subprocess.PIPE, check=True, text=True, timeout=300).stdout
+        return json.loads(out.strip().splitlines()[-1])
+
+
+def cpu_baseline_full(db, dp, nb, npr, gpu_xor32):
+    """The reference HashJoinOp (oracle/_ref, -O3 from its sources) on the
+    whole join, under MergeOp(T) -> ConsumeOp (operators/consume.cpp), the
+    relations copied from the device (the generators are bit-identical on
+    host and device). Every run is a process of its own: the reference's
+    allocator hands out pre-reserved arenas that the first runs of a process
+    use up, after which each build faults in fresh pages and takes about
+    three times as long. T = the cores this process may use, at most 16 (the
+    host cores that come with one GPU on the GPU box). probeprefetchstep /
+    buildprefetchstep: one run each of off, 8, 16, 32 (the warm-up), then 5
+    runs with the fastest; medians reported. Parity: ConsumeOp's XOR of the
+    output's 32-bit words equals the device output's."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import reference_run as R
+    if not R.available():
+        return None
+    threads = R.default_threads()
+    with _Relations(db, dp) as rel:
+        sweep = {}
+        for pf in (-1, 8, 16, 32):
+            r = rel.run(threads, pf)
+            sweep[pf] = r
+            _say("cpu baseline: prefetch %d, %.3f s (build %.3f s)" % (pf, r["seconds"], r["build_seconds"]))
+        best = min(sweep, key=lambda k: sweep[k]["seconds"])
+        runs = [rel.run(threads, best) for _ in range(5)]
+    med = lambda key: sorted(r[key] for r in runs)[2]
+    secs = med("seconds")
+    info = host_info()
+    return {"value": (nb + npr) / secs, "unit": "tuples/s", "cores": threads, "kind": "reference",
+            "sample": "the whole %d x %d join (no sample): the reference HashJoinOp built -O3 from its sources "
+                      "under MergeOp(%d) -> ConsumeOp, modulo hash over next_pow2(|R|) buckets of 2 tuples, "
+                      "1 MiB pages, prefetch step %s (fastest of off, 8, 16, 32); median of 5 runs after one "
+                      "run of each prefetch step" % (nb, npr, threads, "off" if best < 0 else best),
+            "seconds": secs, "build_seconds": med("build_seconds"), "probe_seconds": med("probe_seconds"),
+            "seconds_by_prefetch": {("off" if k < 0 else str(k)): v["seconds"] for k, v in sweep.items()},
+            "runs_seconds": [r["seconds"] for r in runs],
+            "parity": {"xor32": runs[0]["xor32"], "equals_device_xor32": runs[0]["xor32"] == gpu_xor32},
+            "threads_cap": "min(cores this process may use, 16): the GPU box's CPU share is 16 threads a GPU",
+            "host": info}
+
+
+def cpu_baseline_sample(db, dp, nb, npr, nsample, gpu_xor32=None):
+    """The reference HashJoinOp under ConsumeOp on the first (at most 16M)
+    build tuples and the first nsample probe tuples (1 warm-up, median of
+    3). The reference's allocator fails on the 4 GiB chained table of 64M
+    build keys (its numadeallocate then aborts the process), so larger build
+    sides are cut to 16M, a 1 GiB table as in the headline join."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import reference_run as R
+    if not R.available():
+        return None
+    nbs = min(nb, 1 << 24)
+    threads = R.default_threads()
+    with _Relations(db[: nbs * 16], dp[: nsample * 16]) as rel:
+        rel.run(threads, 16)
+        runs = [rel.run(threads, 16) for _ in range(3)]
+    r = sorted(runs, key=lambda x: x["seconds"])[1]
+    return {"value": (nbs + nsample) / r["seconds"], "unit": "tuples/s", "cores": threads, "kind": "reference",
+            "sample": "first %d of %d build + first %d of %d probe tuples; reference HashJoinOp under "
+                      "MergeOp(%d) -> ConsumeOp, modulo hash, prefetch step 16, median of 3 after a warm-up"
+                      % (nbs, nb, nsample, npr, threads),
+            "seconds": r["seconds"], "build_seconds": r["build_seconds"], "probe_seconds": r["probe_seconds"],
+            "host": host_info()}
+
+
+# ---------------------------------------------------------------- learned vs radix
+
+def compare_learned(args, P, torch, dev, stream, with_cpu):
+    """The clustered 64M x 512M join (256 dense clusters, Zipf-1.0 fk) under
+    radix and under learned partitions, in this run, each checked against
+    the generator's closed form (so both give the same output multiset)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    seed, nb, npr = CLUSTERED_WORKLOADS["clustered_64Mx512M"]
+    table = P.cluster_table(seed, nb)
+    cdf = P.zipf_cdf(nb, 1.0)
+    db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
+    dp = torch.empty(npr * 16, dtype=torch.uint8, device=dev)
+    P.check(P.lib().pj_gen_clustered(table.ctypes.data_as(C.c_void_p), npr, cdf.ctypes.data_as(C.c_void_p),
+                                     C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
+    out = torch.empty(npr * 24, dtype=torch.uint8, device=dev)
+    es, ec = O.expected_clustered(table, npr, cdf, threads=16)
+    res = {"workload": "clustered_64Mx512M_z1", "build_tuples": nb, "probe_tuples": npr}
+    for part in ("radix", "learned"):
+        j = P.Join(P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                               partitioner=part, timing=True))
+        for _ in range(max(1, args.warmup)):
+            n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr, stream)
+        torch.cuda.synchronize()
+        steps = max(3, args.steps // 2)
+        ph = {"model_ms": 0.0, "build_partition_ms": 0.0, "probe_partition_ms": 0.0, "join_ms": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr, stream)
+            t = j.timings()
+            for k in ph:
+                ph[k] += getattr(t, k) / steps
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        s, x = P.checksum_device(out.data_ptr(), n, 24, stream)
+        res[part] = {"ms_per_step": ms, "value": (nb + npr) / (ms * 1e-3), "phases_ms": ph,
+                     "used_learned": int(t.used_learned), "partitions": int(t.partitions),
+                     "work_items": int(t.work_items),
+                     "parity": {"count": n, "checksum": s, "matches_closed_form": (n, s) == (ec, es)}}
+        _say("learned_vs_radix: %s %.2f ms/step" % (part, ms))
+        j.close()
+    res["ratio_learned_over_radix"] = res["radix"]["ms_per_step"] / res["learned"]["ms_per_step"]
+    res["identical_output"] = (res["radix"]["parity"]["matches_closed_form"] and
+                               res["learned"]["parity"]["matches_closed_form"])
+    res["target_ratio"] = 1.3
+    del out
+    if with_cpu:
+        res["cpu_baseline"] = cpu_baseline_sample(db, dp, nb, npr, 1 << 26)
+    del db, dp
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,7 +744,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB with torch, a kernel of known traffic for scaling HBM counters")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the learned-vs-radix comparison on the clustered 64M x 512M join")
+    ap.add_argument("--collect-traffic", action="store_true",
+                    help="write profiles/traffic_<workload>.md from two rocprofv3 counter passes")
     args = ap.parse_args()
+    if args.collect_traffic:
+        return collect_traffic(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
               "written up, not built); run with --gpus 1", file=sys.stderr)
@@ -543,7 +844,7 @@ def main():
         n = step()
     torch.cuda.synchronize()
     phases = {"model_ms": 0.0, "build_partition_ms": 0.0, "probe_partition_ms": 0.0, "join_ms": 0.0,
-              "total_ms": 0.0}
+              "total_ms": 0.0, "build_hist_ms": 0.0, "probe_hist_ms": 0.0}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         n = step()
@@ -557,7 +858,7 @@ def main():
         phases[k] /= args.steps
 
     # parity of the measured output: count and checksum vs the closed form
-    s, _ = P.checksum_device(out.data_ptr(), n, j.out_width, stream)
+    s, gpu_xor32 = P.checksum_device(out.data_ptr(), n, j.out_width, stream)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     if clustered:
@@ -617,10 +918,27 @@ def main():
         agg.close()
     w_in, w_out = 16, j.out_width
     b_alg = nb * w_in + npr * w_in + n * w_out
-    join_alg = b_alg  # the join kernel reads R and S partitions once and writes the output
-    achieved = join_alg / (phases["join_ms"] * 1e-3) / 1e9
     job_gbs = b_alg / (ms * 1e-3) / 1e9
     t = j.timings()
+    # per kernel: HIP-event times on the library's stream (the histogram
+    # times include their scans), algorithmic bytes, and the HBM bytes of
+    # the current build's counter summary when there is one
+    traffic = read_traffic(args.workload)
+    kern = [("k_hist build", phases["build_hist_ms"], nb * w_in),
+            ("k_scatter build", phases["build_partition_ms"] - phases["build_hist_ms"], 2 * nb * w_in),
+            ("k_hist probe", phases["probe_hist_ms"], npr * w_in),
+            ("k_scatter_wc probe", phases["probe_partition_ms"] - phases["probe_hist_ms"], 2 * npr * w_in),
+            ("k_join", phases["join_ms"], b_alg)]
+    kernels = []
+    for name, kms, kbytes in kern:
+        ach = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None
+        tr = traffic.get(name) if traffic else None
+        kernels.append({"kernel": name, "ms": kms, "alg_bytes": kbytes, "achieved": ach, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS if ach else None,
+                        "traffic": (tr["read"] + tr["write"]) if tr else None})
+    dom = max(kernels, key=lambda k: k["ms"])
+    dom["dominant"] = True
+    step_traffic = sum(k["traffic"] for k in kernels) if traffic and all(k["traffic"] for k in kernels) else None
     if agg_res is not None:
         phases["aggregate_ms"] = agg_res["ms_median"]
     res = {
@@ -638,7 +956,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int64",
+        "dtype": "int32" if zipf else "int64",
         "data": "synthetic (seeded generator, inputs resident in HBM)",
         "config": {"workload": args.workload + ("_z%g" % args.zipf if zipf or clustered else ""), "seed": seed,
                    "build_tuples": nb, "probe_tuples": npr, "probe_misses": nmiss,
@@ -651,9 +969,13 @@ def main():
         "hbm_gbs_inputs_only": (nb + npr) * w_in / (ms * 1e-3) / 1e9,
         "phases_ms": phases,
         "parity": {"count": n, "expected_count": ec, "checksum_ok": parity},
-        "roofline": {"bound": "hbm", "kernel": "k_join", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": JOIN_TRAFFIC_BYTES.get(args.workload)},
+        "roofline": {"bound": "hbm", "kernel": "whole step (partition R, partition S, join)",
+                     "achieved": job_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": job_gbs / HBM_PEAK_GBS,
+                     "traffic": step_traffic,
+                     "traffic_source": ("profiles/traffic_%s.md (rocprofv3 FETCH_SIZE / WRITE_SIZE of this "
+                                        "library build)" % args.workload) if step_traffic else
+                                       "none for this library build (bench.py --collect-traffic)",
+                     "kernels": kernels},
     }
     if load_res is not None:
         res["load_from_files"] = load_res
@@ -662,14 +984,22 @@ def main():
         res["config"]["aggregate"] = args.aggregate
         res["aggregate"] = agg_res
     j.close()
-    del db, dp, out
+    del out
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         if clustered:
-            res["cpu_baseline"] = None   # the reference table for 64M build tuples: see the uniform workload
+            res["cpu_baseline"] = cpu_baseline_sample(db, dp, nb, npr, 1 << 26)
+        elif zipf:
+            res["cpu_baseline"] = cpu_baseline_zipf(seed, nb, npr, cdf)
         else:
-            res["cpu_baseline"] = cpu_baseline_zipf(seed, nb, npr, cdf) if zipf else cpu_baseline(seed, dom, nb,
-                                                                                                  npr, nmiss)
+            res["cpu_baseline"] = cpu_baseline_full(db, dp, nb, npr, gpu_xor32) or cpu_baseline(seed, dom, nb, npr,
+                                                                                               nmiss)
+    del db, dp
+    torch.cuda.empty_cache()
+    if not args.no_compare and args.workload == "uniform_16Mx256M" and args.aggregate == "none" \
+            and not args.from_files:
+        res["learned_vs_radix"] = compare_learned(args, P, torch, dev, stream, not args.no_cpu_baseline)
+        parity = parity and res["learned_vs_radix"]["identical_output"]
     print(json.dumps(res))
     if not parity:
         sys.exit(1)

@@ -89,6 +89,8 @@ typedef struct {
     uint32_t work_items;       /* join work items */
     uint32_t used_learned;     /* 1 if the learned partitioner was used */
     uint64_t spilled_items;    /* work items whose R part exceeded one LDS table */
+    float build_hist_ms;       /* of build_partition_ms: the histogram and its scan */
+    float probe_hist_ms;       /* of probe_partition_ms: the histogram and its scan */
 } pj_timings;
 
 /* Resumable probe cursor: the state HashJoinOp keeps between getNext calls

@@ -197,6 +197,108 @@ int ref_hashjoin_run(int threads, int nbcols, const int* btypes, const int* bwid
     }
 }
 
+// The headline plan of the CPU baseline: MergeOp(threads) -> ConsumeOp ->
+// HashJoinOp (operators/consume.cpp:61-110): every worker thread drains its
+// own join output and XORs its 32-bit words, so no row is copied and the
+// timed region is the join. *xor32 gets the XOR over all threads (the
+// order-independent check of the output), *t_build the scanStart time.
+int ref_hashjoin_consume(int threads, int nbcols, const int* btypes, const int* bwidths, int npcols,
+                         const int* ptypes, const int* pwidths, int nkeys, const int* bjattr, const int* pjattr,
+                         int nproj, const char* const* proj, const char* hashfn, int buckets,
+                         int tuplesperbucket, int buffsize, int prefetch, const void* build, uint64_t nbuild,
+                         const void* probe, uint64_t nprobe, uint32_t* xor32, double* t_build, double* t_total)
+{
+    try {
+        Query q;
+        MemSourceOp src_build, src_probe;
+        HashJoinOp join;
+        ConsumeOp consume;
+        MergeOp merge;
+        src_build.setup(btypes, bwidths, nbcols, build, nbuild);
+        src_probe.setup(ptypes, pwidths, npcols, probe, nprobe);
+
+        Config cfg;
+        cfg.getRoot().add("path", Setting::TypeString) = "./";
+        cfg.getRoot().add("buffsize", Setting::TypeInt) = buffsize;
+        Setting& sb = cfg.getRoot().add("srcbuild", Setting::TypeGroup);
+        Setting& sp = cfg.getRoot().add("srcprobe", Setting::TypeGroup);
+        Setting& jn = cfg.getRoot().add("join", Setting::TypeGroup);
+        Setting& hn = jn.add("hash", Setting::TypeGroup);
+        hn.add("fn", Setting::TypeString) = hashfn;
+        hn.add("buckets", Setting::TypeInt) = buckets;
+        Setting& tg = jn.add("threadgroups", Setting::TypeList);
+        Setting& g0 = tg.add(Setting::TypeArray);
+        for (int i = 0; i < threads; ++i)
+            g0.add(Setting::TypeInt) = i;
+        jn.add("tuplesperbucket", Setting::TypeInt) = tuplesperbucket;
+        jn.add("allocpolicy", Setting::TypeString) = "local";
+        if (prefetch >= 0) {
+            jn.add("probeprefetchstep", Setting::TypeInt) = prefetch;
+            jn.add("buildprefetchstep", Setting::TypeInt) = prefetch;
+        }
+        if (nkeys == 1) {
+            jn.add("buildjattr", Setting::TypeInt) = bjattr[0];
+            jn.add("probejattr", Setting::TypeInt) = pjattr[0];
+        } else {
+            Setting& bj = jn.add("buildjattr", Setting::TypeArray);
+            Setting& pj = jn.add("probejattr", Setting::TypeArray);
+            for (int i = 0; i < nkeys; ++i) {
+                bj.add(Setting::TypeInt) = bjattr[i];
+                pj.add(Setting::TypeInt) = pjattr[i];
+            }
+        }
+        Setting& pn = jn.add("projection", Setting::TypeList);
+        for (int i = 0; i < nproj; ++i)
+            pn.add(Setting::TypeString) = proj[i];
+        Setting& cn = cfg.getRoot().add("consume", Setting::TypeGroup);
+        Setting& mn = cfg.getRoot().add("merge", Setting::TypeGroup);
+        mn.add("threads", Setting::TypeInt) = threads;
+
+        q.tree = &merge;
+        merge.nextOp = &consume;
+        consume.nextOp = &join;
+        join.buildOp = &src_build;
+        join.probeOp = &src_probe;
+        src_build.init(cfg, sb);
+        src_probe.init(cfg, sp);
+        join.init(cfg, jn);
+        consume.init(cfg, cn);
+        merge.init(cfg, mn);
+
+        q.threadInit();
+        const double t0 = now();
+        if (q.scanStart() == Operator::Error)
+            return -1;
+        const double t1 = now();
+        uint32_t x = 0;
+        Operator::GetNextResultT res;
+        res.first = Operator::Ready;
+        while (res.first == Operator::Ready) {
+            res = q.getNext();
+            if (res.first == Operator::Error)
+                return -1;
+            Operator::Page::Iterator it = res.second->createIterator();
+            void* tup;
+            while ((tup = it.next())) {
+                long long v;
+                memcpy(&v, tup, sizeof(v));
+                x ^= (uint32_t)v;
+            }
+        }
+        if (q.scanStop() == Operator::Error)
+            return -1;
+        const double t2 = now();
+        q.threadClose();
+        q.destroynofree();
+        *xor32 = x;
+        *t_build = t1 - t0;
+        *t_total = t2 - t0;
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 // Runs AggregateSum (op 0, over column sumfield) or AggregateCount (op 1),
 // grouped by nfields columns (0: the whole input), in the reference's global
 // mode: MergeOp(threads) over one aggregate whose hash table all threads

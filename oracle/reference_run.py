@@ -41,6 +41,13 @@ def lib():
             C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
             C.c_int, C.c_char_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
             C.POINTER(C.c_uint64), C.POINTER(C.c_uint), C.POINTER(C.c_double)]
+        L.ref_hashjoin_consume.restype = C.c_int
+        L.ref_hashjoin_consume.argtypes = [
+            C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+            C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int,
+            C.POINTER(C.c_char_p), C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
+            C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+            C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ref_tuple_hash_bytes.restype = C.c_uint
         L.ref_tuple_hash_bytes.argtypes = [C.c_int, C.c_void_p, C.c_int]
         _lib = L
@@ -117,6 +124,39 @@ def run_pkfk(build, probe, threads, prefetch=-1):
     return {"seconds": tt, "build_seconds": tb, "out_tuples": rows.shape[0], "rows": rows}
 
 
+def run_consume(build_cols, probe_cols, buildjattr, probejattr, projection, build, probe, threads,
+                fn="modulo", buckets=1024, tuplesperbucket=2, buffsize=1 << 20, prefetch=-1):
+    """The reference plan MergeOp(threads) -> ConsumeOp -> HashJoinOp
+    (operators/consume.cpp): every thread drains its own output and XORs its
+    32-bit words. Returns (xor32 of all output words, build seconds, total
+    seconds)."""
+    bt, bw, bsz = _schema(build_cols)
+    pt, pw, psz = _schema(probe_cols)
+    nb, npr = build.size // bsz, probe.size // psz
+    x, tb, tt = C.c_uint32(), C.c_double(), C.c_double()
+    proj = (C.c_char_p * len(projection))(*[p.encode() for p in projection])
+    rc = lib().ref_hashjoin_consume(threads, len(bt), _ints(bt), _ints(bw), len(pt), _ints(pt), _ints(pw),
+                                    len(buildjattr), _ints(buildjattr), _ints(probejattr), len(projection), proj,
+                                    fn.encode(), buckets, tuplesperbucket, buffsize, prefetch,
+                                    build.ctypes.data_as(C.c_void_p) if build.size else None, nb,
+                                    probe.ctypes.data_as(C.c_void_p) if probe.size else None, npr,
+                                    C.byref(x), C.byref(tb), C.byref(tt))
+    if rc != 0:
+        raise RuntimeError("reference HashJoinOp under ConsumeOp failed: %d" % rc)
+    return x.value, tb.value, tt.value
+
+
+def consume_pkfk(build, probe, threads, prefetch=-1):
+    """run_consume on the headline join (projection [B$0, B$1, P$1], modulo
+    hash over next_pow2(|R|) buckets of 2 tuples, 1 MiB pages)."""
+    nb = build.size // 2
+    buckets = 1 << max(1, (nb - 1).bit_length())
+    x, tb, tt = run_consume(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
+                            build.view(np.uint8), probe.view(np.uint8), threads, fn="modulo", buckets=buckets,
+                            tuplesperbucket=2, prefetch=prefetch)
+    return {"xor32": x, "build_seconds": tb, "seconds": tt, "probe_seconds": tt - tb}
+
+
 def tuple_hash(fn, buckets, coltype, value, range_min=0, range_max=0, offset=0, skipbits=0):
     a = np.array([value], dtype=np.int32 if coltype == "int" else np.int64)
     return lib().ref_tuple_hash(fn.encode(), buckets, CT[coltype], a.ctypes.data_as(C.c_void_p), range_min,
@@ -153,3 +193,22 @@ def run_aggregate(cols, fields, op, sumfield, tuples, threads=4, fn="modulo", bu
             raise RuntimeError("reference aggregate failed")
         ow = w.value
         return out[: n_out.value * ow].reshape(n_out.value, ow), secs.value
+
+
+def main(argv):
+    """python reference_run.py consume BUILD.npy PROBE.npy THREADS PREFETCH
+    One run of the headline plan (consume_pkfk) in a process of its own, so
+    that each timed run starts with the reference allocator's fresh state
+    (its lookaside arenas are used up by the first runs of a process).
+    Prints the result as JSON."""
+    import json
+    if len(argv) != 6 or argv[1] != "consume":
+        raise SystemExit("usage: reference_run.py consume BUILD.npy PROBE.npy THREADS PREFETCH")
+    b = np.load(argv[2], mmap_mode="r")
+    p = np.load(argv[3], mmap_mode="r")
+    print(json.dumps(consume_pkfk(b, p, int(argv[4]), prefetch=int(argv[5]))))
+
+
+if __name__ == "__main__":
+    import sys
+    main(sys.argv)

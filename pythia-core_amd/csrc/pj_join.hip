@@ -1283,7 +1283,7 @@ struct pj_join_s {
     uint64_t raw_nbuild = 0;
     float model_ms = 0.f;
     pj_timings last;
-    hipEvent_t ev[8];
+    hipEvent_t ev[10];   // 0-5 phases, 6-7 the model, 8-9 after each side's histogram
     bool ev_ok = false;
     uint32_t* host_ctr = nullptr;   // pinned: the join's counters, read back after each probe
     DevBuf gj_count, gj_rows;       // group-join: matches a build record, merged rows
@@ -1536,9 +1536,11 @@ PartGeom part_geom(uint64_t n, const PartFn& pf, bool range, bool write_combine)
     return g;
 }
 
+// mark (optional) is recorded on st once the histogram and its scan are
+// enqueued, before the scatter.
 int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in, uint64_t n, const SideLayout& L,
                    const PartFn& pf, bool range, bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts,
-                   bool host_copy, unsigned long long* pminmax, hipStream_t st)
+                   bool host_copy, unsigned long long* pminmax, hipStream_t st, hipEvent_t mark = nullptr)
 {
     int rc;
     const uint32_t F = pf.F;
@@ -1552,6 +1554,8 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
             hipLaunchKernelGGL(k_fill_u32, dim3((F + 1 + 255) / 256), dim3(256), 0, st, (uint32_t*)starts.p, F + 1, 0u);
             PJ_HIP(hipGetLastError());
         }
+        if (mark)
+            PJ_HIP(hipEventRecord(mark, st));
         return PJ_OK;
     }
     if (n >= (1ull << 31))
@@ -1597,6 +1601,8 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
                            (const uint32_t*)(bsum + nsb), (uint32_t*)starts.p);
     }
     PJ_HIP(hipGetLastError());
+    if (mark)
+        PJ_HIP(hipEventRecord(mark, st));
     uint4* out = (uint4*)dst.p;
     switch (B) {
     case 16: rc = launch_wc<16>(rm, range, split, in, n, L, pf, chunk, nblk, hist, out, pid, st); break;
@@ -1614,12 +1620,15 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
     return PJ_OK;
 }
 
+// The build side (&h->R) marks event 8 after its histogram, the probe side 9.
 int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
                    bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts, bool host_copy,
                    unsigned long long* pminmax, hipStream_t st)
 {
+    const hipEvent_t mark =
+        h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING) ? h->ev[&P == &h->R ? 8 : 9] : nullptr;
     return partition_bufs(h->hist, h->bsum, h->pids, d_in, n, L, pf, range, write_combine, dst, P, starts, host_copy,
-                          pminmax, st);
+                          pminmax, st, mark);
 }
 
 int device_count_cus()
@@ -1915,6 +1924,8 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     t.build_partition_ms = elapsed(h, 0, 1) - h->model_ms;
     t.probe_partition_ms = elapsed(h, 2, 3);
     t.join_ms = elapsed(h, 4, 5);
+    t.build_hist_ms = elapsed(h, 0, 8) - h->model_ms;
+    t.probe_hist_ms = elapsed(h, 2, 9);
     t.model_ms = h->model_ms;
     t.total_ms = elapsed(h, 0, 5);
     t.partitions = h->range ? h->plan.ncoarse : 1u << h->bS;
@@ -2112,7 +2123,7 @@ int pj_join_create(const pj_join_desc* desc, pj_join_t* out)
     h->lay = L;
     memset(&h->last, 0, sizeof(h->last));
     h->ev_ok = true;
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < 10; ++i)
         if (hipEventCreate(&h->ev[i]) != hipSuccess)
             h->ev_ok = false;
     // pinned words the probe phase's counters are read back into (without a
@@ -2141,7 +2152,7 @@ int pj_join_destroy(pj_join_t h)
                       &h->pminmax, &h->pids, &h->plan_parts, &h->coarse_of, &h->pairs})
         b->release();
     if (h->ev_ok)
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < 10; ++i)
             (void)hipEventDestroy(h->ev[i]);
     if (h->host_ctr)
         (void)hipHostFree(h->host_ctr);

@@ -45,7 +45,7 @@ class Timings(C.Structure):
                 ("join_ms", C.c_float), ("model_ms", C.c_float), ("total_ms", C.c_float),
                 ("partitions", C.c_uint32), ("subpartitions", C.c_uint32),
                 ("work_items", C.c_uint32), ("used_learned", C.c_uint32),
-                ("spilled_items", C.c_uint64)]
+                ("spilled_items", C.c_uint64), ("build_hist_ms", C.c_float), ("probe_hist_ms", C.c_float)]
 
 
 PJ_MODEL_LEAVES = 4096

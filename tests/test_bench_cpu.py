@@ -42,3 +42,62 @@ def test_bench_refuses_several_gpus(monkeypatch, capsys):
         bench.main()
     assert e.value.code == 2
     assert "one GPU" in capsys.readouterr().err
+
+
+def _tensors(seed, nb, npr, nmiss):
+    torch = pytest.importorskip("torch")
+    b, p = O.gen_pkfk(seed, 1 << 40, nb, npr, nmiss)
+    return torch.from_numpy(b.view(np.uint8).copy()), torch.from_numpy(p.view(np.uint8).copy()), b, p
+
+
+def test_cpu_baseline_full_join_under_consume():
+    import reference_run as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built")
+    nb, npr, nmiss = 1 << 11, 1 << 14, 1000
+    db, dp, b, p = _tensors(8, nb, npr, nmiss)
+    d = O.join_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"], buckets=1 << 11)
+    rows, _ = O.hashjoin(d, b.view(np.uint8), p.view(np.uint8))
+    x = O.checksum(rows, 24)[1]
+    res = bench.cpu_baseline_full(db, dp, nb, npr, x)
+    assert res["kind"] == "reference" and res["value"] > 0 and 1 <= res["cores"] <= 16
+    assert res["parity"]["equals_device_xor32"]
+    assert len(res["runs_seconds"]) == 5 and set(res["seconds_by_prefetch"]) == {"off", "8", "16", "32"}
+    assert res["probe_seconds"] > 0 and res["build_seconds"] > 0 and "nproc" in res["host"]
+
+
+def test_cpu_baseline_sample_counts_the_sample():
+    import reference_run as R
+    if not R.available():
+        pytest.skip("oracle/_ref not built")
+    nb, npr = 1 << 11, 1 << 14
+    db, dp, _, _ = _tensors(9, nb, npr, 0)
+    res = bench.cpu_baseline_sample(db, dp, nb, npr, 1 << 12)
+    assert res["value"] > 0 and "first 4096 of 16384 probe" in res["sample"]
+
+
+def test_step_kernels_are_labelled_in_launch_order():
+    rows = [("void pj::k_gen(...)", 1.0), ("void pj::k_hist<1, false>(...)", 10.0), ("pj::k_scan_small", 0.5),
+            ("void pj::k_scatter<1, false>(...)", 20.0), ("void pj::k_hist<1, false>(...)", 30.0),
+            ("void pj::k_scatter_wc<1, 8, false, 2>(...)", 40.0), ("pj::k_plan", 0.25),
+            ("void pj::k_join<false>(...)", 50.0)]
+    got = bench._label_step_kernels(rows)
+    assert got == {"k_hist build": [10.0], "scans, partition starts, k_plan": [0.5, 0.25],
+                   "k_scatter build": [20.0], "k_hist probe": [30.0], "k_scatter_wc probe": [40.0],
+                   "k_join": [50.0]}
+
+
+def test_traffic_summary_is_used_only_for_its_library(tmp_path, monkeypatch):
+    monkeypatch.setattr(bench, "PROFILES", str(tmp_path))
+    d = {"library_sha256_16": bench.lib_sha(), "kernels": {"k_join": {"read": 1.0, "write": 2.0}}}
+    (tmp_path / "traffic_w.md").write_text("# t\n\n```json\n%s\n```\n" % json_dumps(d))
+    assert bench.read_traffic("w") == d["kernels"]
+    d["library_sha256_16"] = "0" * 16
+    (tmp_path / "traffic_w.md").write_text("```json\n%s\n```\n" % json_dumps(d))
+    assert bench.read_traffic("w") is None
+    assert bench.read_traffic("absent") is None
+
+
+def json_dumps(d):
+    import json
+    return json.dumps(d)

@@ -160,3 +160,14 @@ def test_oracle_char_projection_equals_reference():
     ora, _ = O.hashjoin(O.join_desc(bc, pc, [1], [0], proj, fn="modulo", buckets=128), build, probe)
     assert ref.shape[0] > 5000 and ref.shape[1] == 24
     assert np.array_equal(O.sorted_rows(ref), O.sorted_rows(ora))
+
+
+def test_reference_consume_plan_xor_equals_output_xor():
+    # MergeOp -> ConsumeOp -> HashJoinOp, the CPU baseline's plan: its XOR of
+    # 32-bit words equals the XOR over the rows the reference writes
+    b, p = O.gen_pkfk(78, 1 << 40, 30000, 120000, 9000)
+    for threads in (1, 4):
+        r = RR.consume_pkfk(b, p, threads)
+        rows = RR.run_pkfk(b, p, threads)["rows"]
+        assert r["xor32"] == O.checksum(rows, 24)[1]
+        assert r["xor32"] != 0

@@ -251,13 +251,21 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
     const bool have_pid = RANGE && pid != nullptr;
+    // A split pair reads its input chunk twice: ordinary loads keep the
+    // lines in the XCD's L2 for the partner, where streaming loads would
+    // mark them to be evicted first.
+    auto ld = [&](uint64_t i) -> uint4 {
+        if (RM == REC_IDENT && SPLIT == 2)
+            return reinterpret_cast<const uint4*>(in)[i];
+        return load_rec<RM>(in, i, L);
+    };
     uint4 nxt[IT];   // the next round's records, loaded one round ahead
     uint32_t nxp[IT] = {};
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
         if (i < end) {
-            nxt[k] = load_rec<RM>(in, i, L);
+            nxt[k] = ld(i);
             if (have_pid)
                 nxp[k] = pid[i];
         }
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             }
             const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
             if (i2 < end) {
-                nxt[k] = load_rec<RM>(in, i2, L);
+                nxt[k] = ld(i2);
                 if (have_pid)
                     nxp[k] = pid[i2];
             }

@@ -1,6 +1,6 @@
 // pj_device.h -- device helpers shared by the partition, join and model
 // kernels: the canonical record, streaming loads and stores, and the two
-// partition functions (radix bits of a key mix, or ordered-key ranges).
+// partition functions (radix bits of a key mix, or the learned function).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -87,45 +87,70 @@ __device__ __forceinline__ uint4 load_rec(const uint32_t* __restrict__ in, uint6
     }
 }
 
-// Range partition of ordered key u: the number of the F - 1 ascending
-// boundaries that are <= u, by a fixed-step search over the boundary array
-// padded with ~0 to p2 - 1 entries (p2 a power of two >= F). Every lane runs
-// the same steps, so a wave never diverges; the clamp maps u = ~0 (which
-// also passes the padding) to the last partition.
-__device__ __forceinline__ uint32_t range_part(const uint64_t* bnd, uint32_t F, uint32_t p2, uint64_t u)
+// Radix partition of a key: the top pf.bits bits of the key mix.
+__device__ __forceinline__ uint32_t part_of(uint64_t key, const PartFn& pf)
 {
-    uint32_t pos = 0;
-    for (uint32_t step = p2 >> 1; step; step >>= 1)
-        if (bnd[pos + step - 1] <= u)
-            pos += step;
-    return pos < F - 1 ? pos : F - 1;
-}
-
-// Partition of a key: RANGE selects the ordered-key ranges in lds_bnd,
-// otherwise the top pf.bits bits of the key mix.
-template <bool RANGE>
-__device__ __forceinline__ uint32_t part_of(uint64_t key, const PartFn& pf, const uint64_t* lds_bnd)
-{
-    if (RANGE)
-        return range_part(lds_bnd, pf.F, pf.p2, ord_key(key));
     return radix_part(key_mix(key), pf.bits);
 }
 
-// Copies the range boundaries into LDS, padded with ~0 to p2 - 1 entries (a
-// no-op for radix partitions); the caller synchronises before the first
-// lookup.
-template <bool RANGE>
-__device__ __forceinline__ void load_bounds(uint64_t* lds_bnd, const PartFn& pf)
-{
-    if (RANGE)
-        for (uint32_t i = threadIdx.x; i + 1 < pf.p2; i += blockDim.x)
-            lds_bnd[i] = i + 1 < pf.F ? pf.bnd[i] : ~0ull;
-}
+// The learned partition function over a copy of its blob (pj_internal.h).
+// Every step uses explicitly rounded single-precision operations, so that
+// the build side's and the probe side's kernels, compiled apart, put every
+// key in the same partition.
+struct MapView {
+    const uint8_t* base;
+    __device__ __forceinline__ const MapHdr& hdr() const { return *reinterpret_cast<const MapHdr*>(base); }
+    __device__ __forceinline__ uint64_t seg_lo(uint32_t s) const
+    {
+        return reinterpret_cast<const uint64_t*>(base + kMapOffSegLo)[s];
+    }
+    __device__ __forceinline__ uint64_t seg_cut(uint32_t s) const
+    {
+        return reinterpret_cast<const uint64_t*>(base + kMapOffSegCut)[s];
+    }
+    __device__ __forceinline__ uint32_t fine(uint64_t u) const
+    {
+        const MapHdr& h = hdr();
+        uint32_t s = 0;
+        if (u >= h.lo0) {
+            const uint16_t* dir = reinterpret_cast<const uint16_t*>(base + kMapOffDir);
+            uint64_t d = (u - h.lo0) >> h.dshift;
+            if (d >= kMapDir)
+                d = kMapDir - 1;
+            // the last segment owning keys from at or below u lies in [dir[d], dir[d + 1]]
+            uint32_t lo = dir[d], hi = dir[d + 1];
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (seg_cut(mid) <= u)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            s = lo;
+        }
+        const uint2 meta = reinterpret_cast<const uint2*>(base + kMapOffSegMeta)[s];
+        const uint32_t leaf0 = meta.x & 0xFFFFu, nl = meta.x >> 16, first = meta.y & 0xFFFFu, np = meta.y >> 16;
+        const float inv = reinterpret_cast<const float*>(base + kMapOffSegInv)[s];
+        // position in leaves from the segment's first sampled key (negative before it)
+        const uint64_t slo = seg_lo(s);
+        const float fl = u >= slo ? __fmul_rn(__ull2float_rn(u - slo), inv) : -__fmul_rn(__ull2float_rn(slo - u), inv);
+        const uint32_t j = fl < 0.f ? 0u : fl < (float)nl ? (uint32_t)fl : nl - 1;
+        const float x = __fsub_rn(fl, (float)j);
+        const float2 cd = reinterpret_cast<const float2*>(base + kMapOffLeaf)[leaf0 + j];
+        const float pf = __fmaf_rn(cd.y, x, cd.x);
+        const uint32_t p = pf > 0.f ? (pf < (float)np ? (uint32_t)pf : np - 1) : 0u;
+        return first + (p < np ? p : np - 1);
+    }
+};
 
-// LDS bytes a kernel reserves for range boundaries.
-__host__ __device__ __forceinline__ uint32_t bounds_lds_bytes(const PartFn& pf, bool range)
+// Copies the blob into LDS (the caller synchronises before the first use).
+__device__ __forceinline__ MapView load_map(uint8_t* lds, const uint8_t* map)
 {
-    return range && pf.p2 > 1 ? (pf.p2 - 1) * 8u : 0u;
+    const uint4* src = reinterpret_cast<const uint4*>(map);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (uint32_t i = threadIdx.x; i < kMapBytes / 16; i += blockDim.x)
+        dst[i] = src[i];
+    return MapView{lds};
 }
 
 }  // namespace pj

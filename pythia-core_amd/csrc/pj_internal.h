@@ -144,25 +144,49 @@ struct Item {           // one join work item
     uint64_t base;           // range partitions: smallest ordered build key of the partition
 };
 
+// The learned partition function (pj_learned.hip plans it, pj_device.h
+// evaluates it): the model's root cuts the ordered key range into segments,
+// each segment into leaves of equal key width, and leaf j of segment s
+// carries its fitted line in partition units, so that ordered key u in leaf
+// j at fraction x of the leaf lies in build partition
+//     first[s] + clamp(floor(c[j] + d[j] * x), 0, np[s] - 1).
+// Segment s owns the keys from the middle of the gap before it (cut[s]) to
+// the middle of the gap after it, so keys just outside the sampled ends of
+// a segment stay with it (before its first leaf: x < 0, its first
+// partition; past its last: its last partition). A directory of kMapDir
+// equal slices of [cut[0], cut[last]] gives the segment of u without a
+// search in almost every slice. All of it sits in one blob of kMapBytes at
+// fixed offsets, which the kernels that evaluate the function copy into LDS.
+constexpr uint32_t kMapSegs = PJ_MODEL_MAX_SEGS;
+constexpr uint32_t kMapLeaves = PJ_MODEL_LEAVES;
+constexpr uint32_t kMapDir = 4096;
+struct MapHdr {
+    uint32_t nseg, nleaves, nfine, dshift;   // dshift: slice width log2
+    uint64_t lo0;                            // cut[0], the first segment's first sampled key
+    uint64_t reserved;
+};
+constexpr uint32_t kMapOffSegLo = 32;                                   // uint64 [kMapSegs]: first sampled key
+constexpr uint32_t kMapOffSegCut = kMapOffSegLo + kMapSegs * 8;         // uint64 [kMapSegs]: first key owned
+constexpr uint32_t kMapOffSegInv = kMapOffSegCut + kMapSegs * 8;        // float [kMapSegs]: 1 / leaf width
+constexpr uint32_t kMapOffSegMeta = kMapOffSegInv + kMapSegs * 4;       // uint2 [kMapSegs]: (leaf0 | nl << 16, first | np << 16)
+constexpr uint32_t kMapOffDir = kMapOffSegMeta + kMapSegs * 8;          // uint16 [kMapDir + 1]
+constexpr uint32_t kMapOffLeaf = kMapOffDir + ((kMapDir + 1) * 2 + 15) / 16 * 16;   // float2 [kMapLeaves]: (c, d)
+constexpr uint32_t kMapBytes = kMapOffLeaf + kMapLeaves * 8;
+static_assert(kMapBytes % 16 == 0, "the blob is copied in 16-byte words");
+
 // A partition function: radix (the top `bits` bits of the key mix, F =
-// 2^bits) or ranges of the ordered key (F - 1 ascending boundaries in bnd,
-// a device array; partition p holds bnd[p-1] <= key < bnd[p]; p2 is the
-// power of two >= F the lookup steps over).
+// 2^bits) or the learned function in the blob at map (device memory), its
+// build partition divided by div (probe partitions pair up div build
+// partitions; div = 1 for the build side).
 struct PartFn {
     uint32_t F;
     uint32_t bits;
-    const uint64_t* bnd;
-    uint32_t p2;
+    const uint8_t* map;
+    uint32_t div;
 };
 
-inline PartFn radix_fn(uint32_t bits) { return PartFn{1u << bits, bits, nullptr, 0}; }
-inline PartFn range_fn(uint32_t F, const uint64_t* bnd)
-{
-    uint32_t p2 = 1;
-    while (p2 < F)
-        p2 <<= 1;
-    return PartFn{F, 0, bnd, p2};
-}
+inline PartFn radix_fn(uint32_t bits) { return PartFn{1u << bits, bits, nullptr, 1}; }
+inline PartFn map_fn(uint32_t F, const uint8_t* map, uint32_t div) { return PartFn{F, 0, map, div}; }
 
 // ---------------------------------------------------------------- host
 // A fixed device region that buffers are carved from in order (the
@@ -198,9 +222,9 @@ int radix_records(RadixRecords& r, const void* d_in, uint64_t n, const SideLayou
 struct RangePlan {
     bool ok = false;                 // false: the model declined (radix is used)
     std::string why;                 // reason when declined
-    uint32_t nfine = 0, ncoarse = 0;
-    std::vector<uint64_t> fine;      // nfine - 1 ordered-key boundaries (build partitions)
-    std::vector<uint64_t> coarse;    // ncoarse - 1 boundaries (probe partitions), a subset of fine
+    uint32_t nfine = 0, ncoarse = 0; // build partitions; probe partitions = ceil(nfine / div)
+    uint32_t div = 1;                // build partitions a probe partition serves
+    std::vector<uint8_t> map;        // kMapBytes: the learned partition function's blob
     std::vector<uint32_t> coarse_of; // nfine: probe partition of each build partition
     uint32_t nseg = 0, nleaves = 0;
     float loss_lsq = 0.f, loss = 0.f;

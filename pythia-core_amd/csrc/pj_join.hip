@@ -41,76 +41,12 @@ namespace pj {
 
 constexpr int kHistThreads = 1024;
 constexpr int kHistItems = 8;
-constexpr uint32_t kDirBits = 12;   // range directory: 4096 equal slices of the boundaries' key range
-
-// A directory over the F - 1 ascending range boundaries bnd[0 .. F-2]: the
-// boundaries' key range [bnd[0], bnd[F-2]] cut into 2^kDirBits slices of
-// 2^shift keys, dir[d] = the boundaries below slice d's first key. The
-// partition of key u (the boundaries <= u, as range_part counts them) is
-// then dir[d] plus the few boundaries inside u's slice.
-struct RangeDir {
-    const uint64_t* bnd;
-    const uint32_t* dir;
-    uint64_t umin, umax;
-    uint32_t shift, nb;   // nb = F - 1 boundaries
-    __device__ __forceinline__ uint32_t part(uint64_t u) const
-    {
-        if (nb == 0 || u < umin)
-            return 0;
-        if (u >= umax)
-            return nb;
-        const uint32_t d = (uint32_t)((u - umin) >> shift);
-        uint32_t lo = dir[d], hi = dir[d + 1];   // the boundaries in slice d lie in [lo, hi)
-        while (lo < hi) {                        // first of them > u
-            const uint32_t mid = (lo + hi) >> 1;
-            if (bnd[mid] <= u)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        return lo;
-    }
-};
-
-// Fills dir (2^kDirBits + 1 entries) for the F - 1 boundaries in bnd (LDS,
-// already loaded); the caller synchronises before the first lookup.
-__device__ __forceinline__ RangeDir build_range_dir(const uint64_t* bnd, uint32_t F, uint32_t* dir)
-{
-    RangeDir R;
-    R.bnd = bnd;
-    R.dir = dir;
-    R.nb = F - 1;
-    R.umin = F > 1 ? bnd[0] : 0;
-    R.umax = F > 1 ? bnd[F - 2] : 0;
-    uint32_t sh = 0;
-    while (sh < 64 && ((R.umax - R.umin) >> sh) >= (1ull << kDirBits))
-        ++sh;
-    R.shift = sh;
-    for (uint32_t d = threadIdx.x; d <= (1u << kDirBits); d += blockDim.x) {
-        uint32_t c = R.nb;   // a slice that starts past umax: every boundary lies below it
-        const uint64_t off = (uint64_t)d << sh;
-        if (R.nb && (off >> sh) == d && off <= R.umax - R.umin) {
-            const uint64_t start = R.umin + off;
-            uint32_t lo = 0, hi = R.nb;          // boundaries < start
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (bnd[mid] < start)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            c = lo;
-        }
-        dir[d] = c;
-    }
-    return R;
-}
 
 // hist[p * nblk + blk] = tuples of block blk's chunk in partition p, rounded
-// up to a multiple of pad. With pmin/pmax (range partitions of a build side)
-// it also lowers pmin[p] / raises pmax[p] to the partition's ordered keys;
-// with pid (range partitions) it stores every tuple's partition, so the
-// scatter need not search the boundaries again.
+// up to a multiple of pad. RANGE: partitions of the learned function (its
+// build partition divided by pf.div); with pmin/pmax (a build side) it also
+// lowers pmin[p] / raises pmax[p] to the partition's ordered keys, and it
+// stores every tuple's partition in pid for the scatter.
 template <int RM, bool RANGE>
 __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restrict__ in, uint64_t n,
                                                         SideLayout L, PartFn pf, uint64_t chunk,
@@ -122,12 +58,13 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
     extern __shared__ uint64_t smh[];
     const uint32_t F = pf.F;
     const bool mm = RANGE && pmin != nullptr;
-    uint64_t* bnd = smh;
-    unsigned long long* lmin = reinterpret_cast<unsigned long long*>(bnd + (RANGE ? pf.p2 - 1 : 0));
+    uint8_t* mapl = reinterpret_cast<uint8_t*>(smh);                                          // RANGE: kMapBytes
+    unsigned long long* lmin = reinterpret_cast<unsigned long long*>(mapl + (RANGE ? kMapBytes : 0));
     unsigned long long* lmax = lmin + (mm ? F : 0);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(lmax + (mm ? F : 0));
-    uint32_t* dirw = cnt + F;   // RANGE: 2^kDirBits + 1 directory entries
-    load_bounds<RANGE>(bnd, pf);
+    MapView mv = {nullptr};
+    if (RANGE)
+        mv = load_map(mapl, pf.map);
     for (uint32_t p = threadIdx.x; p < F; p += kHistThreads) {
         cnt[p] = 0;
         if (mm) {
@@ -136,11 +73,6 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
         }
     }
     __syncthreads();
-    RangeDir rdir = {};
-    if (RANGE) {
-        rdir = build_range_dir(bnd, F, dirw);
-        __syncthreads();
-    }
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     for (uint64_t base = beg; base < end; base += (uint64_t)kHistThreads * kHistItems) {
@@ -155,17 +87,17 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
         uint32_t part[kHistItems];
         if (RANGE) {
 #pragma unroll
-            for (int k = 0; k < kHistItems; ++k)
-                part[k] = rdir.part(ord_key(rec_key(r[k])));   // records past the end search harmlessly
+            for (int k = 0; k < kHistItems; ++k)   // records past the end are looked up harmlessly
+                part[k] = mv.fine(ord_key(rec_key(r[k]))) / pf.div;
         }
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
             uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
             if (i < end) {
                 const uint64_t key = rec_key(r[k]);
-                const uint32_t p = RANGE ? part[k] : part_of<false>(key, pf, bnd);
+                const uint32_t p = RANGE ? part[k] : part_of(key, pf);
                 atomicAdd(&cnt[p], 1u);
-                if (RANGE && pid)
+                if (RANGE)
                     pid[i] = (uint16_t)p;
                 if (mm) {
                     atomicMin(&lmin[p], (unsigned long long)ord_key(key));
@@ -237,10 +169,8 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint32_t half = SPLIT == 1 ? 0u : (blockIdx.x / 8) & 1u;
     const uint32_t p0 = half * FL, pn = SPLIT == 1 ? F : (half ? F - FL : FL);
     uint4* buf = sm4;                                                          // FL * B
-    uint64_t* bnd = reinterpret_cast<uint64_t*>(buf + FL * B);                 // p2 - 1 (range)
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));    // FL: buffered records
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + FL * B);                 // FL: buffered records
     uint32_t* goff = cnt + FL;                                                 // FL: next chunk's record index
-    load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < pn; p += kWcThreads) {
         goff[p] = offs[(uint64_t)(p0 + p) * nblk + part];
         cnt[p] = 0;
@@ -250,7 +180,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
-    const bool have_pid = RANGE && pid != nullptr;
+    const bool have_pid = RANGE;   // range partitions come from the histogram pass
     // A split pair reads its input chunk twice: ordinary loads keep the
     // lines in the XCD's L2 for the partner, where streaming loads would
     // mark them to be evicted first.
@@ -281,7 +211,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             p[k] = 0;
             v[k] = false;
             if (i < end) {   // partition (a split block keeps only its half's partitions)
-                const uint32_t pp = have_pid ? nxp[k] : part_of<RANGE>(rec_key(r[k]), pf, bnd);
+                const uint32_t pp = have_pid ? nxp[k] : part_of(rec_key(r[k]), pf);
                 v[k] = SPLIT == 1 || (pp >= FL) == (half != 0);
                 p[k] = pp - p0;
             }
@@ -356,8 +286,8 @@ constexpr int kScatterItems = 16;
 // Scatter with an LDS-sorted tile, for fan-outs too wide for one LDS buffer
 // per partition: each tile is counting-sorted by partition in LDS, then
 // written so that lanes of a wave store consecutive records of a partition.
-// Range partitions keep each tile record's partition in LDS (tpart) and take
-// it from the histogram pass (pid) when given.
+// Range partitions take each record's partition from the histogram pass
+// (pid) and keep it in LDS (tpart).
 template <int RM, bool RANGE>
 __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __restrict__ in, uint64_t n,
                                                               SideLayout L, PartFn pf, uint64_t chunk,
@@ -368,13 +298,10 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
     extern __shared__ uint4 sm4[];
     const uint32_t F = pf.F;
     uint4* tile = sm4;
-    uint64_t* bnd = reinterpret_cast<uint64_t*>(tile + ts);                       // p2 - 1 (range)
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(bnd + (RANGE ? pf.p2 - 1 : 0));   // F + 1
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(tile + ts);                       // F + 1
     uint32_t* goff = cnt + F + 1;                                                 // F
     uint32_t* wsum = goff + F;                                                    // 16
     uint16_t* tpart = reinterpret_cast<uint16_t*>(wsum + 16);                     // ts (range)
-    if (!(RANGE && pid))
-        load_bounds<RANGE>(bnd, pf);
     for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
         goff[p] = offs[(uint64_t)p * nblk + blockIdx.x];
     const uint64_t beg = (uint64_t)blockIdx.x * chunk;
@@ -391,7 +318,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
             uint32_t i = k * kScatterThreads + threadIdx.x;
             if (i < tn) {
                 r[k] = load_rec<RM>(in, t0 + i, L);
-                if (RANGE && pid)
+                if (RANGE)
                     part[k] = pid[t0 + i];
             }
         }
@@ -399,8 +326,8 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
         for (int k = 0; k < kScatterItems; ++k) {
             uint32_t i = k * kScatterThreads + threadIdx.x;
             if (i < tn) {
-                if (!(RANGE && pid))
-                    part[k] = part_of<RANGE>(rec_key(r[k]), pf, bnd);
+                if (!RANGE)
+                    part[k] = part_of(rec_key(r[k]), pf);
                 rank[k] = atomicAdd(&cnt[part[k]], 1u);
             }
         }
@@ -419,7 +346,7 @@ __global__ __launch_bounds__(kScatterThreads) void k_scatter(const uint32_t* __r
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < tn; i += kScatterThreads) {
             uint4 v = tile[i];
-            const uint32_t p = RANGE ? (uint32_t)tpart[i] : part_of<false>(rec_key(v), pf, bnd);
+            const uint32_t p = RANGE ? (uint32_t)tpart[i] : part_of(rec_key(v), pf);
             out[goff[p] + (i - cnt[p])] = v;
         }
         __syncthreads();
@@ -1285,7 +1212,7 @@ struct pj_join_s {
     bool range = false;
     RangePlan plan;
     LearnedCtx lctx;
-    DevBuf fine_bnd, coarse_bnd, pminmax, pids, plan_parts, coarse_of;
+    DevBuf map_dev, pminmax, pids, plan_parts, coarse_of;
     std::vector<uint64_t> pmin, pmax;   // per build partition, ordered keys
     const void* raw_build = nullptr;    // device build tuples of the current run (radix re-run)
     uint64_t raw_nbuild = 0;
@@ -1318,9 +1245,9 @@ namespace {
 
 constexpr uint32_t kMaxCoarse = 2048;   // probe partitions of a range plan (write-combining fan-out)
 
-uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes, bool range)
+uint32_t scatter_tile(uint32_t F, bool range)
 {
-    const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4 - bnd_bytes;
+    const int64_t avail = (int64_t)kLdsBytes - 256 - (int64_t)(2 * F + 1) * 4 - 16 * 4;
     int64_t ts = avail / (range ? 18 : 16);
     ts = (ts / kScatterThreads) * kScatterThreads;
     ts = std::min<int64_t>(ts, (int64_t)kScatterThreads * kScatterItems);
@@ -1328,15 +1255,14 @@ uint32_t scatter_tile(uint32_t F, uint32_t bnd_bytes, bool range)
 }
 
 // Records per write-combining chunk for a fan-out of F buffered by one block:
-// the largest power of two (at most 16) whose buffers fit in 128 KiB of LDS,
-// and next to the range boundaries in the LDS that is left; 1 selects the
-// tile-sorted scatter (no padding).
-uint32_t wc_chunk(uint32_t F, uint32_t bnd_bytes)
+// the largest power of two (at most 16) whose buffers fit in 128 KiB of LDS;
+// 1 selects the tile-sorted scatter (no padding).
+uint32_t wc_chunk(uint32_t F)
 {
     if (F <= 1)
         return 1;
     uint32_t B = 16;
-    while (B > 1 && ((uint64_t)F * B * 16 > 131072 || (uint64_t)F * B * 16 + 8ull * F + bnd_bytes > kLdsBytes))
+    while (B > 1 && ((uint64_t)F * B * 16 > 131072 || (uint64_t)F * B * 16 + 8ull * F > kLdsBytes))
         B >>= 1;
     return B < 2 ? 1 : B;
 }
@@ -1345,9 +1271,9 @@ uint32_t wc_chunk(uint32_t F, uint32_t bnd_bytes)
 // could only buffer chunks of fewer than 8 records (64 bytes), which HBM
 // writes at half the rate of whole 128-byte lines, and two blocks of half
 // the partitions each reach 8.
-uint32_t wc_split(uint32_t F, uint32_t bnd_bytes)
+uint32_t wc_split(uint32_t F)
 {
-    return F > 1 && wc_chunk(F, bnd_bytes) < 8 && wc_chunk((F + 1) / 2, bnd_bytes) >= 8 ? 2u : 1u;
+    return F > 1 && wc_chunk(F) < 8 && wc_chunk((F + 1) / 2) >= 8 ? 2u : 1u;
 }
 
 // Blocks of a tile-sorted partition pass: about 64K tuples each for large
@@ -1411,7 +1337,7 @@ int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, con
     const uint32_t FL = SPLIT == 1 ? pf.F : (pf.F + 1) / 2;
     if (SPLIT > 1 && nblk % 8)
         return fail(PJ_ERR_INVALID, "split scatter needs a multiple of 8 input chunks");
-    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4 + bounds_lds_bytes(pf, RANGE);
+    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4;
     uint4 pad_other, pad_zero;
     if (RANGE) {
         // ordered key 0 lies in partition 0, ordered key ~0 in the last one
@@ -1462,8 +1388,7 @@ int launch_hist(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, con
                 uint32_t nblk, uint32_t pad, uint32_t* hist, unsigned long long* pmin, unsigned long long* pmax,
                 uint16_t* pid, hipStream_t st)
 {
-    const size_t lds = (size_t)pf.F * 4 + bounds_lds_bytes(pf, RANGE) + (pmin ? (size_t)pf.F * 16 : 0) +
-                       (RANGE ? ((1u << kDirBits) + 1) * 4 : 0);
+    const size_t lds = (size_t)pf.F * 4 + (RANGE ? kMapBytes : 0) + (pmin ? (size_t)pf.F * 16 : 0);
     if (lds > kLdsBytes)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the histogram's LDS budget", pf.F);
     if (rm == REC_IDENT)
@@ -1491,11 +1416,10 @@ int launch_tile_scatter(int rm, const uint32_t* in, uint64_t n, const SideLayout
                         uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
                         hipStream_t st)
 {
-    const uint32_t bb = bounds_lds_bytes(pf, RANGE);
-    const uint32_t ts = scatter_tile(pf.F, bb, RANGE);
+    const uint32_t ts = scatter_tile(pf.F, RANGE);
     if (ts == 0)
         return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the scatter's LDS budget", pf.F);
-    const size_t lds = (size_t)ts * (RANGE ? 18 : 16) + bb + (2 * pf.F + 1) * 4 + 16 * 4;
+    const size_t lds = (size_t)ts * (RANGE ? 18 : 16) + (2 * pf.F + 1) * 4 + 16 * 4;
     if (rm == REC_IDENT)
         return launch_tile_rm<REC_IDENT, RANGE>(in, n, L, pf, chunk, nblk, offs, out, ts, pid, lds, st);
     if (RANGE || rm == REC_WORDS)
@@ -1531,9 +1455,8 @@ PartGeom part_geom(uint64_t n, const PartFn& pf, bool range, bool write_combine)
 {
     PartGeom g;
     const uint32_t F = pf.F;
-    const uint32_t bb = bounds_lds_bytes(pf, range);
-    g.split = write_combine ? wc_split(F, bb) : 1;
-    g.B = write_combine ? wc_chunk(g.split > 1 ? (F + 1) / 2 : F, bb) : 1;
+    g.split = write_combine ? wc_split(F) : 1;
+    g.B = write_combine ? wc_chunk(g.split > 1 ? (F + 1) / 2 : F) : 1;
     g.nblk = g.B > 1 ? part_blocks(n) : hist_blocks(n);
     if (g.split > 1)   // input chunks pair up blocks b and b + 8 (one XCD)
         g.nblk = (g.nblk + 7) / 8 * 8;
@@ -1588,8 +1511,10 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
         PJ_HIP(hipMemsetAsync(pmin, 0xFF, (size_t)F * 8, st));
         PJ_HIP(hipMemsetAsync(pmax, 0, (size_t)F * 8, st));
     }
-    uint16_t* pid = nullptr;
-    if (range && F <= 65536) {
+    uint16_t* pid = nullptr;   // range partitions: each tuple's, from the histogram to the scatter
+    if (range) {
+        if (F > 65536 || !pf.map)
+            return fail(PJ_ERR_INVALID, "range partitions need the learned function and at most 65536 partitions");
         if ((rc = pidsb.ensure(n * 2)))
             return rc;
         pid = (uint16_t*)pidsb.p;
@@ -1726,15 +1651,10 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
     if (!h->plan.ok)
         return PJ_OK;
     const RangePlan& pl = h->plan;
-    if ((rc = h->fine_bnd.ensure(std::max<size_t>(pl.fine.size(), 1) * 8)) ||
-        (rc = h->coarse_bnd.ensure(std::max<size_t>(pl.coarse.size(), 1) * 8)) ||
-        (rc = h->pminmax.ensure((size_t)pl.nfine * 16)))
+    if ((rc = h->map_dev.ensure(kMapBytes)) || (rc = h->pminmax.ensure((size_t)pl.nfine * 16)))
         return rc;
-    if (!pl.fine.empty())
-        PJ_HIP(hipMemcpyAsync(h->fine_bnd.p, pl.fine.data(), pl.fine.size() * 8, hipMemcpyHostToDevice, st));
-    if (!pl.coarse.empty())
-        PJ_HIP(hipMemcpyAsync(h->coarse_bnd.p, pl.coarse.data(), pl.coarse.size() * 8, hipMemcpyHostToDevice, st));
-    const PartFn pf = range_fn(pl.nfine, (const uint64_t*)h->fine_bnd.p);
+    PJ_HIP(hipMemcpyAsync(h->map_dev.p, pl.map.data(), kMapBytes, hipMemcpyHostToDevice, st));
+    const PartFn pf = map_fn(pl.nfine, (const uint8_t*)h->map_dev.p, 1);
     if ((rc = partition_side(h, d_build, nb, h->lay.build, pf, true, false, h->rpart, h->R, h->rstarts, true,
                              (unsigned long long*)h->pminmax.p, st)))
         return rc;
@@ -1756,8 +1676,10 @@ int prepare_range(pj_join_s* h, const void* d_build, uint64_t nb, hipStream_t st
         PlanPart& q = parts[f];
         q.fmt = fmt;
         q.cap = fmt == kFmtDense ? kDenseCap : kCodeCap;
-        q.lo = f ? pl.fine[f - 1] : 0ull;
-        q.hi = f + 1 < pl.nfine ? pl.fine[f] - 1 : ~0ull;
+        // the partition's build keys span [pmin, pmax]: a probe key outside
+        // it cannot be one of them (a key's partition is a function of the key)
+        q.lo = h->pmin[f];
+        q.hi = h->pmax[f];
         q.base = h->pmin[f];
     }
     if ((rc = h->plan_parts.ensure(parts.size() * sizeof(PlanPart))) ||
@@ -1816,7 +1738,7 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     *empty = false;
     record(h, 2, st);
     if (h->range) {
-        const PartFn pf = range_fn(h->plan.ncoarse, (const uint64_t*)h->coarse_bnd.p);
+        const PartFn pf = map_fn(h->plan.ncoarse, (const uint8_t*)h->map_dev.p, h->plan.div);
         rc = partition_side(h, d_probe, np, h->lay.probe, pf, true, true, h->spart, h->S, h->sstarts, false, nullptr,
                             st);
     } else {
@@ -2156,7 +2078,7 @@ int pj_join_destroy(pj_join_t h)
     if (!h)
         return PJ_OK;
     for (DevBuf* b : {&h->d_raw_build, &h->d_raw_probe, &h->d_out_batch, &h->rpart, &h->spart, &h->hist, &h->bsum,
-                      &h->rstarts, &h->sstarts, &h->planoffs, &h->counters, &h->fine_bnd, &h->coarse_bnd,
+                      &h->rstarts, &h->sstarts, &h->planoffs, &h->counters, &h->map_dev,
                       &h->pminmax, &h->pids, &h->plan_parts, &h->coarse_of, &h->pairs})
         b->release();
     if (h->ev_ok)

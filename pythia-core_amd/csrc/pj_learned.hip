@@ -732,6 +732,107 @@ void plan_bounds(const pj_model& M, uint32_t max_coarse, std::vector<uint64_t>& 
     *div_out = std::max<uint32_t>(1, ((uint32_t)out.size() + 1 + max_coarse - 1) / max_coarse);
 }
 
+// The learned partition function from the model (its layout: pj_internal.h).
+// Every segment gets partitions of its own, as many as its predicted keys
+// fill at the planned fill (a key costs 1/kDenseCap of a table in a segment
+// whose sample density says every key value is present, 1/kCodeCap
+// elsewhere), and leaf j's line in partition units continues the previous
+// leaf's: c[j] is the cost of the segment's leaves before j, d[j] leaf j's
+// own, both scaled to the segment's partitions. The fill is the lowest in
+// [kFill, kFillMax] that keeps the fewest probe re-reads (div: build
+// partitions a probe partition serves) within max_coarse probe partitions.
+// Returns false with plan->why when the partitions do not fit.
+bool plan_map(const pj_model& M, uint32_t max_coarse, RangePlan* plan)
+{
+    const uint32_t L = M.nleaves, S = M.nseg;
+    const double scale = M.n ? (double)M.n_keys / (double)M.n : 0.0;   // keys per sample
+    std::vector<double> cost(L, 0.0), segc(S, 0.0);                       // tables at a fill of 1
+    for (uint32_t s = 0; s < S; ++s) {
+        const double seg_keys = (double)(M.leaf_start[M.seg_leaf0[s + 1]] - M.leaf_start[M.seg_leaf0[s]]) * scale;
+        const double span = (double)(M.seg_hi[s] - M.seg_lo[s]) + 1.0;
+        const double cpk = seg_keys / span >= kDenseDensity ? 1.0 / kDenseCap : 1.0 / kCodeCap;
+        for (uint32_t j = M.seg_leaf0[s]; j < M.seg_leaf0[s + 1]; ++j) {
+            cost[j] = std::max(0.0, (double)M.b[j]) * scale * cpk;
+            segc[s] += cost[j];
+        }
+    }
+    auto parts = [&](double fill) {
+        uint64_t t = 0;
+        for (uint32_t s = 0; s < S; ++s)
+            t += std::max<uint64_t>(1, (uint64_t)std::ceil(segc[s] / fill));
+        return t;
+    };
+    const uint64_t nmin = parts(kFillMax);
+    if (nmin > kMaxFine) {
+        plan->why = "too many partitions";
+        return false;
+    }
+    const uint32_t div = (uint32_t)std::max<uint64_t>(1, (nmin + max_coarse - 1) / max_coarse);
+    const uint64_t room = std::min<uint64_t>((uint64_t)div * max_coarse, kMaxFine);
+    double fill = kFill;
+    while (fill < kFillMax && parts(fill) > room)
+        fill += 0.005;
+    if (parts(fill) > room)
+        fill = kFillMax;
+    plan->map.assign(kMapBytes, 0);
+    uint8_t* m = plan->map.data();
+    MapHdr* hdr = reinterpret_cast<MapHdr*>(m);
+    uint64_t* seg_lo = reinterpret_cast<uint64_t*>(m + kMapOffSegLo);
+    uint64_t* seg_cut = reinterpret_cast<uint64_t*>(m + kMapOffSegCut);
+    float* seg_inv = reinterpret_cast<float*>(m + kMapOffSegInv);
+    uint32_t* seg_meta = reinterpret_cast<uint32_t*>(m + kMapOffSegMeta);
+    uint16_t* dir = reinterpret_cast<uint16_t*>(m + kMapOffDir);
+    float* leaf = reinterpret_cast<float*>(m + kMapOffLeaf);
+    uint32_t first = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+        const uint32_t np = (uint32_t)std::max<uint64_t>(1, (uint64_t)std::ceil(segc[s] / fill));
+        const uint32_t l0 = M.seg_leaf0[s], nl = M.seg_leaf0[s + 1] - l0;
+        const double k = segc[s] > 0.0 ? (double)np / segc[s] : 0.0;   // partitions per table of cost
+        double acc = 0.0;
+        for (uint32_t j = l0; j < l0 + nl; ++j) {
+            leaf[2 * j] = (float)(acc * k);
+            leaf[2 * j + 1] = (float)(cost[j] * k);
+            acc += cost[j];
+        }
+        seg_lo[s] = M.seg_lo[s];
+        // the middle of the gap before the segment (segments are cut at gaps wider than 2^32)
+        seg_cut[s] = s ? M.seg_hi[s - 1] + (M.seg_lo[s] - M.seg_hi[s - 1]) / 2 + 1 : M.seg_lo[0];
+        seg_inv[s] = (float)(1.0 / (double)M.seg_lw[s]);
+        seg_meta[2 * s] = l0 | (nl << 16);
+        seg_meta[2 * s + 1] = first | (np << 16);
+        first += np;
+    }
+    hdr->nseg = S;
+    hdr->nleaves = L;
+    hdr->nfine = first;
+    hdr->lo0 = seg_cut[0];
+    const uint64_t range = seg_cut[S - 1] - seg_cut[0];
+    uint32_t sh = 0;
+    while (sh < 63 && (range >> sh) >= kMapDir)
+        ++sh;
+    hdr->dshift = sh;
+    for (uint32_t d = 0; d <= kMapDir; ++d) {   // the last segment starting at or below slice d's first key
+        const uint64_t off = (uint64_t)d << sh;
+        uint32_t s = S - 1;
+        if ((off >> sh) == d && off <= range) {
+            const uint64_t start = seg_cut[0] + off;
+            s = (uint32_t)(std::upper_bound(seg_cut, seg_cut + S, start) - seg_cut) - 1;
+        }
+        dir[d] = (uint16_t)s;
+    }
+    plan->nfine = first;
+    plan->div = div;
+    plan->ncoarse = (first + div - 1) / div;
+    if (plan->ncoarse > max_coarse || first > kMaxFine || first < 4) {
+        plan->why = first < 4 ? "too few partitions" : "too many partitions";
+        return false;
+    }
+    plan->coarse_of.resize(first);
+    for (uint32_t f = 0; f < first; ++f)
+        plan->coarse_of[f] = f / div;
+    return true;
+}
+
 }  // namespace
 
 int learned_plan(LearnedCtx& ctx, const void* d_build, uint64_t nb, const SideLayout& Ls, uint32_t max_coarse,
@@ -768,20 +869,8 @@ int learned_plan(LearnedCtx& ctx, const void* d_build, uint64_t nb, const SideLa
         plan->why = "key domain too fragmented for the root";
         return PJ_OK;
     }
-    uint32_t div = 1;
-    plan_bounds(*M, max_coarse, plan->fine, &div);
-    plan->nfine = (uint32_t)plan->fine.size() + 1;
-    plan->ncoarse = (plan->nfine + div - 1) / div;
-    if (plan->ncoarse > max_coarse || plan->nfine > kMaxFine) {
-        plan->why = "too many partitions";
+    if (!plan_map(*M, max_coarse, plan))
         return PJ_OK;
-    }
-    plan->coarse.clear();
-    for (uint32_t q = 1; q < plan->ncoarse; ++q)
-        plan->coarse.push_back(plan->fine[q * div - 1]);
-    plan->coarse_of.resize(plan->nfine);
-    for (uint32_t fi = 0; fi < plan->nfine; ++fi)
-        plan->coarse_of[fi] = fi / div;
     plan->nseg = M->nseg;
     plan->nleaves = M->nleaves;
     plan->loss_lsq = M->loss_lsq;

@@ -185,6 +185,41 @@ int pj_hash_buckets(const pj_hash_spec* spec, uint32_t* buckets)
     return PJ_OK;
 }
 
+// ParameterizedModuloValueHasher::generate (hash.cpp:251-281): t = the bits
+// of spec's buckets (getlogarithm(buckets() - 1)); passes - 1 functions of
+// t / passes bits each, from the top of the mask down, then one of the bits
+// left over at skipbits. The reference's unsigned loop bound passes - 1 has
+// no meaning at passes = 0, and a one-bucket function has no bits to split
+// (getlogarithm(0) is 32 there); both are refused.
+int pj_hash_generate(const pj_hash_spec* spec, uint32_t passes, pj_hash_spec* out)
+{
+    if (!spec || !out)
+        return fail(PJ_ERR_INVALID, "null argument");
+    if (spec->fn != PJ_HASH_PARAMMODULO)
+        return fail(PJ_ERR_INVALID, "generate is defined for parammodulo functions only");
+    if (passes == 0)
+        return fail(PJ_ERR_INVALID, "generate needs at least one pass");
+    HashParams h;
+    int rc = hash_params(spec, &h);
+    if (rc)
+        return rc;
+    if (h.buckets < 2)
+        return fail(PJ_ERR_INVALID, "a one-bucket function has no bits to split");
+    const uint32_t total = log2_ceil(h.buckets - 1);
+    uint32_t per = total / passes;
+    for (uint32_t i = 0; i < passes; ++i) {
+        out[i] = *spec;
+        if (i + 1 < passes) {
+            out[i].buckets = 1u << per;
+            out[i].skipbits = spec->skipbits + total - (i + 1) * per;
+        } else {
+            out[i].buckets = 1u << (total - (passes - 1) * per);
+            out[i].skipbits = spec->skipbits;
+        }
+    }
+    return PJ_OK;
+}
+
 int pj_tuple_hash_device(const pj_hash_spec* spec, const void* d_tuples, uint64_t n, uint32_t tuple_width,
                          uint32_t key_offset, uint32_t key_size, uint32_t* d_buckets, void* stream)
 {

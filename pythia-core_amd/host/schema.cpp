@@ -357,19 +357,4 @@ HashSpec make_hashspec(const Setting& node)
     return h;
 }
 
-std::vector<std::pair<unsigned, unsigned>> parammodulo_generate(unsigned buckets, unsigned skipbits,
-                                                                unsigned passes)
-{
-    std::vector<std::pair<unsigned, unsigned>> r;
-    if (passes == 0)
-        return r;
-    const unsigned total = getlogarithm((1u << getlogarithm(buckets)) - 1);
-    unsigned per = total / passes;
-    for (unsigned i = 0; i + 1 < passes; ++i)
-        r.push_back({1u << per, skipbits + total - (i + 1) * per});
-    per = total - (passes - 1) * per;
-    r.push_back({1u << per, skipbits});
-    return r;
-}
-
 }  // namespace pjh

@@ -50,6 +50,65 @@ def test_bad_specs_are_rejected():
         assert P.hash_buckets(P.hash_spec(fn, 8, **kw)) == O.lib().or_hasher_buckets(C.byref(h)) == 8
 
 
+def test_generate_equals_the_reference_test_and_the_oracle():
+    # unit_tests/testhash.cpp:53-69: parammodulo(0, 1024, skipbits 2).generate(3)
+    # gives functions of 8, 8 and 16 buckets
+    g = P.hash_generate(P.hash_spec("parammodulo", 1024, skipbits=2), 3)
+    assert [P.hash_buckets(f) for f in g] == [8, 8, 16]
+    for buckets in (2, 3, 100, 1024, 1 << 16, 1 << 31):
+        for skip in (0, 2, 9):
+            for passes in (1, 2, 3, 4, 7):
+                g = P.hash_generate(P.hash_spec("parammodulo", buckets, offset=11, skipbits=skip), passes)
+                b = (C.c_uint * passes)()
+                s = (C.c_uint * passes)()
+                assert O.lib().or_parammodulo_generate(11, buckets, skip, passes, b, s) == passes
+                assert [P.hash_buckets(f) for f in g] == list(b)
+                assert [f.skipbits for f in g] == list(s)
+                assert all(f.offset == 11 and f.fn == P.HASH_FNS["parammodulo"] for f in g)
+                # the passes' masks are disjoint and cover the function's, except
+                # at 2 buckets, where getlogarithm(buckets() - 1) = getlogarithm(1)
+                # is 0 and the reference's split keeps no bit
+                masks = [(P.hash_buckets(f) - 1) << f.skipbits for f in g]
+                whole = (P.hash_buckets(P.hash_spec("parammodulo", buckets, skipbits=skip)) - 1) << skip
+                if buckets == 2:
+                    whole = 0
+                assert sum(masks) == whole and all(a & c == 0 for i, a in enumerate(masks) for c in masks[i + 1:])
+
+
+def test_generate_rejects_what_has_no_split():
+    for spec, passes in ((P.hash_spec("parammodulo", 1024), 0), (P.hash_spec("parammodulo", 1), 2),
+                         (P.hash_spec("modulo", 1024), 2), (P.hash_spec("parammodulo", 0), 2)):
+        with pytest.raises(P.PJError) as e:
+            P.hash_generate(spec, passes)
+        assert e.value.code == P.PJ_ERR_INVALID
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("passes", [2, 3])
+def test_generated_passes_concatenate_to_the_function(passes):
+    """A two- or three-pass radix split on the device: each pass's bucket ids
+    of the same tuples, most significant first, give the one-pass ids."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(passes)
+    n = 1 << 16
+    keys = rng.integers(-(1 << 40), 1 << 40, n, dtype=np.int64)
+    d = torch.from_numpy(keys.view(np.uint8).copy()).cuda()
+    whole = P.hash_spec("parammodulo", 1 << 14, offset=3, skipbits=4)
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    P.tuple_hash_device(whole, d.data_ptr(), n, 8, 0, 8, out.data_ptr())
+    exp = out.cpu().numpy().view(np.uint32).copy()
+    acc = np.zeros(n, np.uint64)
+    for f in P.hash_generate(whole, passes):
+        P.tuple_hash_device(f, d.data_ptr(), n, 8, 0, 8, out.data_ptr())
+        part = out.cpu().numpy().view(np.uint32).astype(np.uint64)
+        assert part.max() < P.hash_buckets(f)
+        acc = acc * np.uint64(P.hash_buckets(f)) + part
+    assert np.array_equal(acc.astype(np.uint32), exp)
+    h = _oracle_hasher("parammodulo", 1 << 14, dict(offset=3, skipbits=4))
+    sample = rng.integers(0, n, 500)
+    assert all(O.lib().or_hasher_hash(C.byref(h), keys[i:i + 1].ctypes.data_as(C.c_void_p), 8) == exp[i] for i in sample)
+
+
 def _keys(fn, kw, n, rng):
     if fn in ("range", "exactrange"):
         return rng.integers(kw["range_min"], kw["range_max"] + 1, n)

@@ -103,6 +103,16 @@ typedef struct {
 
 const char* pj_last_error(void);
 const char* pj_version(void);
+/* A limit on the device memory the library's own buffers may hold in this
+ * process (0, the default: none). No counterpart in the reference, whose
+ * allocations fail only when the machine runs out (numaallocate.cpp). A
+ * buffer that would pass it fails as a failed hipMalloc does: PJ_ERR_DEVICE
+ * from the call that needed it, which a plan operator turns into Error.
+ * Buffers held already are not given back. */
+int pj_set_device_limit(uint64_t bytes);
+/* The device memory the library's buffers hold now, in bytes (scratch of
+ * the no-allocation entry point and caller-owned memory not counted). */
+int pj_device_bytes(uint64_t* held);
 
 /* Replaces HashJoinOp::init (operators/join.cpp:367-518): validates the
  * schemas, key columns and projection; computes the output schema. */
@@ -472,6 +482,12 @@ typedef struct {
 
 /* HashFunction::buckets() of the function spec builds. */
 int pj_hash_buckets(const pj_hash_spec* spec, uint32_t* buckets);
+/* ParameterizedModuloValueHasher::generate (hash.cpp:251-281): the passes
+ * parammodulo functions of a multi-pass radix split of spec (a parammodulo
+ * function of 2 buckets or more) into out[0 .. passes-1], high bits first.
+ * A tuple's bucket ids under them, most significant first, concatenate to its
+ * bucket under spec. PJ_ERR_INVALID for passes = 0 or another function. */
+int pj_hash_generate(const pj_hash_spec* spec, uint32_t passes, pj_hash_spec* out);
 /* d_buckets[i] = the bucket of tuple i (n tuples of tuple_width bytes at
  * d_tuples, device memory). Value hashers read the key_size (4: INTEGER,
  * sign-extended; 8: LONG or DATE) bytes at key_offset; the byte hasher

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1137,6 +1138,11 @@ int derive_layout(const pj_join_desc* d, Layout* L)
     return rowid_layout(d, boff, poff, bsize, psize, L);
 }
 
+// The device memory the library's buffers hold, and the process's limit on
+// it (0: none). A buffer that would pass the limit fails as a failed
+// hipMalloc does.
+std::atomic<uint64_t> g_dev_used{0}, g_dev_limit{0};
+
 int DevBuf::ensure(size_t bytes)
 {
     if (bytes <= cap)
@@ -1152,23 +1158,27 @@ int DevBuf::ensure(size_t bytes)
         arena->used = at + want;
         return PJ_OK;
     }
-    if (p)
-        (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
+    release();
+    const uint64_t lim = g_dev_limit.load();
+    if (lim && g_dev_used.load() + want > lim)
+        return fail(PJ_ERR_DEVICE, "a device buffer of %zu bytes would pass the limit of %llu bytes (%llu held)",
+                    want, (unsigned long long)lim, (unsigned long long)g_dev_used.load());
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
         p = nullptr;
         return fail(PJ_ERR_DEVICE, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
     cap = want;
+    g_dev_used += want;
     return PJ_OK;
 }
 
 void DevBuf::release()
 {
-    if (p && !arena)
+    if (p && !arena) {
         (void)hipFree(p);
+        g_dev_used -= cap;
+    }
     p = nullptr;
     cap = 0;
 }
@@ -2034,6 +2044,20 @@ extern "C" {
 const char* pj_last_error(void) { return pj::last_error(); }
 void pj_set_error(const char* msg) { pj::set_error("%s", msg ? msg : ""); }
 const char* pj_version(void) { return "pjoin 0.1 gfx950"; }
+
+int pj_set_device_limit(uint64_t bytes)
+{
+    pj::g_dev_limit.store(bytes);
+    return PJ_OK;
+}
+
+int pj_device_bytes(uint64_t* held)
+{
+    if (!held)
+        return fail(PJ_ERR_INVALID, "null argument");
+    *held = pj::g_dev_used.load();
+    return PJ_OK;
+}
 
 int pj_join_create(const pj_join_desc* desc, pj_join_t* out)
 {

@@ -38,6 +38,11 @@ void MemScanOp::init(const Setting& root, const Setting& node)
     const unsigned w = schema.getTupleSize();
     per_page = buffsize / w ? buffsize / w : 1;
     pages.assign(MAX_THREADS, Page());
+    served.assign(MAX_THREADS, 0);
+    if (node.exists("errorthread")) {
+        error_thread = (int)node["errorthread"].asInt();
+        error_after = node.exists("errorafter") ? (unsigned)node["errorafter"].asInt() : 0;
+    }
 }
 
 void MemScanOp::threadInit(unsigned short tid)
@@ -48,6 +53,8 @@ void MemScanOp::threadInit(unsigned short tid)
 Operator::GetNextResultT MemScanOp::getNext(unsigned short tid)
 {
     const unsigned w = schema.getTupleSize();
+    if ((int)tid == error_thread && served[tid]++ >= error_after)
+        return {Error, nullptr};
     const uint64_t start = __sync_fetch_and_add(&cursor, per_page);
     if (start >= count) {
         pages[tid] = Page(nullptr, 0, w, true);
@@ -291,6 +298,7 @@ void HashJoinOp::init(const Setting& root, const Setting& node)
     }
     state.resize(MAX_THREADS);
     build_rc.assign(leader_of_group.size(), PJ_OK);
+    build_failed.reset(leader_of_group.size());
 }
 
 bool HashJoinOp::oneGroupOf(unsigned n) const
@@ -322,29 +330,30 @@ Operator::ResultCode HashJoinOp::scanStart(unsigned short tid)
     pj_join_t h = handles.at(g);
     ResultCode rc = Ready;
     if (isLeftThread(tid)) {
-        if (buildOp->scanStart(tid) == Error)
-            rc = Error;
+        bool failed = buildOp->scanStart(tid) == Error;
         GetNextResultT r(Ready, nullptr);
-        while (rc != Error && r.first == Ready) {
+        while (!failed && r.first == Ready) {
             r = buildOp->getNext(tid);
             if (r.first == Error) {
-                rc = Error;
+                failed = true;
                 break;
             }
             if (r.second && r.second->getNumTuples())
                 if (pj_join_build_append(h, r.second->data(), r.second->getNumTuples()))
-                    rc = Error;
+                    failed = true;
         }
-        if (rc != Error && buildOp->scanStop(tid) == Error)
-            rc = Error;
+        if (!failed && buildOp->scanStop(tid) == Error)
+            failed = true;
+        if (failed)
+            build_failed.raise(g);
     }
     if (probeOp->scanStart(tid) == Error)
         rc = Error;
-    barriers[g]->arrive();   // every build page of the group is staged
+    barriers[g]->arrive();   // every build page of the group is staged, or a build thread failed
     if (tid == leader_of_group[g])
-        build_rc[g] = pj_join_build_finish(h);
+        build_rc[g] = build_failed.raised(g) ? PJ_ERR_STATE : pj_join_build_finish(h);
     barriers[g]->arrive();
-    if (rc == Error || build_rc[g] != PJ_OK)
+    if (rc == Error || build_failed.raised(g) || build_rc[g] != PJ_OK)
         return Error;
     // position on the first probe page; an empty probe finishes here
     // (join.cpp:619-633)
@@ -470,6 +479,7 @@ void PartitionOp::init(const Setting& root, const Setting& node)
     sortoutput = node["sort"].asString() == "yes";
     sortattr = node.exists("sortattr") ? (unsigned)node["sortattr"].asInt() : attr;
     barrier.reset(new Barrier(buckets));
+    input_failed.reset(1);
     staged.assign(MAX_THREADS, {});
     outputloc.assign(MAX_THREADS, 0);
     overlay.assign(MAX_THREADS, Page());
@@ -500,8 +510,16 @@ Operator::ResultCode PartitionOp::scanStart(unsigned short tid)
     if (nextOp->scanStop(tid) == Error)
         rc = Error;
     outputloc[tid] = 0;
+    if (rc == Error)
+        input_failed.raise(0);
     barrier->arrive();
-    if (tid == 0) {
+    if (tid == 0 && input_failed.raised(0)) {
+        for (unsigned t = 0; t < buckets; ++t)
+            std::vector<char>().swap(staged[t]);
+        part_size.assign(buckets, 0);
+        part_start.assign(buckets + 1, 0);
+        part_rc = PJ_ERR_STATE;
+    } else if (tid == 0) {
         std::vector<char> all;
         for (unsigned t = 0; t < buckets; ++t) {
             all.insert(all.end(), staged[t].begin(), staged[t].end());
@@ -547,7 +565,7 @@ Operator::ResultCode PartitionOp::scanStart(unsigned short tid)
         part_rc = prc;
     }
     barrier->arrive();
-    if (rc == Error || part_rc != PJ_OK)
+    if (rc == Error || input_failed.raised(0) || part_rc != PJ_OK)
         return Error;
     return Ready;
 }
@@ -648,6 +666,7 @@ void SortMergeJoinOp::init(const Setting& root, const Setting& node)
     probe_staged.assign(MAX_THREADS, {});
     rows.assign(leader_of_group.size(), {});
     join_rc.assign(leader_of_group.size(), PJ_OK);
+    input_failed.reset(leader_of_group.size());
     outputloc.assign(MAX_THREADS, 0);
     outend.assign(MAX_THREADS, 0);
     overlay.assign(MAX_THREADS, Page());
@@ -686,8 +705,17 @@ Operator::ResultCode SortMergeJoinOp::scanStart(unsigned short tid)
     ResultCode rc = drain(buildOp, tid, build_staged[tid]);
     if (drain(probeOp, tid, probe_staged[tid]) == Error)
         rc = Error;
+    if (rc == Error)
+        input_failed.raise(g);
     barriers[g]->arrive();
-    if (tid == leader_of_group[g]) {
+    if (tid == leader_of_group[g] && input_failed.raised(g)) {
+        for (unsigned short t : members[g]) {
+            std::vector<char>().swap(build_staged[t]);
+            std::vector<char>().swap(probe_staged[t]);
+            outputloc[t] = outend[t] = 0;
+        }
+        join_rc[g] = PJ_ERR_STATE;
+    } else if (tid == leader_of_group[g]) {
         std::vector<char> b, p;
         for (unsigned short t : members[g]) {
             b.insert(b.end(), build_staged[t].begin(), build_staged[t].end());
@@ -715,7 +743,7 @@ Operator::ResultCode SortMergeJoinOp::scanStart(unsigned short tid)
         }
     }
     barriers[g]->arrive();
-    return rc == Error || join_rc[g] != PJ_OK ? Error : Ready;
+    return rc == Error || input_failed.raised(g) || join_rc[g] != PJ_OK ? Error : Ready;
 }
 
 // SortMergeJoinOp::getNext (join.cpp:1294-1411): this thread's rows in
@@ -802,6 +830,7 @@ void AggregateOp::init(const Setting& root, const Setting& node)
         threads = (unsigned)t;
         barrier.reset(new Barrier(threads));
     }
+    input_failed.reset(1);
     const uint64_t slots = node.exists("slotperbucket") ? (uint64_t)node["slotperbucket"].asInt() : 6;
     in.to_c(&desc.in);
     desc.op = count ? PJ_AGG_COUNT : PJ_AGG_SUM;
@@ -868,8 +897,16 @@ Operator::ResultCode AggregateOp::scanStartFused(unsigned short tid)
     if (drain(fused->probeOp, tid, groups[tid]) == Error)   // groups[tid] holds the probe pages until the count
         rc = Error;
     outputloc[tid] = outend[tid] = 0;
+    if (rc == Error)
+        input_failed.raise(0);
     barrier->arrive();
-    if (tid == 0) {
+    if (tid == 0 && input_failed.raised(0)) {
+        for (unsigned t = 0; t < threads; ++t) {
+            std::vector<char>().swap(staged[t]);
+            std::vector<char>().swap(groups[t]);
+        }
+        agg_rc = PJ_ERR_STATE;
+    } else if (tid == 0) {
         for (unsigned t = 0; t < threads; ++t) {
             b.insert(b.end(), staged[t].begin(), staged[t].end());
             p.insert(p.end(), groups[t].begin(), groups[t].end());
@@ -893,7 +930,7 @@ Operator::ResultCode AggregateOp::scanStartFused(unsigned short tid)
         }
     }
     barrier->arrive();
-    return rc == Error || agg_rc != PJ_OK ? Error : Ready;
+    return rc == Error || input_failed.raised(0) || agg_rc != PJ_OK ? Error : Ready;
 }
 
 // GenericAggregate::scanStart (genericaggregate.cpp:291-424): drain the
@@ -923,8 +960,14 @@ Operator::ResultCode AggregateOp::scanStart(unsigned short tid)
         outend[tid] = ng;
         return rc == Error || arc != PJ_OK ? Error : Ready;
     }
+    if (rc == Error)
+        input_failed.raise(0);
     barrier->arrive();
-    if (tid == 0) {
+    if (tid == 0 && input_failed.raised(0)) {
+        for (unsigned t = 0; t < threads; ++t)
+            std::vector<char>().swap(staged[t]);
+        agg_rc = PJ_ERR_STATE;
+    } else if (tid == 0) {
         std::vector<char> all;
         for (unsigned t = 0; t < threads; ++t) {
             all.insert(all.end(), staged[t].begin(), staged[t].end());
@@ -938,7 +981,7 @@ Operator::ResultCode AggregateOp::scanStart(unsigned short tid)
         }
     }
     barrier->arrive();
-    return rc == Error || agg_rc != PJ_OK ? Error : Ready;
+    return rc == Error || input_failed.raised(0) || agg_rc != PJ_OK ? Error : Ready;
 }
 
 // GenericAggregate::getNext (genericaggregate.cpp:426-495): this thread's
@@ -1010,9 +1053,11 @@ bool MergeOp::run(const std::function<void(const Page&)>& consume)
             if (a->globalThreads() && a->globalThreads() != threads)
                 throw InvalidParameter("a global aggregation's threads must equal the merge's");
     std::mutex mu;
-    std::atomic<bool> failed(false);
+    std::atomic<unsigned> failed_threads(0);
     std::vector<std::string> errors;
+    std::string first_error;
     auto worker = [&](unsigned short tid) {
+        bool failed = false;
         try {
             for (Operator* op : sub)
                 op->threadInit(tid);
@@ -1040,15 +1085,24 @@ bool MergeOp::run(const std::function<void(const Page&)>& consume)
             errors.push_back(e.what());
             failed = true;
         }
+        if (failed) {
+            ++failed_threads;
+            const char* why = pj_last_error();   // this thread's library failure, if it had one
+            std::lock_guard<std::mutex> g(mu);
+            if (why && *why && first_error.empty())
+                first_error = why;
+        }
     };
     std::vector<std::thread> th;
     for (unsigned t = 0; t < threads; ++t)
         th.emplace_back(worker, (unsigned short)t);
     for (auto& x : th)
         x.join();
+    nfailed = failed_threads.load();
+    failure = first_error;
     if (!errors.empty())
         throw QueryExecutionError(errors.front());
-    return !failed;
+    return nfailed == 0;
 }
 
 // ---------------------------------------------------------------- Query
@@ -1160,8 +1214,14 @@ extern "C" int pj_query_run(const char* plan_text, uint32_t nsources, const char
             used += n;
         });
         *out_bytes = used;
-        if (!ok)
+        if (!ok) {
+            std::string msg = std::to_string(m->failedThreads()) + " of " + std::to_string(m->threads) +
+                              " threads returned Error";
+            if (!m->failure.empty())
+                msg += ": " + m->failure;
+            pj_set_error(msg.c_str());
             return PJ_ERR_STATE;
+        }
         return used > out_cap_bytes ? PJ_ERR_CAPACITY : PJ_OK;
     } catch (const pjh::IllegalSchemaDeclarationException& e) {
         pj_set_error(e.what());

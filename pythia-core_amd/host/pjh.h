@@ -145,10 +145,6 @@ struct HashSpec {
 };
 unsigned getlogarithm(unsigned k);                                   // hash.cpp:49-58
 HashSpec make_hashspec(const Setting& node);                         // throws
-// ParameterizedModuloValueHasher::generate (hash.cpp:251-281): per pass
-// (buckets, skipbits), high bits first.
-std::vector<std::pair<unsigned, unsigned>> parammodulo_generate(unsigned buckets, unsigned skipbits,
-                                                                unsigned passes);
 
 // ------------------------------------------------------------ operators
 class Barrier {
@@ -200,6 +196,11 @@ class MemScanOp : public Operator {
     uint64_t per_page = 1;
     volatile uint64_t cursor = 0;
     std::vector<Page> pages;
+    // Failure injection for tests of error handling: thread error_thread
+    // returns Error from its getNext call after error_after pages.
+    int error_thread = -1;
+    unsigned error_after = 0;
+    std::vector<unsigned> served;   // pages handed to each thread
 };
 
 // A raw binary file of packed tuples with no header, the layout of the
@@ -212,6 +213,27 @@ class BinaryScanOp : public MemScanOp {
   private:
     void* map = nullptr;
     uint64_t mapsize = 0;
+};
+
+// One flag per thread group, raised before the group's barrier by a thread
+// whose input returned Error and read after it: the staged input of the
+// group is then incomplete, so the group's device work is skipped and every
+// thread of the group returns Error (the reference's thread returns Error
+// without arriving, and its peers wait at the barrier for good,
+// join.cpp:545-547, 586-588).
+class GroupFlags {
+  public:
+    void reset(size_t n)
+    {
+        f.reset(new std::atomic<bool>[n]);
+        for (size_t i = 0; i < n; ++i)
+            f[i].store(false);
+    }
+    void raise(size_t g) { f[g].store(true); }
+    bool raised(size_t g) const { return f[g].load(); }
+
+  private:
+    std::unique_ptr<std::atomic<bool>[]> f;
 };
 
 class HashJoinOp : public Operator {
@@ -249,6 +271,7 @@ class HashJoinOp : public Operator {
     std::vector<pj_join_t> handles;   // one device table per thread group
     pj_join_desc desc_ = {};
     std::vector<int> build_rc;
+    GroupFlags build_failed;
     HashSpec hash;
     unsigned tuplesperbucket = 0;
     std::vector<int> group_of, leader_of_group, groupsize;
@@ -275,6 +298,7 @@ class PartitionOp : public Operator {
     bool sortoutput = false;
     unsigned sortattr = 0;
     std::unique_ptr<Barrier> barrier;
+    GroupFlags input_failed;                      // one group: every thread
     std::vector<std::vector<char>> staged;       // per thread
     std::vector<char> partitioned;                // all partitions, back to back
     std::vector<uint64_t> part_start, part_size;
@@ -308,6 +332,7 @@ class SortMergeJoinOp : public Operator {
     std::vector<std::unique_ptr<Barrier>> barriers;
     std::vector<std::vector<char>> build_staged, probe_staged, rows;   // per thread / per group
     std::vector<int> join_rc;
+    GroupFlags input_failed;
     std::vector<uint64_t> outputloc, outend;
     std::vector<Page> overlay;
 };
@@ -354,6 +379,7 @@ class AggregateOp : public Operator {
     std::vector<uint64_t> outputloc, outend;
     std::vector<Page> overlay;
     int agg_rc = 0;
+    GroupFlags input_failed;       // global and fused modes: one group
     std::atomic<unsigned> arrived{0};
 };
 
@@ -370,8 +396,14 @@ class MergeOp : public Operator {
     ResultCode scanStop(unsigned short) override { return Ready; }
     std::vector<Operator*> children() override { return {nextOp}; }
     // Drives threadInit .. threadClose on T threads; consume() sees every
-    // output page (serialised). Returns false if a thread saw Error.
+    // output page (serialised). Returns false if a thread saw Error;
+    // failedThreads() then counts them.
     bool run(const std::function<void(const Page&)>& consume);
+    unsigned failedThreads() const { return nfailed; }
+    std::string failure;   // the first library error a failed thread saw, if any
+
+  private:
+    unsigned nfailed = 0;
 };
 
 // Query::create (query.cpp:198-461): builds the operator tree of a plan.

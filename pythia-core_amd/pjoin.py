@@ -85,6 +85,7 @@ class PJError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("pjoin error %d: %s" % (code, msg))
         self.code = code
+        self.msg = msg
 
 
 _lib = None
@@ -140,6 +141,9 @@ _SIGS = {
     "pj_agg_out_width": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "pj_hash_buckets": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "pj_set_device_limit": (C.c_int, [C.c_uint64]),
+    "pj_device_bytes": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "pj_hash_generate": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "pj_tuple_hash_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_void_p, C.c_void_p]),
     "pj_join_count_by_key": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p,
@@ -563,6 +567,26 @@ def hash_buckets(spec):
     b = C.c_uint32()
     check(lib().pj_hash_buckets(C.byref(spec), C.byref(b)))
     return b.value
+
+
+def set_device_limit(nbytes):
+    """Caps the device memory the library's buffers may hold (0: no cap)."""
+    check(lib().pj_set_device_limit(nbytes))
+
+
+def device_bytes():
+    v = C.c_uint64()
+    check(lib().pj_device_bytes(C.byref(v)))
+    return v.value
+
+
+def hash_generate(spec, passes):
+    """The passes functions of a multi-pass radix split of a parammodulo
+    spec, high bits first (ParameterizedModuloValueHasher::generate,
+    hash.cpp:251-281)."""
+    out = (HashSpec * max(passes, 1))()
+    check(lib().pj_hash_generate(C.byref(spec), passes, out))
+    return list(out)[:passes]
 
 
 def tuple_hash_device(spec, d_tuples, n, width, key_offset, key_size, d_buckets, stream=None):

@@ -22,6 +22,7 @@ import glob
 import hashlib
 import json
 import os
+import shutil
 import subprocess
 import sys
 import time
@@ -604,6 +605,67 @@ class _Relations:
         return json.loads(out.strip().splitlines()[-1])
 
 
+# ---------------------------------------------------------------- ceilings
+# What the machine gives the join's access patterns, measured by the
+# library's measurement kernels (pj_measure) on buffers of the bench's size.
+
+CEILINGS = os.path.join(PROFILES, "ceilings_mi355x.md")
+
+
+def measure_ceilings(args):
+    import torch
+    import pjoin as P
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    n = 1 << 28                       # 4 GiB of 16-byte records a buffer
+    a = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    b = torch.empty(n * 16, dtype=torch.uint8, device=dev)
+    P.check(P.lib().pj_gen_pkfk(7, 1 << 40, n, 0, 0, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), 1,
+                                C.c_void_p(stream)))   # distinct random keys in a
+    torch.cuda.synchronize()
+    rows = []
+
+    def run(kind, param=0, nn=n, idx=0, note=""):
+        r = P.measure(kind, a.data_ptr(), b.data_ptr(), idx, nn, param, 5, stream)
+        rows.append(dict(kind=kind, param=param, records=r["ops"], note=note, **r))
+        _say("ceiling %-9s param %-6d %10.3f ms  %8.1f GB/s  %7.2f G/s  %s" % (kind, param, r["ms"], r["gbps"],
+                                                                                r["gops"], note))
+
+    for g in (2048, 8192, 16384):
+        for kind in ("read", "write", "copy"):
+            run(kind, g, note="grid %d" % g)
+    ng = 1 << 27                      # 2 GiB of records for the random patterns
+    perm = torch.randperm(ng, device=dev, dtype=torch.int32)
+    torch.cuda.synchronize()
+    run("gather", nn=ng, idx=perm.data_ptr(), note="random permutation")
+    run("scatter", nn=ng, idx=perm.data_ptr(), note="random permutation")
+    del perm
+    for bits in (8, 11, 12, 14):
+        run("histogram", bits, note="%d bins" % (1 << bits))
+    for addrs in (1, 64, 2048, 65536):
+        run("atomic", addrs, note="%d counters" % addrs)
+    best = {k: max(r["gbps"] for r in rows if r["kind"] == k) for k in ("read", "write", "copy")}
+    props = torch.cuda.get_device_properties(0)
+    out = {"library_sha": lib_sha(), "device": getattr(props, "gcnArchName", "") or props.name,
+           "cus": props.multi_processor_count, "rows": rows, "best_gbps": best}
+    with open(CEILINGS, "w") as f:
+        f.write("# Ceilings of the join's access patterns on one MI355X\n\n")
+        f.write("Command: `python bench.py --ceilings` (the library's pj_measure kernels, "
+                "pythia-core_amd/csrc/pj_measure.hip; fastest of 5 launches after a warm-up, HIP events).\n\n")
+        f.write("Bytes count what the pattern must move: read or write n x 16, copy 2 x n x 16, gather and "
+                "scatter n x (16 + 4 + 16) (record, index, record). Histogram: n x 16 bytes read, one LDS "
+                "atomic per record. Atomic: 2048 x 256 lanes x 64 device atomicAdds, results used.\n\n")
+        f.write("| pattern | parameter | records or atomics | ms | GB/s | G ops/s | note |\n"
+                "|---|---|---|---|---|---|---|\n")
+        for r in rows:
+            f.write("| %s | %d | %d | %.3f | %.1f | %.3f | %s |\n" % (r["kind"], r["param"], r["records"], r["ms"],
+                                                                     r["gbps"], r["gops"], r["note"]))
+        f.write("\n```json\n%s\n```\n" % json.dumps(out, indent=1))
+    os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+    shutil.copy(CEILINGS, os.path.join(ROOT, "gpurun_out", os.path.basename(CEILINGS)))
+    print(json.dumps({"ceilings": best, "file": os.path.relpath(CEILINGS, ROOT)}))
+
+
 def cpu_baseline_full(db, dp, nb, npr, gpu_xor32):
     """The reference HashJoinOp (oracle/_ref, -O3 from its sources) on the
     whole join, under MergeOp(T) -> ConsumeOp (operators/consume.cpp), the
@@ -746,11 +808,16 @@ def main():
                     help="first copy 1 GiB with torch, a kernel of known traffic for scaling HBM counters")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the learned-vs-radix comparison on the clustered 64M x 512M join")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="measure the read/write/copy, gather/scatter, histogram and atomic rates of the GPU "
+                         "(pj_measure) and write profiles/ceilings_mi355x.md")
     ap.add_argument("--collect-traffic", action="store_true",
                     help="write profiles/traffic_<workload>.md from two rocprofv3 counter passes")
     args = ap.parse_args()
     if args.collect_traffic:
         return collect_traffic(args)
+    if args.ceilings:
+        return measure_ceilings(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
               "written up, not built); run with --gpus 1", file=sys.stderr)

@@ -513,6 +513,39 @@ int pj_binary_tuples(const char* pattern, uint32_t tuple_width, uint64_t* ntuple
 int pj_load_binary(const char* pattern, uint32_t tuple_width, void* d_dst, uint64_t cap, uint64_t* ntuples,
                    void* stream);
 
+/* ------------------------------------------------------------------------
+ * Measurement kernels (no counterpart in the reference): the rates the
+ * machine gives the access patterns the join is made of, to set a kernel's
+ * time against. n 16-byte records in d_a (and d_b); each call times reps
+ * launches after one warm-up, on stream, and keeps the fastest.
+ *   READ       n records of d_a streamed in; d_b gets grid x 256 words.
+ *   WRITE      n records streamed out to d_b.
+ *   COPY       d_a to d_b (bytes counts both directions).
+ *   GATHER     d_b[i] = d_a[d_idx[i]] (bytes: record in + index + record out);
+ *              d_idx holds n indices below n (a permutation, say).
+ *   SCATTER    d_b[d_idx[i]] = d_a[i].
+ *   HISTOGRAM  LDS counts of 2^param bins (1..15) over the top bits of the
+ *              mixed first 8 bytes of every record of d_a; d_b gets the
+ *              2048 x 2^param block counts.
+ *   ATOMIC     64 device atomicAdds a lane, results used, over param (a
+ *              power of two) counters at d_a; d_b gets a word a lane.
+ * param is the grid of READ, WRITE and COPY (0: 8192). Buffers are the
+ * caller's; the call allocates nothing on the device.
+ * ---------------------------------------------------------------------- */
+enum { PJ_MEASURE_READ = 0, PJ_MEASURE_WRITE = 1, PJ_MEASURE_COPY = 2, PJ_MEASURE_GATHER = 3,
+       PJ_MEASURE_SCATTER = 4, PJ_MEASURE_HISTOGRAM = 5, PJ_MEASURE_ATOMIC = 6 };
+
+typedef struct {
+    float ms;         /* fastest launch */
+    uint64_t bytes;   /* bytes the pattern moves a launch (0 for ATOMIC) */
+    uint64_t ops;     /* records (or atomics) a launch */
+    double gbps;      /* bytes / ms, GB/s */
+    double gops;      /* ops / ms, G/s */
+} pj_measure_result;
+
+int pj_measure(uint32_t kind, void* d_a, void* d_b, const uint32_t* d_idx, uint64_t n, uint32_t param, uint32_t reps,
+               pj_measure_result* res, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

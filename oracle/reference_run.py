@@ -200,13 +200,24 @@ def main(argv):
     One run of the headline plan (consume_pkfk) in a process of its own, so
     that each timed run starts with the reference allocator's fresh state
     (its lookaside arenas are used up by the first runs of a process).
-    Prints the result as JSON."""
+    Prints the result as JSON.
+
+    python reference_run.py rows BUILD.npy PROBE.npy THREADS OUT.npy
+    The headline plan's output rows (run_pkfk) into OUT.npy, in a process of
+    its own for the same reason; prints the seconds as JSON."""
     import json
-    if len(argv) != 6 or argv[1] != "consume":
-        raise SystemExit("usage: reference_run.py consume BUILD.npy PROBE.npy THREADS PREFETCH")
+    usage = ("usage: reference_run.py consume BUILD.npy PROBE.npy THREADS PREFETCH | "
+             "rows BUILD.npy PROBE.npy THREADS OUT.npy")
+    if len(argv) != 6 or argv[1] not in ("consume", "rows"):
+        raise SystemExit(usage)
     b = np.load(argv[2], mmap_mode="r")
     p = np.load(argv[3], mmap_mode="r")
-    print(json.dumps(consume_pkfk(b, p, int(argv[4]), prefetch=int(argv[5]))))
+    if argv[1] == "consume":
+        print(json.dumps(consume_pkfk(b, p, int(argv[4]), prefetch=int(argv[5]))))
+        return
+    r = run_pkfk(np.ascontiguousarray(b), np.ascontiguousarray(p), int(argv[4]))
+    np.save(argv[5], r.pop("rows"))
+    print(json.dumps(r))
 
 
 if __name__ == "__main__":

@@ -23,7 +23,7 @@ HIP_FLAGS = [
 ]
 
 KERNEL_SOURCES = ["pj_join.hip", "pj_aux.hip", "pj_learned.hip", "pj_batch.hip", "pj_agg.hip", "pj_load.hip", "pj_smj.hip",
-                  "pj_hash.hip", "pj_rows.hip"]
+                  "pj_hash.hip", "pj_rows.hip", "pj_measure.hip"]
 HOST_SOURCES = ["schema.cpp", "operators.cpp"]
 
 

@@ -26,6 +26,33 @@ void Operator::init(const Setting& root, const Setting& node)
 
 // ---------------------------------------------------------------- scans
 
+// Drains op on thread tid into `to`. An Error from it, or an exception it
+// throws (whose message becomes the thread's library error), gives Error;
+// either way the caller goes on to its group's barrier.
+static Operator::ResultCode drain(Operator* op, unsigned short tid, std::vector<char>& to)
+{
+    Operator::ResultCode rc = Operator::Ready;
+    try {
+        if (op->scanStart(tid) == Operator::Error)
+            rc = Operator::Error;
+        Operator::GetNextResultT r(Operator::Ready, nullptr);
+        while (rc != Operator::Error && r.first == Operator::Ready) {
+            r = op->getNext(tid);
+            if (r.first == Operator::Error)
+                rc = Operator::Error;
+            else if (r.second && r.second->getUsedSpace())
+                to.insert(to.end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
+        }
+        if (op->scanStop(tid) == Operator::Error)
+            rc = Operator::Error;
+    } catch (const std::exception& e) {
+        pj_set_error(e.what());
+        rc = Operator::Error;
+    }
+    return rc;
+}
+
+
 void MemScanOp::init(const Setting& root, const Setting& node)
 {
     Operator::init(root, node);
@@ -330,25 +357,36 @@ Operator::ResultCode HashJoinOp::scanStart(unsigned short tid)
     pj_join_t h = handles.at(g);
     ResultCode rc = Ready;
     if (isLeftThread(tid)) {
-        bool failed = buildOp->scanStart(tid) == Error;
-        GetNextResultT r(Ready, nullptr);
-        while (!failed && r.first == Ready) {
-            r = buildOp->getNext(tid);
-            if (r.first == Error) {
-                failed = true;
-                break;
-            }
-            if (r.second && r.second->getNumTuples())
-                if (pj_join_build_append(h, r.second->data(), r.second->getNumTuples()))
+        bool failed = false;
+        try {   // a child's exception fails the build as its Error does
+            failed = buildOp->scanStart(tid) == Error;
+            GetNextResultT r(Ready, nullptr);
+            while (!failed && r.first == Ready) {
+                r = buildOp->getNext(tid);
+                if (r.first == Error) {
                     failed = true;
-        }
-        if (!failed && buildOp->scanStop(tid) == Error)
+                    break;
+                }
+                if (r.second && r.second->getNumTuples())
+                    if (pj_join_build_append(h, r.second->data(), r.second->getNumTuples()))
+                        failed = true;
+            }
+            if (!failed && buildOp->scanStop(tid) == Error)
+                failed = true;
+        } catch (const std::exception& e) {
+            pj_set_error(e.what());
             failed = true;
+        }
         if (failed)
             build_failed.raise(g);
     }
-    if (probeOp->scanStart(tid) == Error)
+    try {
+        if (probeOp->scanStart(tid) == Error)
+            rc = Error;
+    } catch (const std::exception& e) {
+        pj_set_error(e.what());
         rc = Error;
+    }
     barriers[g]->arrive();   // every build page of the group is staged, or a build thread failed
     if (tid == leader_of_group[g])
         build_rc[g] = build_failed.raised(g) ? PJ_ERR_STATE : pj_join_build_finish(h);
@@ -496,19 +534,7 @@ void PartitionOp::threadInit(unsigned short tid)
 // then serves its own partition.
 Operator::ResultCode PartitionOp::scanStart(unsigned short tid)
 {
-    ResultCode rc = Ready;
-    if (nextOp->scanStart(tid) == Error)
-        rc = Error;
-    GetNextResultT r(Ready, nullptr);
-    while (rc != Error && r.first == Ready) {
-        r = nextOp->getNext(tid);
-        if (r.first == Error)
-            rc = Error;
-        else if (r.second && r.second->getUsedSpace())
-            staged[tid].insert(staged[tid].end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
-    }
-    if (nextOp->scanStop(tid) == Error)
-        rc = Error;
+    const ResultCode rc = drain(nextOp, tid, staged[tid]);
     outputloc[tid] = 0;
     if (rc == Error)
         input_failed.raise(0);
@@ -678,23 +704,6 @@ void SortMergeJoinOp::threadInit(unsigned short tid)
         throw InvalidParameter("thread " + std::to_string(tid) + " is in no thread group");
 }
 
-static Operator::ResultCode drain(Operator* op, unsigned short tid, std::vector<char>& to)
-{
-    Operator::ResultCode rc = Operator::Ready;
-    if (op->scanStart(tid) == Operator::Error)
-        rc = Operator::Error;
-    Operator::GetNextResultT r(Operator::Ready, nullptr);
-    while (rc != Operator::Error && r.first == Operator::Ready) {
-        r = op->getNext(tid);
-        if (r.first == Operator::Error)
-            rc = Operator::Error;
-        else if (r.second && r.second->getUsedSpace())
-            to.insert(to.end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
-    }
-    if (op->scanStop(tid) == Operator::Error)
-        rc = Operator::Error;
-    return rc;
-}
 
 // SortMergeJoinOp::scanStart (join.cpp:1234-1292, BufferAndSort
 // :1181-1232): every thread stages both inputs; the group's leader joins
@@ -939,19 +948,7 @@ Operator::ResultCode AggregateOp::scanStart(unsigned short tid)
 {
     if (fused)
         return scanStartFused(tid);
-    ResultCode rc = Ready;
-    if (nextOp->scanStart(tid) == Error)
-        rc = Error;
-    GetNextResultT r(Ready, nullptr);
-    while (rc != Error && r.first == Ready) {
-        r = nextOp->getNext(tid);
-        if (r.first == Error)
-            rc = Error;
-        else if (r.second && r.second->getUsedSpace())
-            staged[tid].insert(staged[tid].end(), r.second->data(), r.second->data() + r.second->getUsedSpace());
-    }
-    if (nextOp->scanStop(tid) == Error)
-        rc = Error;
+    const ResultCode rc = drain(nextOp, tid, staged[tid]);
     outputloc[tid] = outend[tid] = 0;
     if (!global) {
         uint64_t ng = 0;
@@ -1058,9 +1055,14 @@ bool MergeOp::run(const std::function<void(const Page&)>& consume)
     std::string first_error;
     auto worker = [&](unsigned short tid) {
         bool failed = false;
+        size_t inited = 0, closed = 0;   // operators of sub whose threadInit / threadClose ran
+        bool stopped = true;             // no scanStop owed
         try {
-            for (Operator* op : sub)
+            for (Operator* op : sub) {
                 op->threadInit(tid);
+                ++inited;
+            }
+            stopped = false;
             ResultCode rc = nextOp->scanStart(tid);
             if (rc == Error)
                 failed = true;
@@ -1076,14 +1078,30 @@ bool MergeOp::run(const std::function<void(const Page&)>& consume)
                     consume(*r.second);
                 }
             }
+            stopped = true;
             if (nextOp->scanStop(tid) == Error)
                 failed = true;
-            for (Operator* op : sub)
-                op->threadClose(tid);
+            while (closed < inited)
+                sub[closed++]->threadClose(tid);
         } catch (const std::exception& e) {
-            std::lock_guard<std::mutex> g(mu);
-            errors.push_back(e.what());
+            {
+                std::lock_guard<std::mutex> g(mu);
+                errors.push_back(e.what());
+            }
             failed = true;
+            // finish the thread's part of the protocol, so that the threads of
+            // its groups waiting at a barrier in scanStop or threadClose go on
+            try {
+                if (!stopped)
+                    nextOp->scanStop(tid);
+            } catch (const std::exception&) {
+            }
+            while (closed < inited) {
+                try {
+                    sub[closed++]->threadClose(tid);
+                } catch (const std::exception&) {
+                }
+            }
         }
         if (failed) {
             ++failed_threads;

@@ -142,6 +142,8 @@ _SIGS = {
     "pj_binary_tuples": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_uint64)]),
     "pj_hash_buckets": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "pj_set_device_limit": (C.c_int, [C.c_uint64]),
+    "pj_measure": (C.c_int, [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32,
+                             C.c_void_p, C.c_void_p]),
     "pj_device_bytes": (C.c_int, [C.POINTER(C.c_uint64)]),
     "pj_hash_generate": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "pj_tuple_hash_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
@@ -567,6 +569,22 @@ def hash_buckets(spec):
     b = C.c_uint32()
     check(lib().pj_hash_buckets(C.byref(spec), C.byref(b)))
     return b.value
+
+
+MEASURES = {"read": 0, "write": 1, "copy": 2, "gather": 3, "scatter": 4, "histogram": 5, "atomic": 6}
+
+
+class MeasureResult(C.Structure):
+    _fields_ = [("ms", C.c_float), ("bytes", C.c_uint64), ("ops", C.c_uint64), ("gbps", C.c_double),
+                ("gops", C.c_double)]
+
+
+def measure(kind, d_a, d_b, d_idx, n, param=0, reps=5, stream=None):
+    """One measurement kernel (pj_measure): the fastest of reps launches."""
+    r = MeasureResult()
+    check(lib().pj_measure(MEASURES[kind], C.c_void_p(d_a), C.c_void_p(d_b), C.c_void_p(d_idx), n, param, reps,
+                           C.byref(r), C.c_void_p(stream)))
+    return dict(ms=r.ms, bytes=r.bytes, ops=r.ops, gbps=r.gbps, gops=r.gops)
 
 
 def set_device_limit(nbytes):

@@ -168,3 +168,17 @@ def test_exactrange_partition_arguments_are_checked_before_the_device():
     sizes[0] = 99
     assert P.lib().pj_partition_exactrange_device(C.byref(sch), 0, 0, hi - 8, 8, None, 0, None, sizes, None) == P.PJ_OK
     assert list(sizes) == [0] * 8
+
+
+def test_measure_refuses_bad_arguments_before_any_launch():
+    # pj_measure checks its buffers and parameters on the host first
+    for kind, a, b, idx, n, param in (("read", 0, 0, 0, 1 << 20, 0), ("copy", 1 << 20, 0, 0, 1 << 20, 0),
+                                      ("gather", 1 << 20, 1 << 20, 0, 1 << 20, 0),
+                                      ("histogram", 1 << 20, 1 << 20, 0, 1 << 20, 16),
+                                      ("histogram", 1 << 20, 1 << 20, 0, 16, 11),
+                                      ("atomic", 1 << 20, 1 << 20, 0, 1 << 20, 3)):
+        with pytest.raises(P.PJError) as e:
+            P.measure(kind, a, b, idx, n, param)
+        assert e.value.code == P.PJ_ERR_INVALID
+    r = P.MeasureResult()
+    assert P.lib().pj_measure(99, None, None, None, 1, 0, 1, C.byref(r), None) == P.PJ_ERR_INVALID

@@ -297,6 +297,46 @@ def test_gpu_equals_reference_hashjoin():
     assert np.array_equal(O.sorted_rows(got), O.sorted_rows(ref["rows"]))
 
 
+def _rows_in_order(rows):
+    """24-byte rows of three int64 columns in a canonical order for a multiset
+    compare: by the third column when its values are distinct (the probe
+    payloads here, a sort of seconds at 16M rows), else by all three."""
+    c = np.ascontiguousarray(rows).view("<u8").reshape(-1, 3)
+    o = c[np.argsort(c[:, 2])]
+    if np.all(o[1:, 2] > o[:-1, 2]):
+        return o
+    return c[np.lexsort((c[:, 2], c[:, 1], c[:, 0]))]
+
+
+def test_full_1m_x_16m_sorted_compare_with_reference(tmp_path):
+    """Config 1 at full size: the device join's 16.8M rows against the rows of
+    the reference HashJoinOp itself (oracle/_ref, run in a process of its own
+    on the same host inputs), as sorted multisets."""
+    import os
+    import subprocess
+    import reference_run as RR
+    if not RR.available():
+        pytest.skip("oracle/_ref not built")
+    nb, npr = 1 << 20, 1 << 24
+    _say("case: 1M x 16M sorted compare, generating")
+    b, p = O.gen_pkfk(103, nb, nb, npr, 0)
+    fb, fp, fo = tmp_path / "b.npy", tmp_path / "p.npy", tmp_path / "rows.npy"
+    np.save(fb, b)
+    np.save(fp, p)
+    _say("case: 1M x 16M sorted compare, reference HashJoinOp")
+    threads = min(16, os.cpu_count() or 1)
+    r = subprocess.run([sys.executable, RR.__file__, "rows", str(fb), str(fp), str(threads), str(fo)],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = np.load(fo)
+    assert ref.shape == (npr, 24)
+    _say("case: 1M x 16M sorted compare, device join")
+    got, _ = run_join(P.make_desc(**HEADLINE), b.view(np.uint8), p.view(np.uint8), nb, npr)
+    assert got.shape == ref.shape
+    _say("case: 1M x 16M sorted compare, sorting")
+    assert np.array_equal(_rows_in_order(got), _rows_in_order(ref))
+
+
 COMPOSITE = dict(build_cols=["int", "int", "long"], probe_cols=["int", "int", "long"], buildjattr=[0, 1],
                  probejattr=[0, 1], projection=["B$0", "B$1", "B$2", "P$2"])
 

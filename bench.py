@@ -642,6 +642,10 @@ def measure_ceilings(args):
     del perm
     for bits in (8, 11, 12, 14):
         run("histogram", bits, note="%d bins" % (1 << bits))
+    for bits in (18, 22, 24, 25, 26, 27, 28):   # tables of 4 MiB .. 4 GiB
+        run("lookup", bits, note="table of %d MiB" % ((16 << bits) >> 20))
+    for bits in (20, 24, 25, 26):
+        run("probe", bits, note="stream in, look up in %d MiB, stream out" % ((16 << bits) >> 20))
     for addrs in (1, 64, 2048, 65536):
         run("atomic", addrs, note="%d counters" % addrs)
     best = {k: max(r["gbps"] for r in rows if r["kind"] == k) for k in ("read", "write", "copy")}
@@ -653,7 +657,8 @@ def measure_ceilings(args):
         f.write("Command: `python bench.py --ceilings` (the library's pj_measure kernels, "
                 "pythia-core_amd/csrc/pj_measure.hip; fastest of 5 launches after a warm-up, HIP events).\n\n")
         f.write("Bytes count what the pattern must move: read or write n x 16, copy 2 x n x 16, gather and "
-                "scatter n x (16 + 4 + 16) (record, index, record). Histogram: n x 16 bytes read, one LDS "
+                "scatter n x (16 + 4 + 16) (record, index, record); lookup n x 16 (random reads into a table of "
+                "2^param records). Histogram: n x 16 bytes read, one LDS "
                 "atomic per record. Atomic: 2048 x 256 lanes x 64 device atomicAdds, results used.\n\n")
         f.write("| pattern | parameter | records or atomics | ms | GB/s | G ops/s | note |\n"
                 "|---|---|---|---|---|---|---|\n")

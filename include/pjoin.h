@@ -529,11 +529,19 @@ int pj_load_binary(const char* pattern, uint32_t tuple_width, void* d_dst, uint6
  *              2048 x 2^param block counts.
  *   ATOMIC     64 device atomicAdds a lane, results used, over param (a
  *              power of two) counters at d_a; d_b gets a word a lane.
+ *   LOOKUP     n random 16-byte reads at hashed positions of a table of
+ *              2^param records at the front of d_a (2^param <= n); d_b gets
+ *              a word a lane. The lookup rate of a table that size.
+ *   PROBE      n records of d_a streamed in, each looked up at a hashed
+ *              position of a table of 2^param records at the front of d_a,
+ *              a 16-byte result a record streamed out to d_b (bytes: 32 a
+ *              record, the lookups not counted).
  * param is the grid of READ, WRITE and COPY (0: 8192). Buffers are the
  * caller's; the call allocates nothing on the device.
  * ---------------------------------------------------------------------- */
 enum { PJ_MEASURE_READ = 0, PJ_MEASURE_WRITE = 1, PJ_MEASURE_COPY = 2, PJ_MEASURE_GATHER = 3,
-       PJ_MEASURE_SCATTER = 4, PJ_MEASURE_HISTOGRAM = 5, PJ_MEASURE_ATOMIC = 6 };
+       PJ_MEASURE_SCATTER = 4, PJ_MEASURE_HISTOGRAM = 5, PJ_MEASURE_ATOMIC = 6, PJ_MEASURE_LOOKUP = 7,
+       PJ_MEASURE_PROBE = 8 };
 
 typedef struct {
     float ms;         /* fastest launch */

@@ -77,6 +77,36 @@ __global__ __launch_bounds__(kThreads) void k_histogram(const u32x4* __restrict_
         hist[(uint64_t)blockIdx.x * F + b] = cnt[b];
 }
 
+// n random 16-byte reads from a table of 2^bits records at the front of
+// in, at hashed positions: the lookup rate of a table of that size, which
+// the caches hold in part or whole.
+__global__ __launch_bounds__(kThreads) void k_lookup(const u32x4* __restrict__ in, uint64_t n, uint32_t bits,
+                                                     uint32_t* __restrict__ sink)
+{
+    uint32_t acc = 0;
+    const uint64_t mask = (1ull << bits) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        const u32x4 v = in[key_mix(i) & mask];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    sink[(uint64_t)blockIdx.x * kThreads + threadIdx.x] = acc;
+}
+
+// The shape of a probe against one table with no partitioning: stream n
+// records in, look each up at random in a table of 2^bits records at the
+// front of in, stream a 16-byte result out.
+__global__ __launch_bounds__(kThreads) void k_probe_stream(const u32x4* __restrict__ in, uint64_t n, uint32_t bits,
+                                                           u32x4* __restrict__ out)
+{
+    const uint64_t mask = (1ull << bits) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kThreads) {
+        const u32x4 s = __builtin_nontemporal_load(in + i);
+        const u32x4 t = in[key_mix(((uint64_t)s.y << 32) | s.x) & mask];
+        const u32x4 o = {s.x, s.y, t.z, s.z};
+        __builtin_nontemporal_store(o, out + i);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_zero(uint32_t* __restrict__ a, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
@@ -150,6 +180,21 @@ int pj_measure(uint32_t kind, void* d_a, void* d_b, const uint32_t* d_idx, uint6
         bytes = n * 16;
         ops = n;
         break;
+    case PJ_MEASURE_LOOKUP:
+        grid = 8192;
+        if (!d_a || !d_b || n == 0 || param == 0 || param > 40 || (1ull << param) > n ||
+            (uint64_t)grid * kThreads * 4 > n * 16)
+            return fail(PJ_ERR_INVALID, "the lookup measure needs a table of 2^param records within n records");
+        bytes = n * 16;
+        ops = n;
+        break;
+    case PJ_MEASURE_PROBE:
+        grid = 8192;
+        if (!d_a || !d_b || n == 0 || param == 0 || param > 40 || (1ull << param) > n)
+            return fail(PJ_ERR_INVALID, "the probe measure needs a table of 2^param records within n records");
+        bytes = n * 32;
+        ops = n;
+        break;
     case PJ_MEASURE_ATOMIC:
         grid = 2048;
         if (!d_a || !d_b || param == 0 || (param & (param - 1)) || param > n * 4 ||
@@ -194,6 +239,12 @@ int pj_measure(uint32_t kind, void* d_a, void* d_b, const uint32_t* d_idx, uint6
             break;
         case PJ_MEASURE_HISTOGRAM:
             hipLaunchKernelGGL(k_histogram, dim3(grid), dim3(kThreads), lds, st, a, n, param, (uint32_t*)d_b);
+            break;
+        case PJ_MEASURE_LOOKUP:
+            hipLaunchKernelGGL(k_lookup, dim3(grid), dim3(kThreads), 0, st, a, n, param, (uint32_t*)d_b);
+            break;
+        case PJ_MEASURE_PROBE:
+            hipLaunchKernelGGL(k_probe_stream, dim3(grid), dim3(kThreads), 0, st, a, n, param, b);
             break;
         case PJ_MEASURE_ATOMIC:
             hipLaunchKernelGGL(k_atomic, dim3(grid), dim3(kThreads), 0, st, (uint32_t*)d_a, param, 64u,

@@ -571,7 +571,7 @@ def hash_buckets(spec):
     return b.value
 
 
-MEASURES = {"read": 0, "write": 1, "copy": 2, "gather": 3, "scatter": 4, "histogram": 5, "atomic": 6}
+MEASURES = {"read": 0, "write": 1, "copy": 2, "gather": 3, "scatter": 4, "histogram": 5, "atomic": 6, "lookup": 7, "probe": 8}
 
 
 class MeasureResult(C.Structure):

@@ -143,6 +143,12 @@ struct MapView {
     }
 };
 
+// The probe partition of build partition fine (below 2^16) under pf.
+__device__ __forceinline__ uint32_t coarse_of_fine(uint32_t fine, const PartFn& pf)
+{
+    return pf.div > 1 ? __umulhi(fine, pf.divm) : fine;
+}
+
 // Copies the blob into LDS (the caller synchronises before the first use).
 __device__ __forceinline__ MapView load_map(uint8_t* lds, const uint8_t* map)
 {

@@ -177,16 +177,23 @@ static_assert(kMapBytes % 16 == 0, "the blob is copied in 16-byte words");
 // A partition function: radix (the top `bits` bits of the key mix, F =
 // 2^bits) or the learned function in the blob at map (device memory), its
 // build partition divided by div (probe partitions pair up div build
-// partitions; div = 1 for the build side).
+// partitions; div = 1 for the build side). divm = ceil(2^32 / div) for div
+// > 1: the high word of fine * divm is fine / div exactly for fine and div
+// below 2^16 (fine = q div + r: fine * divm / 2^32 = q + (r + fine e / 2^32)
+// / div with e = divm div - 2^32 < div, and fine e / 2^32 < 1).
 struct PartFn {
     uint32_t F;
     uint32_t bits;
     const uint8_t* map;
     uint32_t div;
+    uint32_t divm;
 };
 
-inline PartFn radix_fn(uint32_t bits) { return PartFn{1u << bits, bits, nullptr, 1}; }
-inline PartFn map_fn(uint32_t F, const uint8_t* map, uint32_t div) { return PartFn{F, 0, map, div}; }
+inline PartFn radix_fn(uint32_t bits) { return PartFn{1u << bits, bits, nullptr, 1, 0}; }
+inline PartFn map_fn(uint32_t F, const uint8_t* map, uint32_t div)
+{
+    return PartFn{F, 0, map, div, div > 1 ? (uint32_t)(((1ull << 32) + div - 1) / div) : 0u};
+}
 
 // ---------------------------------------------------------------- host
 // A fixed device region that buffers are carved from in order (the

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist(const uint32_t* __restric
         if (RANGE) {
 #pragma unroll
             for (int k = 0; k < kHistItems; ++k)   // records past the end are looked up harmlessly
-                part[k] = mv.fine(ord_key(rec_key(r[k]))) / pf.div;
+                part[k] = coarse_of_fine(mv.fine(ord_key(rec_key(r[k]))), pf);
         }
 #pragma unroll
         for (int k = 0; k < kHistItems; ++k) {
@@ -151,9 +151,13 @@ constexpr int kWcThreads = 1024;
 // 8-record (128-byte, whole cache line) chunks: blocks b and b + 8 share
 // input chunk `part` (under round-robin dispatch both run on the same XCD, so
 // the second read of a tile mostly hits its L2) and block half h buffers and
-// writes only partitions [h * FL, (h + 1) * FL). On MI355X a 2048-way scatter
-// of 256M 16-byte records takes 2.5 ms with 64-byte chunks and 1.8-1.9 ms
-// split with 128-byte chunks.
+// writes only its half of the partitions: [h * FL, (h + 1) * FL) for hash
+// partitions, the partitions p with p % 2 == h for range partitions. On
+// MI355X a 2048-way scatter of 256M 16-byte records takes 2.5 ms with 64-byte
+// chunks and 1.8-1.9 ms split with 128-byte chunks. Alternate range
+// partitions took 5.3 ms for a 2044-way scatter of 512M clustered records
+// where halves of the key order took 6.1 ms (hash partitions: 4.6 ms in
+// halves, 5.2 ms alternating).
 template <int RM, int B, bool RANGE, int SPLIT>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
                                                             SideLayout L, PartFn pf, uint64_t chunk,
@@ -168,12 +172,15 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint32_t FL = SPLIT == 1 ? F : (F + 1) / 2;   // partitions this block buffers
     const uint32_t part = SPLIT == 1 ? blockIdx.x : (blockIdx.x / 16) * 8 + blockIdx.x % 8;
     const uint32_t half = SPLIT == 1 ? 0u : (blockIdx.x / 8) & 1u;
-    const uint32_t p0 = half * FL, pn = SPLIT == 1 ? F : (half ? F - FL : FL);
+    constexpr bool ALT = SPLIT == 2 && RANGE;   // alternate partitions: local q is global 2q + half
+    const uint32_t p0 = ALT ? 0u : half * FL;
+    const uint32_t pn = SPLIT == 1 ? F : ALT ? (half ? F / 2 : (F + 1) / 2) : (half ? F - FL : FL);
+    auto global_part = [&](uint32_t q) { return ALT ? 2 * q + half : p0 + q; };
     uint4* buf = sm4;                                                          // FL * B
     uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + FL * B);                 // FL: buffered records
     uint32_t* goff = cnt + FL;                                                 // FL: next chunk's record index
     for (uint32_t p = threadIdx.x; p < pn; p += kWcThreads) {
-        goff[p] = offs[(uint64_t)(p0 + p) * nblk + part];
+        goff[p] = offs[(uint64_t)global_part(p) * nblk + part];
         cnt[p] = 0;
     }
     __syncthreads();
@@ -213,8 +220,8 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             v[k] = false;
             if (i < end) {   // partition (a split block keeps only its half's partitions)
                 const uint32_t pp = have_pid ? nxp[k] : part_of(rec_key(r[k]), pf);
-                v[k] = SPLIT == 1 || (pp >= FL) == (half != 0);
-                p[k] = pp - p0;
+                v[k] = SPLIT == 1 || (ALT ? (pp & 1u) == half : (pp >= FL) == (half != 0));
+                p[k] = ALT ? pp >> 1 : pp - p0;
             }
             const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
             if (i2 < end) {
@@ -273,7 +280,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         const uint32_t c = cnt[q];
         if (c > 0) {
             const uint4 r = buf[q * B + gl];
-            const bool keep = gl < c, z = p0 + q == 0;
+            const bool keep = gl < c, z = global_part(q) == 0;
             const uint4 pad = make_uint4(z ? pad_zero.x : pad_other.x, z ? pad_zero.y : pad_other.y,
                                          z ? pad_zero.z : pad_other.z, z ? pad_zero.w : pad_other.w);
             out[goff[q] + gl] = make_uint4(keep ? r.x : pad.x, keep ? r.y : pad.y, keep ? r.z : pad.z,

@@ -41,13 +41,14 @@ constexpr uint64_t kGap = 1ull << 32;      // segment cut
 constexpr uint32_t kWindow = 16;           // least-squares window, leaves either side
 constexpr uint64_t kLeafSamples = 128;     // samples per leaf the root aims for (up to 4096 leaves)
 constexpr float kLambda = 64.f;            // continuity weight (in samples)
-constexpr uint32_t kSteps = 8;             // training steps in a join
+constexpr uint32_t kSteps = 3;             // training steps in a join
 constexpr float kEta = 0.8f;               // step size of the Newton steps
 constexpr double kFill = 0.85;             // planned fill of an LDS table
 constexpr double kFillMax = 0.92;          // highest fill accepted to save a probe re-read
 constexpr double kDenseDensity = 0.97;     // keys per key value above which a segment counts as dense
 
 constexpr uint32_t kMaxFine = 4096;        // build partitions the range kernels' LDS can hold
+static_assert(kMaxFine <= 65536, "build partitions travel as 16-bit ids and divide by multiply-high");
 
 constexpr uint32_t kSortThreads = 1024;
 constexpr uint32_t kSortChunk = 4096;

@@ -94,9 +94,16 @@ __device__ __forceinline__ uint32_t part_of(uint64_t key, const PartFn& pf)
 }
 
 // The learned partition function over a copy of its blob (pj_internal.h).
-// Every step uses explicitly rounded single-precision operations, so that
+// Every step is one single-precision operation rounded to nearest, so that
 // the build side's and the probe side's kernels, compiled apart, put every
-// key in the same partition.
+// key in the same partition. __fmul_rn and __fsub_rn are plain * and - in
+// this toolchain's headers, and hipcc's default contraction (fast, which
+// ignores #pragma clang fp contract) fuses such a multiply and subtract into
+// an FMA where it sees fit, differently in each kernel they are inlined into:
+// two evaluations of this map in one kernel disagreed by one partition on
+// 214 of 64M clustered keys that way. The library is therefore built with
+// -ffp-contract=off; so built, the device's partitions equal an IEEE
+// single-precision evaluation on the host for every one of those keys.
 struct MapView {
     const uint8_t* base;
     __device__ __forceinline__ const MapHdr& hdr() const { return *reinterpret_cast<const MapHdr*>(base); }
@@ -108,6 +115,10 @@ struct MapView {
     {
         return reinterpret_cast<const uint64_t*>(base + kMapOffSegCut)[s];
     }
+    // The build partition of ordered key u: the segment that owns u (the
+    // last whose cut is at or below it, found from the directory slice of u
+    // and a binary search inside it), the leaf of u's position in it, and
+    // that leaf's line, clamped to the segment's partitions.
     __device__ __forceinline__ uint32_t fine(uint64_t u) const
     {
         const MapHdr& h = hdr();
@@ -117,7 +128,6 @@ struct MapView {
             uint64_t d = (u - h.lo0) >> h.dshift;
             if (d >= kMapDir)
                 d = kMapDir - 1;
-            // the last segment owning keys from at or below u lies in [dir[d], dir[d + 1]]
             uint32_t lo = dir[d], hi = dir[d + 1];
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) >> 1;
@@ -131,9 +141,12 @@ struct MapView {
         const uint2 meta = reinterpret_cast<const uint2*>(base + kMapOffSegMeta)[s];
         const uint32_t leaf0 = meta.x & 0xFFFFu, nl = meta.x >> 16, first = meta.y & 0xFFFFu, np = meta.y >> 16;
         const float inv = reinterpret_cast<const float*>(base + kMapOffSegInv)[s];
-        // position in leaves from the segment's first sampled key (negative before it)
         const uint64_t slo = seg_lo(s);
-        const float fl = u >= slo ? __fmul_rn(__ull2float_rn(u - slo), inv) : -__fmul_rn(__ull2float_rn(slo - u), inv);
+        // |u - slo| rounded to single precision, from 32 bits where it fits
+        // (the same value as from 64 bits, in one conversion)
+        const uint64_t dist = u >= slo ? u - slo : slo - u;
+        const float fd = dist < (1ull << 32) ? __uint2float_rn((uint32_t)dist) : __ull2float_rn(dist);
+        const float fl = u >= slo ? __fmul_rn(fd, inv) : -__fmul_rn(fd, inv);
         const uint32_t j = fl < 0.f ? 0u : fl < (float)nl ? (uint32_t)fl : nl - 1;
         const float x = __fsub_rn(fl, (float)j);
         const float2 cd = reinterpret_cast<const float2*>(base + kMapOffLeaf)[leaf0 + j];

@@ -431,6 +431,38 @@ def test_full_64m_x_512m_clustered(partitioner):
         assert used == 1
 
 
+def test_full_64m_x_512m_clustered_learned_probe_of_another_layout():
+    """The learned partition function is evaluated by one histogram kernel
+    per record layout: with 16-byte build tuples and 20-byte probe tuples
+    (long fk, int, long payload) the two sides run different kernels, which
+    must put every key in the same partition (one partition apart on a few
+    hundred keys when the compiler is free to fuse their multiplies and
+    subtracts differently). Same rows as the 16-byte join: the closed form."""
+    _say("case: 64M x 512M clustered, learned, 20-byte probe tuples")
+    n, m = 1 << 26, 1 << 29
+    t, cdf, db, dp = _clustered(404, n, m)
+    p20 = torch.empty((m, 20), dtype=torch.uint8, device="cuda")
+    src = dp.view(m, 16)
+    p20[:, :8] = src[:, :8]
+    p20[:, 8:12] = 7
+    p20[:, 12:] = src[:, 8:]
+    del dp, src
+    torch.cuda.empty_cache()
+    desc = P.make_desc(["long", "long"], ["long", "int", "long"], [0], [0], ["B$0", "B$1", "P$2"],
+                       partitioner="learned", timing=True)
+    j = P.Join(desc)
+    out = torch.empty(m * 24, dtype=torch.uint8, device="cuda")
+    cnt = j.run_device(db.data_ptr(), n, p20.data_ptr(), m, out.data_ptr(), m)
+    s, _ = P.checksum_device(out.data_ptr(), cnt, 24)
+    used = j.timings().used_learned
+    j.close()
+    del db, p20, out
+    torch.cuda.empty_cache()
+    es, ec = O.expected_clustered(t, m, cdf, threads=16)
+    assert used == 1
+    assert (cnt, s) == (ec, es)
+
+
 def _fin64(x):
     m = (1 << 64) - 1
     x ^= x >> 30

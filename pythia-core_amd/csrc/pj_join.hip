@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "pj_device.h"
@@ -197,37 +198,30 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             return reinterpret_cast<const uint4*>(in)[i];
         return load_rec<RM>(in, i, L);
     };
-    uint4 nxt[IT];   // the next round's records, loaded one round ahead
-    uint32_t nxp[IT] = {};
-#pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
-        if (i < end) {
-            nxt[k] = ld(i);
-            if (have_pid)
-                nxp[k] = pid[i];
-        }
-    }
-    for (uint64_t t0 = beg; t0 < end; t0 += (uint64_t)kWcThreads * IT) {
-        uint4 r[IT];
+    constexpr uint64_t R = (uint64_t)kWcThreads * IT;   // records a round
+    // One round over the records in cur (loaded a round ahead), loading the
+    // next round's into nxt. FULL: every record of this round and of the
+    // next lies inside the chunk, so no record needs a bounds check.
+    auto round = [&](uint64_t t0, uint4(&cur)[IT], uint32_t(&curp)[IT], uint4(&nxt)[IT], uint32_t(&nxtp)[IT],
+                     auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
         uint32_t p[IT], s[IT];
         bool v[IT];
 #pragma unroll
         for (int k = 0; k < IT; ++k) {
             const uint64_t i = t0 + (uint64_t)k * kWcThreads + threadIdx.x;
-            r[k] = nxt[k];
             p[k] = 0;
             v[k] = false;
-            if (i < end) {   // partition (a split block keeps only its half's partitions)
-                const uint32_t pp = have_pid ? nxp[k] : part_of(rec_key(r[k]), pf);
+            if (FULL || i < end) {   // partition (a split block keeps only its half's partitions)
+                const uint32_t pp = have_pid ? curp[k] : part_of(rec_key(cur[k]), pf);
                 v[k] = SPLIT == 1 || (ALT ? (pp & 1u) == half : (pp >= FL) == (half != 0));
                 p[k] = ALT ? pp >> 1 : pp - p0;
             }
-            const uint64_t i2 = i + (uint64_t)kWcThreads * IT;
-            if (i2 < end) {
+            const uint64_t i2 = i + R;
+            if (FULL || i2 < end) {
                 nxt[k] = ld(i2);
                 if (have_pid)
-                    nxp[k] = pid[i2];
+                    nxtp[k] = pid[i2];
             }
         }
         // claim
@@ -236,7 +230,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             if (v[k]) {
                 s[k] = atomicAdd(&cnt[p[k]], 1u);
                 if (s[k] < (uint32_t)B)
-                    buf[p[k] * B + s[k]] = r[k];
+                    buf[p[k] * B + s[k]] = cur[k];
             }
         __syncthreads();
         // full buffers leave; whole chunks of overflow go straight out
@@ -251,7 +245,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                 const uint32_t rank = s[k] - B;
                 const uint32_t direct = (cnt[p[k]] - B) / B * B;   // overflow records in whole chunks
                 if (rank < direct)
-                    out[goff[p[k]] + B + rank] = r[k];
+                    out[goff[p[k]] + B + rank] = cur[k];
                 else
                     keep[k] = rank - direct;
             }
@@ -260,7 +254,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
 #pragma unroll
         for (int k = 0; k < IT; ++k)
             if (keep[k] != ~0u)
-                buf[p[k] * B + keep[k]] = r[k];
+                buf[p[k] * B + keep[k]] = cur[k];
         for (uint32_t q = threadIdx.x; q < pn; q += kWcThreads) {
             const uint32_t c = cnt[q];
             if (c >= (uint32_t)B) {
@@ -270,6 +264,45 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             }
         }
         __syncthreads();
+    };
+    uint4 ra[IT], rb[IT];   // records of the current and the next round
+    uint32_t pa[IT] = {}, pb[IT] = {};
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const uint64_t i = beg + (uint64_t)k * kWcThreads + threadIdx.x;
+        if (i < end) {
+            ra[k] = ld(i);
+            if (have_pid)
+                pa[k] = pid[i];
+        }
+    }
+    uint64_t t0 = beg;
+    // Full rounds without bounds checks: two an iteration with the record
+    // registers taking turns (no copies), or, with the partition ids as well
+    // (range partitions), one a round with a copy, which needs fewer
+    // registers. The chunk's last rounds with bounds checks.
+    if (!have_pid) {
+        for (; t0 + 3 * R <= end; t0 += 2 * R) {
+            round(t0, ra, pa, rb, pb, std::true_type{});
+            round(t0 + R, rb, pb, ra, pa, std::true_type{});
+        }
+    } else {
+        for (; t0 + 2 * R <= end; t0 += R) {
+            round(t0, ra, pa, rb, pb, std::true_type{});
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                ra[k] = rb[k];
+                pa[k] = pb[k];
+            }
+        }
+    }
+    for (; t0 < end; t0 += R) {
+        round(t0, ra, pa, rb, pb, std::false_type{});
+#pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            ra[k] = rb[k];
+            pa[k] = pb[k];
+        }
     }
     // partial buffers leave padded to a whole chunk. The buffer slot is read
     // whatever c is (gl < B keeps it inside the buffer) and the padding record

@@ -313,10 +313,11 @@ int pj_model_bounds(const pj_model* model, uint32_t max_coarse, uint64_t* bounds
 
 /* ---------------------------------------------------------------- batch
  * Many independent joins of one descriptor, run as one captured pipeline
- * (one hipGraph launch: batched partition passes, a device-side work list
- * and one persistent join kernel). No reference counterpart: the reference
- * runs one HashJoinOp per plan (query.cpp:301-334); each join of a batch
- * gives the multiset of rows that operator gives for its two inputs. */
+ * (one hipGraph launch: a histogram, a scan and a scatter over every side of
+ * every join, each join's work items counted on the device, and one
+ * persistent join kernel over all of them). No reference counterpart: the
+ * reference runs one HashJoinOp per plan (query.cpp:301-334); each join of a
+ * batch gives the multiset of rows that operator gives for its two inputs. */
 typedef struct pj_batch_s* pj_batch_t;
 typedef struct {
     const void* build;   /* device tuples of the build side */
@@ -326,12 +327,17 @@ typedef struct {
     void* out;           /* device rows, 16-byte aligned */
     uint64_t out_cap;    /* rows out holds */
 } pj_batch_join;
-/* Per size class (floor(log2(build tuples))): radix bits 4..12, passes 1 or
- * 2, and the LDS open-addressing tables' load factor 0.5..0.9. */
+/* Per size class (floor(log2(build tuples))): the radix bits of both sides
+ * (1..12; 0: the fewest whose mean partition fills at most load_factor of an
+ * LDS table, or, with load_factor 0, the single join's choice), partition
+ * passes (1: every side is partitioned into its 2^bits partitions in one
+ * pass; 2 is refused), and the probe records of a work item in units of 4096
+ * (piece; 0: about 1/8 of a CU's share of the join's probe side, at least
+ * 32768). */
 typedef struct {
     uint8_t bits;
     uint8_t passes;
-    uint16_t reserved;
+    uint16_t piece;
     float load_factor;
 } pj_batch_cfg;
 
@@ -355,9 +361,9 @@ int pj_batch_prepare(pj_batch_t h, uint32_t njoins, const pj_batch_join* joins, 
  * parallel branches) into a graph at its first run after pj_batch_prepare
  * or a change of radix bits, and then runs the batch as one graph launch;
  * done_us then holds the whole batch's time for every join, and
- * pj_batch_run_times zeros. This build ships the pipeline switched off:
- * PJ_BATCH_PLAIN and PJ_BATCH_GRAPH return PJ_ERR_UNSUPPORTED, and
- * pj_batch_tune then tunes the radix bits of the other modes. */
+ * pj_batch_run_times zeros. The configurations of pj_batch_set_config and
+ * pj_batch_tune (timed through the pipeline) set the radix bits of every
+ * mode. */
 #define PJ_BATCH_PLAIN 0
 #define PJ_BATCH_GRAPH 1
 #define PJ_BATCH_SEQUENTIAL 2

@@ -262,6 +262,65 @@ int rows_materialize(const uint2* pairs, const unsigned long long* ncand, uint64
                      const void* d_probe, const RowSpec& spec, void* d_out, uint64_t out_cap,
                      unsigned long long* rows, uint32_t* nfail, bool compact, hipStream_t st);
 
+// ---------------------------------------------------------------- batched joins
+// Many radix joins of one descriptor run as one pipeline of kernels, each
+// launched once for the whole batch (pj_join.hip; pj_batch.hip drives it):
+// per-tile histograms of every side of every join, one scan over all of
+// them, a tile-sorted scatter into one record buffer, each side's partition
+// starts, each join's work items counted and scanned on the device, and one
+// persistent join kernel over every join's items.
+
+// One side (build or probe) of one join of a batch.
+struct MSide {
+    const uint32_t* in;   // raw tuples
+    uint32_t n;           // tuples
+    uint32_t bits;        // radix bits: 1 << bits partitions
+    uint32_t probe;       // 0: build side (layout L0), 1: probe side (L1)
+    uint32_t nblk;        // tiles (blocks of the histogram and scatter kernels)
+    uint32_t chunk;       // tuples a tile
+    uint32_t blk0;        // first block of this side's tiles in those kernels' grids
+    uint32_t hbase;       // first histogram counter: (1 << bits) * nblk of them, partition-major
+    uint32_t rbase;       // first record of the side in the batch's record buffer
+    uint32_t pbase;       // first of its (1 << bits) + 1 partition starts
+    uint32_t pad;
+};
+
+// One join of a batch: its work items (v: offs, rst and sst point into the
+// batch's buffers; rst and sst count records from the record buffer's start)
+// and where its rows go.
+struct MJoin {
+    PlanView v;
+    uint32_t* out;
+    uint64_t out_cap;
+    uint32_t bits;        // radix bits of both sides
+    uint32_t pad;
+};
+
+// Per join, written by the kernels.
+struct MState {
+    unsigned long long rows;     // output rows reserved (rows at or past out_cap are not written)
+    uint32_t remaining;          // work items not finished
+    uint32_t items;              // work items
+    unsigned long long t_done;   // device clock (100 MHz) when its last item finished
+};
+
+// A prepared batch: descriptors and buffers on the device.
+struct MBatch {
+    uint32_t njoins = 0, nsides = 0, nblocks = 0, maxF = 1;
+    uint64_t nhist = 0, nrec = 0, nstarts = 0, noffs = 0, max_items = 0;
+    int rm = 0;                  // record mode of the tuples
+    Layout lay;
+    DevBuf sides, joins, hist, bsum, recs, starts, offs, ibase, state, misc;
+};
+
+// Lets the pipeline's kernels use the LDS a batch of this shape needs (call
+// outside a stream capture), then enqueues the whole pipeline on st (no host
+// sync, no memset: a caller may capture it into a graph).
+int mbatch_setup(const MBatch& b);
+int mbatch_enqueue(MBatch& b, hipStream_t st);
+// LDS records of the batch scatter's tiles for maxF partitions a side.
+uint32_t mbatch_scatter_tile(uint32_t maxF);
+
 // Device state of the learned partitioner (pj_learned.hip).
 struct LearnedCtx {
     DevBuf sample, tmp, hist, bsum, model, params, moments, resid, bounds, segs, loss;

@@ -916,6 +916,267 @@ __global__ __launch_bounds__(kEmitThreads) void k_groupjoin_emit(const uint4* __
     }
 }
 
+// ---------------------------------------------------------------- batched joins
+// The pipeline of a batch (MBatch, pj_internal.h): every kernel runs once for
+// all joins. A block of the histogram and scatter kernels finds its side by
+// binary search over the sides' first blocks; a work item of the join kernel
+// finds its join by binary search over the joins' first items. The device
+// loops are bounded by the descriptors the host wrote and checked.
+
+constexpr uint32_t kMaxBatchBits = 12;
+
+__device__ __forceinline__ uint32_t side_of_block(const MSide* __restrict__ sides, uint32_t nsides, uint32_t b)
+{
+    uint32_t lo = 0, hi = nsides;   // the last side whose first block is at or below b
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (sides[mid].blk0 <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// hist[S.hbase + p * S.nblk + t] = tuples of tile t of side S in partition p.
+template <int RM>
+__global__ __launch_bounds__(kHistThreads) void k_mhist(const MSide* __restrict__ sides, uint32_t nsides,
+                                                         SideLayout L0, SideLayout L1, uint32_t* __restrict__ hist)
+{
+    extern __shared__ uint32_t mcnt[];
+    const MSide S = sides[side_of_block(sides, nsides, blockIdx.x)];
+    const uint32_t t = blockIdx.x - S.blk0, F = 1u << S.bits;
+    const SideLayout& L = S.probe ? L1 : L0;
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+        mcnt[p] = 0;
+    __syncthreads();
+    const uint64_t beg = (uint64_t)t * S.chunk;
+    const uint64_t end = beg + S.chunk < S.n ? beg + S.chunk : S.n;
+    for (uint64_t base = beg; base < end; base += (uint64_t)kHistThreads * kHistItems) {
+        uint4 r[kHistItems];
+#pragma unroll
+        for (int k = 0; k < kHistItems; ++k) {
+            const uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
+            if (i < end)
+                r[k] = load_rec<RM>(S.in, i, L);
+        }
+#pragma unroll
+        for (int k = 0; k < kHistItems; ++k) {
+            const uint64_t i = base + (uint64_t)k * kHistThreads + threadIdx.x;
+            if (i < end)
+                atomicAdd(&mcnt[radix_part(key_mix(rec_key(r[k])), S.bits)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < F; p += kHistThreads)
+        hist[S.hbase + p * S.nblk + t] = mcnt[p];
+}
+
+// Tile-sorted scatter of tile t of side S into recs at the scanned offsets:
+// each LDS tile of ts records is counting-sorted by partition, then written
+// so that lanes of a wave store consecutive records of a partition.
+template <int RM>
+__global__ __launch_bounds__(kScatterThreads) void k_mscatter(const MSide* __restrict__ sides, uint32_t nsides,
+                                                               SideLayout L0, SideLayout L1,
+                                                               const uint32_t* __restrict__ offs, uint32_t ts,
+                                                               uint32_t maxF, uint4* __restrict__ recs)
+{
+    extern __shared__ uint4 sm4[];
+    const MSide S = sides[side_of_block(sides, nsides, blockIdx.x)];
+    const uint32_t t = blockIdx.x - S.blk0, F = 1u << S.bits;
+    const SideLayout& L = S.probe ? L1 : L0;
+    uint4* tile = sm4;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(tile + ts);   // maxF + 1
+    uint32_t* goff = cnt + maxF + 1;                          // maxF
+    uint32_t* wsum = goff + maxF;                             // 16
+    for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
+        goff[p] = offs[S.hbase + p * S.nblk + t];
+    const uint64_t beg = (uint64_t)t * S.chunk;
+    const uint64_t end = beg + S.chunk < S.n ? beg + S.chunk : S.n;
+    for (uint64_t t0 = beg; t0 < end; t0 += ts) {
+        const uint32_t tn = (uint32_t)(end - t0 < ts ? end - t0 : ts);
+        for (uint32_t p = threadIdx.x; p <= F; p += kScatterThreads)
+            cnt[p] = 0;
+        __syncthreads();
+        uint4 r[kScatterItems];
+        uint32_t part[kScatterItems], rank[kScatterItems];
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) {
+            const uint32_t i = k * kScatterThreads + threadIdx.x;
+            if (i < tn) {
+                r[k] = load_rec<RM>(S.in, t0 + i, L);
+                part[k] = radix_part(key_mix(rec_key(r[k])), S.bits);
+                rank[k] = atomicAdd(&cnt[part[k]], 1u);
+            }
+        }
+        __syncthreads();
+        block_scan_array<kScatterThreads>(cnt, F, wsum);
+#pragma unroll
+        for (int k = 0; k < kScatterItems; ++k) {
+            const uint32_t i = k * kScatterThreads + threadIdx.x;
+            if (i < tn)
+                tile[cnt[part[k]] + rank[k]] = r[k];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < tn; i += kScatterThreads) {
+            const uint4 v = tile[i];
+            const uint32_t p = radix_part(key_mix(rec_key(v)), S.bits);
+            recs[goff[p] + (i - cnt[p])] = v;
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < F; p += kScatterThreads)
+            goff[p] += cnt[p + 1] - cnt[p];
+        __syncthreads();
+    }
+}
+
+// Partition starts of every side (one block a side): starts[S.pbase + p] =
+// the first record of partition p in recs, starts[S.pbase + F] = its end.
+__global__ void k_mbounds(const MSide* __restrict__ sides, const uint32_t* __restrict__ offs,
+                          uint32_t* __restrict__ starts)
+{
+    const MSide S = sides[blockIdx.x];
+    const uint32_t F = 1u << S.bits;
+    for (uint32_t p = threadIdx.x; p <= F; p += blockDim.x)
+        starts[S.pbase + p] = p == F ? S.rbase + S.n : S.nblk ? offs[S.hbase + p * S.nblk] : S.rbase;
+}
+
+// The work items of join blockIdx.x (one block a join, as k_plan): offs[p]
+// = first item of build partition p, offs[FR] = the join's items; its state
+// is cleared and armed with that count.
+__global__ __launch_bounds__(kPlanThreads) void k_mplan(const MJoin* __restrict__ joins, MState* __restrict__ state)
+{
+    __shared__ uint32_t wsum[kPlanThreads / 64 + 1];
+    const PlanView v = joins[blockIdx.x].v;
+    uint32_t* offs = const_cast<uint32_t*>(v.offs);
+    const uint32_t per = (v.FR + kPlanThreads - 1) / kPlanThreads;
+    const uint32_t p0 = threadIdx.x * per;
+    uint32_t c[kPlanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kPlanPer; ++q) {
+        c[q] = 0;
+        if (q < per && p0 + q < v.FR) {
+            uint32_t splits;
+            c[q] = plan_count(v, p0 + q, &splits);
+            sum += c[q];
+        }
+    }
+    uint32_t total;
+    uint32_t pre = block_excl_scan<kPlanThreads>(sum, wsum, &total);
+#pragma unroll
+    for (uint32_t q = 0; q < kPlanPer; ++q)
+        if (q < per && p0 + q < v.FR) {
+            offs[p0 + q] = pre;
+            pre += c[q];
+        }
+    if (threadIdx.x == 0) {
+        offs[v.FR] = total;
+        MState& m = state[blockIdx.x];
+        m.rows = 0;
+        m.remaining = total;
+        m.items = total;
+        m.t_done = 0;
+    }
+}
+
+// One block: ibase[j] = the first work item of join j, ibase[nj] = all
+// items; clears the item counter and the error flag and stamps the batch's
+// start (misc words 2-3).
+__global__ __launch_bounds__(1024) void k_mitems(const MState* __restrict__ state, uint32_t nj,
+                                                  uint32_t* __restrict__ ibase, uint32_t* __restrict__ misc,
+                                                  unsigned long long* __restrict__ t0)
+{
+    __shared__ uint32_t wsum[1024 / 64 + 1];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0)
+        carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < nj; b += 1024) {
+        const uint32_t j = b + threadIdx.x;
+        const uint32_t v = j < nj ? state[j].items : 0u;
+        uint32_t total;
+        const uint32_t pre = block_excl_scan<1024>(v, wsum, &total);
+        if (j < nj)
+            ibase[j] = carry + pre;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ibase[nj] = carry;
+        misc[0] = 0;   // item counter
+        misc[1] = 0;   // error flag
+        *t0 = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
+// Every join's work items from one counter (one block a CU, as k_join): the
+// item's join by binary search over ibase, then the build partition's LDS
+// table and the probe records streamed past it into that join's output. The
+// last item of a join stamps its completion. A count of items above the
+// host's bound (max_items) is refused with misc[1] = 1 before any item runs.
+__global__ __launch_bounds__(kJoinThreads) void k_mjoin(const uint4* __restrict__ recs,
+                                                         const MJoin* __restrict__ joins, uint32_t nj,
+                                                         const uint32_t* __restrict__ ibase,
+                                                         MState* __restrict__ state, uint32_t* __restrict__ misc,
+                                                         uint32_t max_items, OutFormat fmt)
+{
+    extern __shared__ uint64_t lds64[];
+    const uint32_t tid = threadIdx.x;
+    uint32_t* lmisc = reinterpret_cast<uint32_t*>(lds64);   // [0] item, [2..33] wave bases
+    uint32_t* wsum = lmisc + 40;
+    uint64_t* table = lds64 + kMiscWords / 2;
+    uint64_t* tkey = table;                                            // kTableCap
+    uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);          // kTableCap
+    uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);    // kTableBuckets / 2 + 1
+    const bool kbp = fmt.words == 6 && fmt.src[0] == SRC_BK0 && fmt.src[1] == SRC_BK1 && fmt.src[2] == SRC_BP0 &&
+                     fmt.src[3] == SRC_BP1 && fmt.src[4] == SRC_PP0 && fmt.src[5] == SRC_PP1;
+    const uint32_t nitems = ibase[nj];
+    if (nitems > max_items) {
+        if (blockIdx.x == 0 && tid == 0)
+            misc[1] = 1;
+        return;
+    }
+    for (;;) {
+        if (tid == 0)
+            lmisc[0] = atomicAdd(&misc[0], 1u);
+        __syncthreads();
+        const uint32_t it = lmisc[0];
+        if (it >= nitems)
+            break;
+        uint32_t lo = 0, hi = nj;   // the last join whose first item is at or below it
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (ibase[mid] <= it)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const MJoin& J = joins[lo];
+        const PlanView v = J.v;
+        const uint32_t bits = J.bits, bkt_shift = 64 - bits - 12;
+        const Item I = plan_item(v, it - ibase[lo]);
+        build_buckets(
+            recs, I.r_off, I.r_cnt, dirw, wsum,
+            [&](const uint4& r) -> uint32_t {
+                const uint64_t h = key_mix(rec_key(r));
+                return radix_part(h, bits) == I.part ? (uint32_t)(h >> bkt_shift) & (kTableBuckets - 1) : ~0u;
+            },
+            [&](uint32_t j, const uint4& r, uint32_t) {
+                tkey[j] = rec_key(r);
+                tpay[j] = make_uint2(r.z, r.w);
+            });
+        const RadixTab tab{tkey, tpay, reinterpret_cast<const uint16_t*>(dirw), bits, bkt_shift, I.part};
+        probe_rounds<kJoinThreads>(tab, recs + I.s_off, I.s_cnt, J.out, J.out_cap, &state[lo].rows, fmt, kbp, wsum,
+                                   lmisc + 2);
+        __syncthreads();
+        if (tid == 0 && atomicSub(&state[lo].remaining, 1u) == 1u)
+            state[lo].t_done = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
 // LDS bytes of the join kernel.
 constexpr size_t join_lds_bytes(bool range)
 {
@@ -1965,6 +2226,75 @@ int pj::join_enqueue(pj_join_s* h, const void* d_build, uint64_t nb, const void*
         return rc;
     bool empty;
     return probe_enqueue(h, d_probe, np, d_out, out_cap, slot, &empty, st);
+}
+
+// ---------------------------------------------------------------- batched joins
+
+uint32_t pj::mbatch_scatter_tile(uint32_t maxF) { return scatter_tile(maxF, false); }
+
+namespace {
+
+size_t mbatch_scatter_lds(uint32_t maxF)
+{
+    return (size_t)scatter_tile(maxF, false) * 16 + (2 * (size_t)maxF + 1) * 4 + 16 * 4;
+}
+
+}  // namespace
+
+int pj::mbatch_setup(const MBatch& b)
+{
+    int rc;
+    const size_t hl = (size_t)b.maxF * 4, sl = mbatch_scatter_lds(b.maxF), jl = join_lds_bytes(false);
+    if ((rc = allow_lds((const void*)k_mhist<REC_WORDS>, hl)) || (rc = allow_lds((const void*)k_mhist<REC_IDENT>, hl)) ||
+        (rc = allow_lds((const void*)k_mscatter<REC_WORDS>, sl)) ||
+        (rc = allow_lds((const void*)k_mscatter<REC_IDENT>, sl)) || (rc = allow_lds((const void*)k_mjoin, jl)))
+        return rc;
+    return PJ_OK;
+}
+
+int pj::mbatch_enqueue(MBatch& b, hipStream_t st)
+{
+    if (b.njoins == 0 || b.maxF > (1u << kMaxBatchBits) || b.max_items >= (1ull << 32))
+        return fail(PJ_ERR_STATE, "no prepared batch");
+    const MSide* sides = (const MSide*)b.sides.p;
+    uint32_t* hist = (uint32_t*)b.hist.p;
+    const SideLayout& L0 = b.lay.build;
+    const SideLayout& L1 = b.lay.probe;
+    if (b.nblocks) {
+        const size_t hl = (size_t)b.maxF * 4, sl = mbatch_scatter_lds(b.maxF);
+        const uint32_t ts = scatter_tile(b.maxF, false);
+        if (ts == 0)
+            return fail(PJ_ERR_UNSUPPORTED, "%u partitions exceed the batch scatter's LDS budget", b.maxF);
+        if (b.rm == REC_IDENT)
+            hipLaunchKernelGGL(k_mhist<REC_IDENT>, dim3(b.nblocks), dim3(kHistThreads), hl, st, sides, b.nsides, L0,
+                               L1, hist);
+        else
+            hipLaunchKernelGGL(k_mhist<REC_WORDS>, dim3(b.nblocks), dim3(kHistThreads), hl, st, sides, b.nsides, L0,
+                               L1, hist);
+        const uint32_t nsb = (uint32_t)((b.nhist + kScanSeg - 1) / kScanSeg);
+        uint32_t* bsum = (uint32_t*)b.bsum.p;
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, hist, b.nhist, bsum);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
+        hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, hist, b.nhist, (const uint32_t*)bsum);
+        if (b.rm == REC_IDENT)
+            hipLaunchKernelGGL(k_mscatter<REC_IDENT>, dim3(b.nblocks), dim3(kScatterThreads), sl, st, sides, b.nsides,
+                               L0, L1, (const uint32_t*)hist, ts, b.maxF, (uint4*)b.recs.p);
+        else
+            hipLaunchKernelGGL(k_mscatter<REC_WORDS>, dim3(b.nblocks), dim3(kScatterThreads), sl, st, sides, b.nsides,
+                               L0, L1, (const uint32_t*)hist, ts, b.maxF, (uint4*)b.recs.p);
+    }
+    hipLaunchKernelGGL(k_mbounds, dim3(b.nsides), dim3(256), 0, st, sides, (const uint32_t*)hist,
+                       (uint32_t*)b.starts.p);
+    MState* state = (MState*)b.state.p;
+    uint32_t* misc = (uint32_t*)b.misc.p;
+    hipLaunchKernelGGL(k_mplan, dim3(b.njoins), dim3(kPlanThreads), 0, st, (const MJoin*)b.joins.p, state);
+    hipLaunchKernelGGL(k_mitems, dim3(1), dim3(1024), 0, st, (const MState*)state, b.njoins, (uint32_t*)b.ibase.p,
+                       misc, (unsigned long long*)(misc + 2));
+    hipLaunchKernelGGL(k_mjoin, dim3(device_count_cus()), dim3(kJoinThreads), join_lds_bytes(false), st,
+                       (const uint4*)b.recs.p, (const MJoin*)b.joins.p, b.njoins, (const uint32_t*)b.ibase.p, state,
+                       misc, (uint32_t)b.max_items, b.lay.out);
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
 }
 
 // ---------------------------------------------------------------- scratch

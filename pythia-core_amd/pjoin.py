@@ -69,7 +69,7 @@ class BatchJoin(C.Structure):
 
 
 class BatchCfg(C.Structure):
-    _fields_ = [("bits", C.c_uint8), ("passes", C.c_uint8), ("reserved", C.c_uint16), ("load_factor", C.c_float)]
+    _fields_ = [("bits", C.c_uint8), ("passes", C.c_uint8), ("piece", C.c_uint16), ("load_factor", C.c_float)]
 
 
 class AggDesc(C.Structure):
@@ -304,7 +304,9 @@ class Join:
 
 
 class Batch:
-    """Owns a pj_batch_t: many joins of one descriptor in one captured pipeline."""
+    """Owns a pj_batch_t: many joins of one descriptor, run as one pipeline
+    (PLAIN, or GRAPH: one graph launch) or as whole single joins
+    (SEQUENTIAL, CONCURRENT, CONCURRENT_GRAPH)."""
 
     def __init__(self, desc):
         self.h = C.c_void_p()
@@ -322,13 +324,18 @@ class Batch:
         arr = self._joins(joins)
         check(lib().pj_batch_tune(self.h, len(joins), arr, trial, C.c_void_p(stream)))
 
+    PIECE_UNIT = 4096   # probe records of a work item are set in these units
+
     def config(self, size_class):
+        """(radix bits, passes, load factor, probe records a work item); 0
+        bits: from the load factor, or the single join's choice when that is
+        0; 0 probe records: the default."""
         c = BatchCfg()
         check(lib().pj_batch_get_config(self.h, size_class, C.byref(c)))
-        return c.bits, c.passes, c.load_factor
+        return c.bits, c.passes, c.load_factor, c.piece * self.PIECE_UNIT
 
-    def set_config(self, size_class, bits, passes, load_factor):
-        c = BatchCfg(bits, passes, 0, load_factor)
+    def set_config(self, size_class, bits, passes=1, load_factor=0.0, piece=0):
+        c = BatchCfg(bits, passes, piece // self.PIECE_UNIT, load_factor)
         check(lib().pj_batch_set_config(self.h, size_class, C.byref(c)))
 
     def prepare(self, joins, stream=None):

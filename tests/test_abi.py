@@ -99,20 +99,26 @@ def test_probe_before_build_finish_is_a_state_error():
     j.close()
 
 
-def test_batch_modes_before_and_after_prepare():
-    """The batch handle checks its mode and state on the host: running before
-    prepare is a state error, an unknown mode is invalid, and the pipeline
-    modes report that they are switched off (no device work is started)."""
+def test_batch_modes_and_configs_before_prepare():
+    """The batch handle checks its mode, state and configurations on the host
+    before any device work: running or reading results before prepare is a
+    state error, an unknown mode is invalid, two partition passes are
+    refused, and so are radix bits above 12 and load factors outside
+    0.05..1."""
     bt = P.Batch(_desc())
     assert P.lib().pj_batch_run(bt.h, P.Batch.SEQUENTIAL, None) == P.PJ_ERR_STATE
-    buf = (C.c_uint8 * 64)()
-    bt.prepare([(C.addressof(buf), 1, C.addressof(buf), 1, C.addressof(buf), 1)])
+    assert P.lib().pj_batch_run(bt.h, P.Batch.GRAPH, None) == P.PJ_ERR_STATE
     assert P.lib().pj_batch_run(bt.h, 7, None) == P.PJ_ERR_INVALID
-    for mode in (P.Batch.PLAIN, P.Batch.GRAPH):
-        assert P.lib().pj_batch_run(bt.h, mode, None) == P.PJ_ERR_UNSUPPORTED
-        assert b"PJ_BATCH_SEQUENTIAL" in P.lib().pj_last_error()
     cnt = (C.c_uint64 * 1)()
     assert P.lib().pj_batch_results(bt.h, cnt, None, None) == P.PJ_ERR_STATE
+    for bits, passes, lf, code in [(8, 2, 0.0, P.PJ_ERR_UNSUPPORTED), (13, 1, 0.0, P.PJ_ERR_INVALID),
+                                   (0, 1, 1.5, P.PJ_ERR_INVALID), (0, 1, 0.01, P.PJ_ERR_INVALID)]:
+        with pytest.raises(P.PJError) as e:
+            bt.set_config(10, bits, passes, lf)
+        assert e.value.code == code
+    bt.set_config(10, 0, 1, 0.7, piece=65536)
+    assert bt.config(10) == (0, 1, pytest.approx(0.7), 65536)
+    assert bt.config(11) == (0, 1, 0.0, 0)   # untuned: the single join's choice
     bt.close()
 
 

@@ -1,9 +1,10 @@
 """GPU tests of the batched joins (pj_batch_*): every join of a batch
 against the generator's closed form (count and 64-bit checksum) or the
-CPU oracle's HashJoinOp (sorted rows). The sequential mode (each join a
-whole pj_join_run_device with the radix bits of its size class) runs; the
-batched pipeline's tests (graph replay, plain launches, its forced
-configurations) are skipped while that pipeline is switched off.
+CPU oracle's HashJoinOp (sorted rows), in the modes that run each join as a
+whole single join (one after another, on four streams, or those streams
+captured into one graph). The batched pipeline's tests (its kernels
+launched one by one, or captured into one graph) are skipped while that
+pipeline is switched off.
 """
 import ctypes as C
 import sys
@@ -16,8 +17,8 @@ import pjoin as P
 
 pytestmark = pytest.mark.gpu
 
-PIPELINE_OFF = pytest.mark.skip(reason="the batched pipeline is switched off: its first device run hung and the "
-                                       "cause has not been found")
+PIPELINE_OFF = pytest.mark.skip(reason="the batched pipeline is switched off: its first device run (a three-join "
+                                       "batch) did not finish and the cause has not been found")
 
 torch = pytest.importorskip("torch")
 
@@ -63,6 +64,23 @@ def _check(seed, joins, cnt):
 
 
 @PIPELINE_OFF
+def test_batch_pipeline_three_joins():
+    _say("case: pipeline, three joins")
+    seed = 20
+    joins, keep = _make(seed, [5000, 300, 70000])
+    bt = P.Batch(P.make_desc(**DESC))
+    bt.prepare(joins)
+    for mode in (P.Batch.PLAIN, P.Batch.GRAPH):
+        for j in range(len(joins)):
+            keep[3 * j + 2].fill_(0xAB)
+        bt.run(mode)
+        cnt, done = bt.results()
+        _check(seed, joins, cnt)
+        assert np.all(done > 0)
+    bt.close()
+
+
+@PIPELINE_OFF
 def test_batch_small_joins_graph_and_plain():
     _say("case: batch of 40 joins, graph replays and plain launches")
     seed = 21
@@ -88,16 +106,17 @@ def test_batch_small_joins_graph_and_plain():
     bt.close()
 
 
+@pytest.mark.parametrize("bits,lf,piece", [(4, 0.0, 0), (8, 0.0, 32768), (12, 0.0, 0), (0, 0.5, 0),
+                                           (0, 0.9, 262144), (9, 0.0, 4096), (1, 0.0, 0)])
 @PIPELINE_OFF
-@pytest.mark.parametrize("bits,passes,lf", [(4, 1, 0.5), (8, 2, 0.9), (12, 1, 0.7), (12, 2, 0.5), (9, 2, 0.7)])
-def test_batch_forced_configs(bits, passes, lf):
-    _say("case: batch config bits=%d passes=%d lf=%.1f" % (bits, passes, lf))
+def test_batch_forced_configs(bits, lf, piece):
+    _say("case: pipeline config bits=%d lf=%.1f piece=%d" % (bits, lf, piece))
     seed = 22
     sizes = P.batch_sizes(seed, 12, 1024, 1 << 20)
     joins, keep = _make(seed, sizes)
     bt = P.Batch(P.make_desc(**DESC))
     for c in range(32):
-        bt.set_config(c, bits, passes, lf)
+        bt.set_config(c, bits, 1, lf, piece=piece)
     bt.prepare(joins)
     bt.run(P.Batch.GRAPH)
     cnt, _ = bt.results()
@@ -106,6 +125,10 @@ def test_batch_forced_configs(bits, passes, lf):
 
 
 MODES = pytest.mark.parametrize("mode", [P.Batch.SEQUENTIAL, P.Batch.CONCURRENT], ids=["sequential", "concurrent"])
+ALL_MODES = pytest.mark.parametrize("mode", [pytest.param(P.Batch.PLAIN, marks=PIPELINE_OFF),
+                                             pytest.param(P.Batch.GRAPH, marks=PIPELINE_OFF),
+                                             P.Batch.SEQUENTIAL, P.Batch.CONCURRENT],
+                                    ids=["plain", "graph", "sequential", "concurrent"])
 
 
 @MODES
@@ -162,7 +185,7 @@ def test_batch_concurrent_graph_replays():
     bt.close()
 
 
-@MODES
+@ALL_MODES
 @pytest.mark.parametrize("bits", [4, 9, 12])
 def test_batch_sequential_forced_bits(bits, mode):
     _say("case: batch mode %d, bits=%d" % (mode, bits))
@@ -179,7 +202,7 @@ def test_batch_sequential_forced_bits(bits, mode):
     bt.close()
 
 
-@MODES
+@ALL_MODES
 def test_batch_sequential_empty_sides(mode):
     """Joins with no build tuples, no probe tuples or neither, between normal
     ones: zero rows for the empty ones, the closed form for the others."""
@@ -204,7 +227,7 @@ def test_batch_sequential_empty_sides(mode):
     bt.close()
 
 
-@MODES
+@ALL_MODES
 def test_batch_tuner_choices_and_results(mode):
     _say("case: batch tuner")
     seed = 23
@@ -214,8 +237,8 @@ def test_batch_tuner_choices_and_results(mode):
     bt.tune(joins, trial=2)
     for nb in sizes:
         c = int(np.floor(np.log2(int(nb))))
-        bits, passes, lf = bt.config(c)
-        assert 4 <= bits <= 12 and passes in (1, 2) and 0.5 <= round(lf, 2) <= 0.9
+        bits, passes, lf, piece = bt.config(c)
+        assert 4 <= bits <= 12 and passes == 1 and piece in (0, 32768, 262144)
     bt.prepare(joins)
     bt.run(mode)
     cnt, _ = bt.results()
@@ -223,7 +246,7 @@ def test_batch_tuner_choices_and_results(mode):
     bt.close()
 
 
-@MODES
+@ALL_MODES
 def test_batch_repeated_keys_match_oracle(mode):
     """Joins with repeated build keys (many-to-many) against HashJoinOp."""
     _say("case: batch, repeated keys")
@@ -256,7 +279,7 @@ def test_batch_repeated_keys_match_oracle(mode):
     bt.close()
 
 
-@MODES
+@ALL_MODES
 def test_batch_output_overflow_reported(mode):
     _say("case: batch, output too small")
     seed = 25

@@ -642,10 +642,14 @@ def measure_ceilings(args):
     del perm
     for bits in (8, 11, 12, 14):
         run("histogram", bits, note="%d bins" % (1 << bits))
-    for bits in (18, 22, 24, 25, 26, 27, 28):   # tables of 4 MiB .. 4 GiB
-        run("lookup", bits, note="table of %d MiB" % ((16 << bits) >> 20))
-    for bits in (20, 24, 25, 26):
-        run("probe", bits, note="stream in, look up in %d MiB, stream out" % ((16 << bits) >> 20))
+    def size(bits):
+        b = 16 << bits
+        return "%d MiB" % (b >> 20) if b >= 1 << 20 else "%d KiB" % (b >> 10)
+
+    for bits in (12, 14, 16, 17, 18, 19, 20, 22, 24, 25, 26, 27, 28):   # tables of 64 KiB .. 4 GiB
+        run("lookup", bits, note="table of %s" % size(bits))
+    for bits in (12, 14, 16, 17, 18, 19, 20, 24, 25, 26):
+        run("probe", bits, note="stream in, look up in %s, stream out" % size(bits))
     for addrs in (1, 64, 2048, 65536):
         run("atomic", addrs, note="%d counters" % addrs)
     best = {k: max(r["gbps"] for r in rows if r["kind"] == k) for k in ("read", "write", "copy")}

@@ -438,6 +438,11 @@ def load_from_files(args, P, torch, db, nb, dp, npr, stream):
                     "parity check covers them"}
 
 
+def layout_flags(args, P):
+    """PJ_FLAG_LAYOUT_* bits of --layout (auto: the library chooses)."""
+    return {"auto": 0, "exact": P.PJ_FLAG_LAYOUT_EXACT, "append": P.PJ_FLAG_LAYOUT_APPEND}[args.layout]
+
+
 def lib_sha():
     """First 16 hex digits of the SHA-256 of the built library."""
     h = hashlib.sha256()
@@ -760,7 +765,7 @@ def compare_learned(args, P, torch, dev, stream, with_cpu):
     res = {"workload": "clustered_64Mx512M_z1", "build_tuples": nb, "probe_tuples": npr}
     for part in ("radix", "learned"):
         j = P.Join(P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                               partitioner=part, timing=True))
+                               partitioner=part, timing=True, flags=layout_flags(args, P)))
         for _ in range(max(1, args.warmup)):
             n = j.run_device(db.data_ptr(), nb, dp.data_ptr(), npr, out.data_ptr(), npr, stream)
         torch.cuda.synchronize()
@@ -778,6 +783,7 @@ def compare_learned(args, P, torch, dev, stream, with_cpu):
         res[part] = {"ms_per_step": ms, "value": (nb + npr) / (ms * 1e-3), "phases_ms": ph,
                      "used_learned": int(t.used_learned), "partitions": int(t.partitions),
                      "work_items": int(t.work_items),
+                     "probe_layout": "appended" if t.probe_appended else "contiguous",
                      "parity": {"count": n, "checksum": s, "matches_closed_form": (n, s) == (ec, es)}}
         _say("learned_vs_radix: %s %.2f ms/step" % (part, ms))
         j.close()
@@ -802,6 +808,9 @@ def main():
                     sorted(BATCH_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
+    ap.add_argument("--layout", default="auto", choices=["auto", "exact", "append"],
+                    help="probe-side partition layout: the library's choice, contiguous after a histogram pass, or "
+                         "appended chunks listed by partition")
     ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key", "count_by_key_fused"],
                     help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
                          "per build key (int64-key workloads); count_by_key_fused: the count per build key "
@@ -858,7 +867,7 @@ def main():
         P.check(P.lib().pj_gen_clustered(table.ctypes.data_as(C.c_void_p), npr, cdf.ctypes.data_as(C.c_void_p),
                                          C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                           partitioner=args.partitioner, timing=True)
+                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P))
     elif zipf:
         seed, nb, npr = ZIPF_WORKLOADS[args.workload]
         nmiss, dom = 0, None
@@ -868,7 +877,8 @@ def main():
         P.check(P.lib().pj_gen_pairs_zipf(seed, nb, npr, cdf.ctypes.data_as(C.c_void_p), C.c_void_p(db.data_ptr()),
                                           C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["int", "int", "long"], ["int", "int", "long"], [0, 1], [0, 1],
-                           ["B$0", "B$1", "B$2", "P$2"], partitioner=args.partitioner, timing=True)
+                           ["B$0", "B$1", "B$2", "P$2"], partitioner=args.partitioner, timing=True,
+                           flags=layout_flags(args, P))
     else:
         seed, dom, nb, npr, nmiss = WORKLOADS[args.workload]
         db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
@@ -876,7 +886,7 @@ def main():
         P.check(P.lib().pj_gen_pkfk(seed, dom, nb, npr, nmiss, C.c_void_p(db.data_ptr()),
                                     C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                           partitioner=args.partitioner, timing=True)
+                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P))
     load_res = None
     if args.from_files:
         load_res = load_from_files(args, P, torch, db, nb, dp, npr, stream)
@@ -1039,7 +1049,8 @@ def main():
                    "tuple_bytes": 16, "out_tuple_bytes": w_out, "out_tuples": n,
                    "partitioner": args.partitioner, "partitions": t.partitions,
                    "subpartitions": t.subpartitions, "work_items": t.work_items,
-                   "used_learned": t.used_learned, "spilled_items": t.spilled_items},
+                   "used_learned": t.used_learned, "spilled_items": t.spilled_items,
+                   "probe_layout": "appended" if t.probe_appended else "contiguous"},
         "hbm_gbs_algorithmic": job_gbs,
         "hbm_frac_of_peak": job_gbs / HBM_PEAK_GBS,
         "hbm_gbs_inputs_only": (nb + npr) * w_in / (ms * 1e-3) / 1e9,

@@ -74,6 +74,13 @@ typedef struct {
                                            LONG / DATE column or two INTEGER columns, wider projections):
                                            keep 10 bits of the key hash, so that unequal keys share hashes
                                            and the key check on the tuples decides (a test of that check) */
+#define PJ_FLAG_LAYOUT_EXACT 4u         /* partition the probe side after a histogram pass, into contiguous
+                                           partitions, whatever its size */
+#define PJ_FLAG_LAYOUT_APPEND 8u        /* partition the probe side without a histogram pass (each block
+                                           appends whole chunks, then the chunks are listed by partition),
+                                           whatever its size, where the join allows it (radix partitions
+                                           read once, 16-byte records or the row-id form); by default
+                                           probe sides of 2^22 tuples or more take it where allowed */
 
 typedef struct pj_join_s* pj_join_t;
 
@@ -91,6 +98,7 @@ typedef struct {
     uint64_t spilled_items;    /* work items whose R part exceeded one LDS table */
     float build_hist_ms;       /* of build_partition_ms: the histogram and its scan */
     float probe_hist_ms;       /* of probe_partition_ms: the histogram and its scan */
+    uint32_t probe_appended;   /* 1 if the probe side took the appended layout (no histogram pass) */
 } pj_timings;
 
 /* Resumable probe cursor: the state HashJoinOp keeps between getNext calls

@@ -131,6 +131,7 @@ struct PlanView {
     const uint32_t* sst;         // probe partition starts
     const uint32_t* coarse_of;   // range plans: probe partition of each build partition
     const PlanPart* pp;          // range plans: table of each build partition (null: radix)
+    const uint32_t* cl;          // appended probe layout: chunk numbers, partition by partition (null: contiguous)
     uint64_t chunk;              // probe records a piece, about
     uint32_t FR, shift;          // build partitions; radix: probe partition = p >> shift
 };

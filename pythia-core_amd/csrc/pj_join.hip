@@ -159,14 +159,37 @@ constexpr int kWcThreads = 1024;
 // partitions took 5.3 ms for a 2044-way scatter of 512M clustered records
 // where halves of the key order took 6.1 ms (hash partitions: 4.6 ms in
 // halves, 5.2 ms alternating).
-template <int RM, int B, bool RANGE, int SPLIT>
+//
+// APP (the appended layout, radix partitions): no histogram pass before it.
+// Slot k of block b is chunk k * gridDim.x + b of out: the blocks append
+// their chunks side by side at about the same pace. Every chunk a block
+// writes takes its next slot and is tagged with its partition in
+// app.cmap[b * app.region + k] (region: the input chunk's records in whole
+// chunks plus one partial chunk a partition). Lane l of wave w owns
+// partition w * 64 + l: it claims the slots of its partition's chunks each
+// round (a wave-wide scan of the owners' counts, one LDS atomic a wave) and
+// keeps the round's first slot and overflow records in LDS, from which the
+// records that leave as overflow or refill the buffer find their place. At
+// the end the block stores its chunks a partition, app.bhist[p * gridDim.x
+// + b], and its slots used, app.nchunks[b]; k_chunk_place then lists the
+// chunks partition by partition.
+struct AppendOut {
+    uint16_t* cmap;      // partition of each chunk slot
+    uint32_t* bhist;     // F x gridDim.x chunk counts, partition-major
+    uint32_t* nchunks;   // slots each block used
+    uint64_t region;     // chunk slots a block
+};
+
+template <int RM, int B, bool RANGE, int SPLIT, bool APP>
 __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __restrict__ in, uint64_t n,
                                                             SideLayout L, PartFn pf, uint64_t chunk,
                                                             uint32_t nblk, const uint32_t* __restrict__ offs,
                                                             uint4* __restrict__ out, uint4 pad_other, uint4 pad_zero,
-                                                            const uint16_t* __restrict__ pid)
+                                                            const uint16_t* __restrict__ pid, AppendOut app)
 {
     static_assert(SPLIT == 1 || SPLIT == 2, "one or two blocks an input chunk");
+    static_assert(!APP || (!RANGE && 64 % B == 0 && kWcThreads == 1024), "appended: radix, a lane a partition");
+    // (APP: at most 1024 partitions a block, so a partition fits 11 bits)
     constexpr int IT = SPLIT == 1 ? 4 : 8;   // records a thread per round
     extern __shared__ uint4 sm4[];
     const uint32_t F = pf.F;
@@ -179,16 +202,46 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     auto global_part = [&](uint32_t q) { return ALT ? 2 * q + half : p0 + q; };
     uint4* buf = sm4;                                                          // FL * B
     uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + FL * B);                 // FL: buffered records
-    uint32_t* goff = cnt + FL;                                                 // FL: next chunk's record index
+    uint32_t* goff = cnt + FL;   // FL: next chunk's record index (APP: the round's first chunk slot)
+    uint32_t* nch = goff + FL;   // APP, FL: chunks written a partition
+    uint32_t* ndir = nch + FL;   // APP, FL: the round's overflow records that leave in whole chunks
+    uint32_t* ccur = ndir + FL;  // APP: chunk slots the block has used
+    const uint64_t rbase = APP ? (uint64_t)blockIdx.x * app.region : 0;   // APP: the block's first tag
+    auto slot_rec = [&](uint64_t slot) -> uint64_t { return (rbase + slot) * B; };
     for (uint32_t p = threadIdx.x; p < pn; p += kWcThreads) {
-        goff[p] = offs[(uint64_t)global_part(p) * nblk + part];
+        if (APP)
+            nch[p] = 0;
+        else
+            goff[p] = offs[(uint64_t)global_part(p) * nblk + part];
         cnt[p] = 0;
     }
+    if (APP && threadIdx.x == 0)
+        *ccur = 0;
     __syncthreads();
     const uint64_t beg = (uint64_t)part * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
+    // APP: own_slots(n), called by every lane of a wave with its own
+    // partition's chunk count, claims the wave's slots with one LDS atomic
+    // and returns the lane's first slot; group g of the wave then writes the
+    // wave's partitions g, g + 64 / B, ... with the owners' slots and counts.
+    const uint32_t wl = threadIdx.x & 63u, wq0 = threadIdx.x & ~63u, gi = wl / B;
+    // The wave's exclusive scan of the counts runs bit by bit over ballots
+    // (a few bits: counts are small), without LDS round trips.
+    const uint64_t below = wl ? ~0ull >> (64 - wl) : 0ull;
+    auto own_slots = [&](uint32_t nq) -> uint32_t {
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t b = 0; __ballot(nq >> b) != 0; ++b) {
+            const uint64_t m = __ballot((nq >> b) & 1u);
+            pre += (uint32_t)__popcll(m & below) << b;
+            tot += (uint32_t)__popcll(m) << b;
+        }
+        uint32_t base = 0;
+        if (wl == 0 && tot)
+            base = atomicAdd(ccur, tot);
+        return __builtin_amdgcn_readfirstlane(base) + pre;
+    };
     const bool have_pid = RANGE;   // range partitions come from the histogram pass
     // A split pair reads its input chunk twice: ordinary loads keep the
     // lines in the XCD's L2 for the partner, where streaming loads would
@@ -224,42 +277,98 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                     nxtp[k] = pid[i2];
             }
         }
-        // claim
+        // claim: the round's LDS atomics leave back to back, the buffer
+        // writes wait for their results after
 #pragma unroll
         for (int k = 0; k < IT; ++k)
-            if (v[k]) {
+            if (v[k])
                 s[k] = atomicAdd(&cnt[p[k]], 1u);
-                if (s[k] < (uint32_t)B)
-                    buf[p[k] * B + s[k]] = cur[k];
-            }
-        __syncthreads();
-        // full buffers leave; whole chunks of overflow go straight out
-        for (uint32_t q = grp; q < pn; q += ngrp)
-            if (cnt[q] >= (uint32_t)B)
-                out[goff[q] + gl] = buf[q * B + gl];
-        uint32_t keep[IT];
 #pragma unroll
-        for (int k = 0; k < IT; ++k) {
-            keep[k] = ~0u;
-            if (v[k] && s[k] >= (uint32_t)B) {
-                const uint32_t rank = s[k] - B;
-                const uint32_t direct = (cnt[p[k]] - B) / B * B;   // overflow records in whole chunks
-                if (rank < direct)
-                    out[goff[p[k]] + B + rank] = cur[k];
-                else
-                    keep[k] = rank - direct;
-            }
+        for (int k = 0; k < IT; ++k)
+            if (v[k] && s[k] < (uint32_t)B)
+                buf[p[k] * B + s[k]] = cur[k];
+        // APP: record k's partition and claim in one word (~0: none), which
+        // frees the registers of both for the slot claims below
+        uint32_t ps[IT];
+        if (APP) {
+#pragma unroll
+            for (int k = 0; k < IT; ++k)
+                ps[k] = v[k] ? p[k] | (s[k] << 11) : ~0u;
         }
         __syncthreads();
+        // full buffers leave; whole chunks of overflow go straight out
+        if (APP) {
+            // the owner of each partition claims its chunks' slots (the
+            // buffer's chunk, then the overflow's) and keeps them in LDS
+            {
+                const uint32_t qo = wq0 + wl;
+                const uint32_t co = qo < pn ? cnt[qo] : 0u;
+                const uint32_t no = co >= (uint32_t)B ? 1 + (co - B) / B : 0u;
+                const uint32_t so = own_slots(no);
+                if (no) {
+                    goff[qo] = so;
+                    ndir[qo] = (no - 1) * B;
+                    nch[qo] += no;
+                }
+            }
+            // the groups read the slots their wave's owners wrote to LDS
+            // (one wave's LDS accesses complete in order)
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t j = gi; j < 64; j += 64 / B) {
+                const uint32_t q = wq0 + j;
+                const uint32_t c = q < pn ? cnt[q] : 0u;
+                if (c >= (uint32_t)B) {
+                    const uint32_t slot = goff[q], nq = 1 + (c - B) / B;
+                    out[slot_rec(slot) + gl] = buf[q * B + gl];
+                    for (uint32_t t = gl; t < nq; t += B)
+                        app.cmap[rbase + slot + t] = (uint16_t)global_part(q);
+                }
+            }
+            __syncthreads();
+            // overflow records: whole chunks leave at their slots, the rest
+            // refill the buffer. (The two stores differ in kind: two plain
+            // stores of one value merge into a flat store to either address.)
 #pragma unroll
-        for (int k = 0; k < IT; ++k)
-            if (keep[k] != ~0u)
-                buf[p[k] * B + keep[k]] = cur[k];
+            for (int k = 0; k < IT; ++k)
+                if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {
+                    const uint32_t pk = ps[k] & 0x7FFu, rank = (ps[k] >> 11) - B, direct = ndir[pk];
+                    if (rank < direct) {
+                        const u32x4 r = {cur[k].x, cur[k].y, cur[k].z, cur[k].w};
+                        __builtin_nontemporal_store(
+                            r, reinterpret_cast<u32x4*>(out + slot_rec(goff[pk] + 1 + rank / B) + rank % B));
+                    } else {
+                        buf[pk * B + rank - direct] = cur[k];
+                    }
+                }
+        } else {
+            for (uint32_t q = grp; q < pn; q += ngrp)
+                if (cnt[q] >= (uint32_t)B)
+                    out[goff[q] + gl] = buf[q * B + gl];
+            uint32_t keep[IT];
+#pragma unroll
+            for (int k = 0; k < IT; ++k) {
+                keep[k] = ~0u;
+                if (v[k] && s[k] >= (uint32_t)B) {
+                    const uint32_t rank = s[k] - B;
+                    const uint32_t direct = (cnt[p[k]] - B) / B * B;   // overflow records in whole chunks
+                    if (rank < direct)
+                        out[goff[p[k]] + B + rank] = cur[k];
+                    else
+                        keep[k] = rank - direct;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < IT; ++k)
+                if (keep[k] != ~0u)
+                    buf[p[k] * B + keep[k]] = cur[k];
+        }
         for (uint32_t q = threadIdx.x; q < pn; q += kWcThreads) {
             const uint32_t c = cnt[q];
             if (c >= (uint32_t)B) {
                 const uint32_t direct = (c - B) / B * B;
-                goff[q] += B + direct;
+                if (!APP)
+                    goff[q] += B + direct;
                 cnt[q] = c - B - direct;
             }
         }
@@ -281,7 +390,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     // registers taking turns (no copies), or, with the partition ids as well
     // (range partitions), one a round with a copy, which needs fewer
     // registers. The chunk's last rounds with bounds checks.
-    if (!have_pid) {
+    if (!have_pid && !APP) {
         for (; t0 + 3 * R <= end; t0 += 2 * R) {
             round(t0, ra, pa, rb, pb, std::true_type{});
             round(t0 + R, rb, pb, ra, pa, std::true_type{});
@@ -309,17 +418,117 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     // chosen word by word: a select between an LDS record and an argument
     // record compiles to a flat load from either address, with the argument
     // copied to scratch memory.
-    for (uint32_t q = grp; q < pn; q += ngrp) {
+    auto tail = [&](uint32_t q, uint64_t at) {
         const uint32_t c = cnt[q];
-        if (c > 0) {
-            const uint4 r = buf[q * B + gl];
-            const bool keep = gl < c, z = global_part(q) == 0;
-            const uint4 pad = make_uint4(z ? pad_zero.x : pad_other.x, z ? pad_zero.y : pad_other.y,
-                                         z ? pad_zero.z : pad_other.z, z ? pad_zero.w : pad_other.w);
-            out[goff[q] + gl] = make_uint4(keep ? r.x : pad.x, keep ? r.y : pad.y, keep ? r.z : pad.z,
-                                           keep ? r.w : pad.w);
+        const uint4 r = buf[q * B + gl];
+        const bool keep = gl < c, z = global_part(q) == 0;
+        const uint4 pad = make_uint4(z ? pad_zero.x : pad_other.x, z ? pad_zero.y : pad_other.y,
+                                     z ? pad_zero.z : pad_other.z, z ? pad_zero.w : pad_other.w);
+        out[at + gl] = make_uint4(keep ? r.x : pad.x, keep ? r.y : pad.y, keep ? r.z : pad.z, keep ? r.w : pad.w);
+    };
+    if (APP) {
+        const uint32_t qo = wq0 + wl;
+        const uint32_t no = qo < pn && cnt[qo] > 0 ? 1u : 0u;
+        const uint32_t so = own_slots(no);
+        if (no) {
+            nch[qo] += 1;
+            app.cmap[rbase + so] = (uint16_t)global_part(qo);
         }
+        for (uint32_t j = gi; j < 64; j += 64 / B) {
+            const uint32_t nq = __shfl(no, j, 64), slot = __shfl(so, j, 64);
+            if (nq)
+                tail(wq0 + j, slot_rec(slot));
+        }
+        __syncthreads();
+        // chunk counts partition-major; the partner block's partitions read 0
+        for (uint32_t gp = threadIdx.x; gp < F; gp += kWcThreads) {
+            uint32_t c = 0;
+            if (SPLIT == 1)
+                c = nch[gp];
+            else if ((gp >= FL) == (half != 0))
+                c = nch[gp - p0];
+            app.bhist[(uint64_t)gp * gridDim.x + blockIdx.x] = c;
+        }
+        if (threadIdx.x == 0)
+            app.nchunks[blockIdx.x] = *ccur;
+    } else {
+        for (uint32_t q = grp; q < pn; q += ngrp)
+            if (cnt[q] > 0)
+                tail(q, goff[q]);
     }
+}
+
+// The appended layout's chunk list: block b of the scatter's grid moves the
+// numbers of its chunks (slot k is chunk k * NB + b) to cl, each at the next
+// place of its partition in this block's share (offs: the chunk counts
+// scanned, partition-major), so that cl lists partition 0's chunks, then
+// partition 1's, and so on. A tile of the block's slots is counting-sorted by
+// partition in LDS first, so that consecutive lanes write a partition's
+// consecutive places.
+constexpr uint32_t kPlaceThreads = 1024, kPlaceTile = 18432, kPlaceMaxF = 4096;
+
+__global__ __launch_bounds__(kPlaceThreads) void k_chunk_place(const uint16_t* __restrict__ cmap,
+                                                                const uint32_t* __restrict__ nchunks,
+                                                                const uint32_t* __restrict__ offs, uint32_t F,
+                                                                uint64_t region, uint32_t* __restrict__ cl)
+{
+    __shared__ uint32_t idx[kPlaceTile];          // the tile's slots, sorted by partition
+    __shared__ uint16_t tpart[kPlaceTile];        // their partitions
+    __shared__ uint32_t cnt[kPlaceMaxF + 1];      // the tile's chunks a partition, then their starts
+    __shared__ uint32_t gpos[kPlaceMaxF];         // next place of each partition in cl
+    __shared__ uint32_t wsum[kPlaceThreads / 64 + 1];
+    constexpr uint32_t PER = kPlaceTile / kPlaceThreads;
+    const uint32_t b = blockIdx.x, NB = gridDim.x, tid = threadIdx.x;
+    for (uint32_t p = tid; p < F; p += kPlaceThreads)
+        gpos[p] = offs[(uint64_t)p * NB + b];
+    const uint16_t* tags = cmap + (uint64_t)b * region;
+    const uint32_t nc = nchunks[b];
+    for (uint32_t t0 = 0; t0 < nc; t0 += kPlaceTile) {
+        const uint32_t tn = nc - t0 < kPlaceTile ? nc - t0 : kPlaceTile;
+        for (uint32_t p = tid; p <= F; p += kPlaceThreads)
+            cnt[p] = 0;
+        __syncthreads();
+        uint32_t part[PER], rank[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t i = j * kPlaceThreads + tid;
+            if (i < tn) {
+                part[j] = tags[t0 + i];
+                rank[j] = atomicAdd(&cnt[part[j]], 1u);
+            }
+        }
+        __syncthreads();
+        block_scan_array<kPlaceThreads>(cnt, F, wsum);   // cnt[p]: the tile's first place of p
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j) {
+            const uint32_t i = j * kPlaceThreads + tid;
+            if (i < tn) {
+                idx[cnt[part[j]] + rank[j]] = t0 + i;
+                tpart[cnt[part[j]] + rank[j]] = (uint16_t)part[j];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < tn; i += kPlaceThreads) {
+            const uint32_t p = tpart[i];
+            cl[gpos[p] + (i - cnt[p])] = (uint32_t)(b * region) + idx[i];
+        }
+        __syncthreads();
+        for (uint32_t p = tid; p < F; p += kPlaceThreads)
+            gpos[p] += cnt[p + 1] - cnt[p];
+        __syncthreads();
+    }
+}
+
+// Partition starts of the appended layout in records: part_start[p] =
+// records before partition p's first listed chunk; part_start[F] = all.
+__global__ void k_chunk_bounds(const uint32_t* __restrict__ offs, uint32_t F, uint32_t NB,
+                               const uint32_t* __restrict__ total, uint32_t* __restrict__ part_start)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < F)
+        part_start[p] = offs[(uint64_t)p * NB] * kChunkRecs;
+    if (p == F)
+        part_start[F] = *total * kChunkRecs;
 }
 
 constexpr int kScatterItems = 16;
@@ -577,8 +786,14 @@ __device__ __forceinline__ Item plan_item(const PlanView& v, uint32_t it)
     Item I;
     I.r_off = r0 + rs * cap;
     I.r_cnt = r - rs * cap < cap ? r - rs * cap : cap;
-    I.s_off = s0 + (uint32_t)(sn * k / np);
-    I.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
+    if (v.cl) {   // listed chunks: pieces of whole chunks (partition starts and sizes are whole chunks)
+        const uint64_t nc = sn / kChunkRecs;
+        I.s_off = s0 + (uint32_t)(nc * k / np) * kChunkRecs;
+        I.s_cnt = (uint32_t)(nc * (k + 1) / np - nc * k / np) * kChunkRecs;
+    } else {
+        I.s_off = s0 + (uint32_t)(sn * k / np);
+        I.s_cnt = (uint32_t)(sn * (k + 1) / np - sn * k / np);
+    }
     I.part = p;
     I.fmt = v.pp ? v.pp[p].fmt : 0u;
     I.lo = v.pp ? v.pp[p].lo : 0ull;
@@ -641,8 +856,12 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(PlanView v, uint32_t* __r
 // table formats' probe loops: with the radix kernel's 8 it spills registers
 // to scratch inside the probe loop.
 constexpr int kRangeProbeItems = 6;
+// Probe records in flight a thread when the probe side is listed by chunk
+// (the appended layout): the chunk numbers and their addresses take the
+// registers of one record (8 spill 6 words to scratch).
+constexpr int kListProbeItems = 7;
 
-template <bool RANGE>
+template <bool RANGE, bool CL>
 __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__ R, const uint4* __restrict__ S,
                                                         PlanView plan, uint32_t* __restrict__ item_ctr,
                                                         uint32_t* __restrict__ out, uint64_t out_cap,
@@ -669,7 +888,10 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
         if (it >= nitems)
             break;
         const Item I = plan_item(plan, it);
-        const uint4* Sp = S + I.s_off;
+        // CL: the probe records are listed by chunk from plan.cl (whole
+        // chunks, so I.s_off is a multiple of the chunk's records)
+        const uint4* Sp = CL ? S : S + I.s_off;
+        const uint32_t* clp = CL ? plan.cl + I.s_off / kChunkRecs : nullptr;
 
         if (!RANGE) {
             uint64_t* tkey = table;                                            // kTableCap
@@ -686,7 +908,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
                     tpay[j] = make_uint2(r.z, r.w);
                 });
             const RadixTab tab{tkey, tpay, reinterpret_cast<const uint16_t*>(dirw), bitsR, bkt_shift, I.part};
-            probe_rounds<kJoinThreads>(tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2);
+            probe_rounds<kJoinThreads, RadixTab, CL ? kListProbeItems : kProbeItems, CL>(
+                tab, Sp, I.s_cnt, out, out_cap, out_cursor, fmt, kbp, wsum, misc + 2, clp);
         } else {
             uint2* tpay = reinterpret_cast<uint2*>(table);
             bool dense = I.fmt == kFmtDense;
@@ -720,8 +943,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
             }
             if (dense) {
                 const DenseTab tab{tpay, I.lo, I.hi, I.base, I.r_cnt};
-                probe_rounds<kJoinThreads, DenseTab, kRangeProbeItems>(tab, Sp, I.s_cnt, out, out_cap, out_cursor,
-                                                                       fmt, kbp, wsum, misc + 2);
+                probe_rounds<kJoinThreads, DenseTab, kRangeProbeItems, CL>(tab, Sp, I.s_cnt, out, out_cap,
+                                                                           out_cursor, fmt, kbp, wsum, misc + 2, clp);
             } else {
                 uint32_t* tcode = reinterpret_cast<uint32_t*>(tpay + kCodeCap);   // kCodeCap
                 uint32_t* dirw = tcode + kCodeCap;                                 // kTableBuckets / 2 + 1
@@ -734,8 +957,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
                         tpay[j] = make_uint2(r.z, r.w);
                     });
                 const CodeTab tab{tcode, tpay, reinterpret_cast<const uint16_t*>(dirw), I.lo, I.hi, I.base};
-                probe_rounds<kJoinThreads, CodeTab, kRangeProbeItems>(tab, Sp, I.s_cnt, out, out_cap, out_cursor,
-                                                                      fmt, kbp, wsum, misc + 2);
+                probe_rounds<kJoinThreads, CodeTab, kRangeProbeItems, CL>(tab, Sp, I.s_cnt, out, out_cap,
+                                                                          out_cursor, fmt, kbp, wsum, misc + 2, clp);
             }
         }
         __syncthreads();
@@ -1534,6 +1757,9 @@ struct pj_join_s {
     uint32_t* host_ctr = nullptr;   // pinned: the join's counters, read back after each probe
     DevBuf gj_count, gj_rows;       // group-join: matches a build record, merged rows
     DevBuf pairs;                   // row-id form: the join's (build row, probe row) pairs
+    // the probe side's appended layout: chunk tags, counts, chunk list
+    DevBuf a_cmap, a_bhist, a_bsum, a_nchunks, a_clist;
+    bool s_app = false;             // the last probe side took the appended layout
     uint64_t pair_cap = 0;          // pairs the last probe phase could hold
     uint64_t pair_want = 0;         // pairs a previous run needed
 };
@@ -1628,27 +1854,28 @@ int allow_lds(const void* fn, size_t bytes)
     return PJ_OK;
 }
 
-template <int RM, int B, bool RANGE, int SPLIT>
+template <int RM, int B, bool RANGE, int SPLIT, bool APP>
 int launch_wc_rm(const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
                  uint32_t nblk, const uint32_t* offs, uint4* out, uint4 pad_other, uint4 pad_zero, const uint16_t* pid,
-                 size_t lds, hipStream_t st)
+                 const AppendOut& app, size_t lds, hipStream_t st)
 {
-    if (int rc = allow_lds((const void*)k_scatter_wc<RM, B, RANGE, SPLIT>, lds))
+    if (int rc = allow_lds((const void*)k_scatter_wc<RM, B, RANGE, SPLIT, APP>, lds))
         return rc;
-    hipLaunchKernelGGL((k_scatter_wc<RM, B, RANGE, SPLIT>), dim3(nblk * SPLIT), dim3(kWcThreads), lds, st, in, n, L,
-                       pf, chunk, nblk, offs, out, pad_other, pad_zero, pid);
+    hipLaunchKernelGGL((k_scatter_wc<RM, B, RANGE, SPLIT, APP>), dim3(nblk * SPLIT), dim3(kWcThreads), lds, st, in, n,
+                       L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid, app);
     PJ_HIP(hipGetLastError());
     return PJ_OK;
 }
 
-template <int B, bool RANGE, int SPLIT>
+template <int B, bool RANGE, int SPLIT, bool APP>
 int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, const PartFn& pf, uint64_t chunk,
-                uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, hipStream_t st)
+                uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid, const AppendOut& app,
+                hipStream_t st)
 {
     const uint32_t FL = SPLIT == 1 ? pf.F : (pf.F + 1) / 2;
     if (SPLIT > 1 && nblk % 8)
         return fail(PJ_ERR_INVALID, "split scatter needs a multiple of 8 input chunks");
-    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4;
+    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4 + (APP ? 2 * (size_t)FL * 4 + 16 : 0);
     uint4 pad_other, pad_zero;
     if (RANGE) {
         // ordered key 0 lies in partition 0, ordered key ~0 in the last one
@@ -1660,13 +1887,13 @@ int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, con
         pad_zero = make_uint4((uint32_t)pk, (uint32_t)(pk >> 32), 0, 0);
     }
     if (rm == REC_IDENT)
-        return launch_wc_rm<REC_IDENT, B, RANGE, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
-                                                        lds, st);
-    if (RANGE || rm == REC_WORDS)
-        return launch_wc_rm<REC_WORDS, B, RANGE, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
-                                                        lds, st);
-    return launch_wc_rm<REC_HASH, B, false, SPLIT>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero, pid,
-                                                   lds, st);
+        return launch_wc_rm<REC_IDENT, B, RANGE, SPLIT, APP>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero,
+                                                             pid, app, lds, st);
+    if (!APP && (RANGE || rm == REC_WORDS))   // (the appended layout takes no word-by-word tuples)
+        return launch_wc_rm<REC_WORDS, B, RANGE, SPLIT, false>(in, n, L, pf, chunk, nblk, offs, out, pad_other,
+                                                               pad_zero, pid, app, lds, st);
+    return launch_wc_rm<REC_HASH, B, false, SPLIT, APP>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero,
+                                                        pid, app, lds, st);
 }
 
 template <int B>
@@ -1674,11 +1901,12 @@ int launch_wc(int rm, bool range, uint32_t split, const uint32_t* in, uint64_t n
               const PartFn& pf, uint64_t chunk, uint32_t nblk, const uint32_t* offs, uint4* out, const uint16_t* pid,
               hipStream_t st)
 {
+    const AppendOut none = {};
     if (split > 1)
-        return range ? launch_wc_t<B, true, 2>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, st)
-                     : launch_wc_t<B, false, 2>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
-    return range ? launch_wc_t<B, true, 1>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, st)
-                 : launch_wc_t<B, false, 1>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, st);
+        return range ? launch_wc_t<B, true, 2, false>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, none, st)
+                     : launch_wc_t<B, false, 2, false>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, none, st);
+    return range ? launch_wc_t<B, true, 1, false>(rm, in, n, L, pf, chunk, nblk, offs, out, pid, none, st)
+                 : launch_wc_t<B, false, 1, false>(rm, in, n, L, pf, chunk, nblk, offs, out, nullptr, none, st);
 }
 
 template <int RM, bool RANGE>
@@ -1861,6 +2089,98 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
         return rc;
     if (host_copy)
         PJ_HIP(hipMemcpyAsync(P.start.data(), starts.p, (F + 1) * 4, hipMemcpyDeviceToHost, st));
+    return PJ_OK;
+}
+
+// Probe sides of at least this many tuples take the appended layout by
+// default (PJ_FLAG_LAYOUT_EXACT / _APPEND force either).
+constexpr uint64_t kAppendMin = 1ull << 22;
+// A scatter block's lanes own its partitions, one each: at most 1024 a
+// block, 2048 over a split pair.
+constexpr uint32_t kAppendMaxF = 2 * kWcThreads;
+static_assert(kAppendMaxF <= kPlaceMaxF, "the chunk list's tile sort holds every partition");
+
+// Whether the probe side takes the appended layout: radix partitions whose
+// build partitions are the probe partitions (the join reads each probe
+// partition once: listed chunks cost the join a record in flight a thread,
+// paid on every read), at least two partitions (one has no key outside it to
+// pad chunks with) and at most kAppendMaxF, 16-byte records or the row-id
+// form (the word-by-word variant spills registers), and not in a fixed
+// scratch (it needs about twice the room).
+bool want_append(const pj_join_s* h, const void* d_probe, uint64_t np)
+{
+    const SideLayout& L = h->lay.probe;
+    const bool words = !L.hashed && !(L.ident && ((uintptr_t)d_probe % 16 == 0));
+    if (np == 0 || h->range || h->bS == 0 || (1u << h->bS) > kAppendMaxF || h->bR != h->bS || words ||
+        (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) || h->spart.arena)
+        return false;
+    return (h->desc.flags & PJ_FLAG_LAYOUT_APPEND) || np >= kAppendMin;
+}
+
+// Partitions the probe side into the appended layout (radix partitions):
+// the write-combining scatter (8-record chunks) appends each block's chunks
+// without a histogram pass before it, then the chunk tags are counted,
+// scanned and placed into h->a_clist, partition by partition; sstarts[p] is
+// partition p's first record in the list's order (8 a chunk). Marks event 9
+// before the scatter.
+int partition_append(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf,
+                     hipStream_t st)
+{
+    int rc;
+    const uint32_t F = pf.F;
+    constexpr uint32_t B = kChunkRecs;
+    Partitioned& P = h->S;
+    P.F = F;
+    P.n = n;
+    P.start.clear();
+    if (n >= (1ull << 31))
+        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
+    if (F > kAppendMaxF || F < 2)
+        return fail(PJ_ERR_UNSUPPORTED, "the appended layout takes 2 to %u partitions", kAppendMaxF);
+    const uint32_t split = F > 1024 ? 2u : 1u;   // 8-record buffers of at most 1024 partitions a block
+    const uint32_t FL = split > 1 ? (F + 1) / 2 : F;
+    uint32_t nblk = part_blocks(n);
+    if (split > 1)
+        nblk = (nblk + 7) / 8 * 8;
+    const uint64_t chunk = (n + nblk - 1) / nblk;
+    const uint32_t NB = nblk * split;
+    const uint64_t region = (chunk + B - 1) / B + FL;   // the most chunks a block can write
+    const uint64_t slots = (uint64_t)NB * region;
+    const uint64_t m = (uint64_t)F * NB;
+    const uint32_t nsb = (uint32_t)((m + kScanSeg - 1) / kScanSeg);
+    if (slots * B >= (1ull << 32) || m >= (1ull << 32))
+        return fail(PJ_ERR_UNSUPPORTED, "appended layout exceeds 2^32 records");
+    if ((rc = h->sstarts.ensure((F + 1) * 4)) || (rc = h->spart.ensure(slots * B * 16)) ||
+        (rc = h->a_cmap.ensure(slots * 2)) || (rc = h->a_bhist.ensure(m * 4)) ||
+        (rc = h->a_bsum.ensure((nsb + 1) * 4)) || (rc = h->a_nchunks.ensure((size_t)NB * 4)) ||
+        (rc = h->a_clist.ensure(slots * 4)))
+        return rc;
+    const uint32_t* in = (const uint32_t*)d_in;
+    const int rm = L.hashed ? REC_HASH : (L.ident && ((uintptr_t)d_in % 16 == 0)) ? REC_IDENT : REC_WORDS;
+    if (rm == REC_WORDS)
+        return fail(PJ_ERR_INVALID, "the appended layout takes 16-byte records or the row-id form");
+    if (h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING))
+        PJ_HIP(hipEventRecord(h->ev[9], st));
+    AppendOut app;
+    app.cmap = (uint16_t*)h->a_cmap.p;
+    app.bhist = (uint32_t*)h->a_bhist.p;
+    app.nchunks = (uint32_t*)h->a_nchunks.p;
+    app.region = region;
+    uint4* out = (uint4*)h->spart.p;
+    rc = split > 1 ? launch_wc_t<B, false, 2, true>(rm, in, n, L, pf, chunk, nblk, nullptr, out, nullptr, app, st)
+                   : launch_wc_t<B, false, 1, true>(rm, in, n, L, pf, chunk, nblk, nullptr, out, nullptr, app, st);
+    if (rc)
+        return rc;
+    uint32_t* bh = (uint32_t*)h->a_bhist.p;
+    uint32_t* bsum = (uint32_t*)h->a_bsum.p;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanThreads), 0, st, bh, m, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nsb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nsb), dim3(kScanThreads), 0, st, bh, m, (const uint32_t*)bsum);
+    hipLaunchKernelGGL(k_chunk_place, dim3(NB), dim3(kPlaceThreads), 0, st, (const uint16_t*)h->a_cmap.p,
+                       (const uint32_t*)h->a_nchunks.p, (const uint32_t*)bh, F, region, (uint32_t*)h->a_clist.p);
+    hipLaunchKernelGGL(k_chunk_bounds, dim3((F + 1 + 255) / 256), dim3(256), 0, st, (const uint32_t*)bh, F, NB,
+                       (const uint32_t*)(bsum + nsb), (uint32_t*)h->sstarts.p);
+    PJ_HIP(hipGetLastError());
     return PJ_OK;
 }
 
@@ -2048,15 +2368,14 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     int rc;
     *empty = false;
     record(h, 2, st);
-    if (h->range) {
-        const PartFn pf = map_fn(h->plan.ncoarse, (const uint8_t*)h->map_dev.p, h->plan.div);
-        rc = partition_side(h, d_probe, np, h->lay.probe, pf, true, true, h->spart, h->S, h->sstarts, false, nullptr,
-                            st);
-    } else {
-        const PartFn pf = radix_fn(h->bS);
-        rc = partition_side(h, d_probe, np, h->lay.probe, pf, false, true, h->spart, h->S, h->sstarts, false, nullptr,
-                            st);
-    }
+    h->s_app = want_append(h, d_probe, np);
+    const PartFn spf =
+        h->range ? map_fn(h->plan.ncoarse, (const uint8_t*)h->map_dev.p, h->plan.div) : radix_fn(h->bS);
+    if (h->s_app)
+        rc = partition_append(h, d_probe, np, h->lay.probe, spf, st);
+    else
+        rc = partition_side(h, d_probe, np, h->lay.probe, spf, h->range, true, h->spart, h->S, h->sstarts, false,
+                            nullptr, st);
     if (rc)
         return rc;
     record(h, 3, st);
@@ -2074,6 +2393,7 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     v.sst = (const uint32_t*)h->sstarts.p;
     v.coarse_of = h->range ? (const uint32_t*)h->coarse_of.p : nullptr;
     v.pp = h->range ? (const PlanPart*)h->plan_parts.p : nullptr;
+    v.cl = h->s_app ? (const uint32_t*)h->a_clist.p : nullptr;
     v.chunk = probe_chunk(np);
     if (v.FR > kPlanThreads * kPlanPer)
         return fail(PJ_ERR_UNSUPPORTED, "%u build partitions exceed the planner", v.FR);
@@ -2086,7 +2406,8 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     unsigned long long* cursor = (unsigned long long*)(counters + 2);
     uint32_t* fail_ctr = counters + 4;
     const size_t lds = join_lds_bytes(h->range);
-    const void* kfn = h->range ? (const void*)k_join<true> : (const void*)k_join<false>;
+    const void* kfn = h->range ? (const void*)k_join<true, false>
+                               : (h->s_app ? (const void*)k_join<false, true> : (const void*)k_join<false, false>);
     if ((rc = allow_lds(kfn, lds)))
         return rc;
     const uint32_t bkt_shift = 64 - h->bR - 12;
@@ -2105,14 +2426,16 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     }
     record(h, 4, st);
     const uint32_t grid = (uint32_t)device_count_cus();   // blocks without an item exit at once
-    if (h->range)
-        hipLaunchKernelGGL(k_join<true>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, 0u,
-                           0u, fail_ctr);
+    const uint32_t jbits = h->range ? 0u : h->bR, jshift = h->range ? 0u : bkt_shift;
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
+                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, jbits,
+                           jshift, fail_ctr);
+    };
+    if (h->range)   // range plans keep the contiguous layout
+        launch(k_join<true, false>);
     else
-        hipLaunchKernelGGL(k_join<false>, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, h->bR,
-                           bkt_shift, fail_ctr);
+        h->s_app ? launch(k_join<false, true>) : launch(k_join<false, false>);
     PJ_HIP(hipGetLastError());
     if (h->lay.rowid &&
         (rc = rows_materialize((const uint2*)h->pairs.p, cursor, jcap, h->raw_build, d_probe, h->lay.rows, d_out,
@@ -2174,6 +2497,7 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     t.work_items = produced[6];
     t.used_learned = h->range ? 1u : 0u;
     t.spilled_items = produced[7];
+    t.probe_appended = h->s_app ? 1u : 0u;
     return PJ_OK;
 }
 

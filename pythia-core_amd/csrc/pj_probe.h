@@ -69,25 +69,60 @@ __device__ __forceinline__ void write_row(uint32_t* __restrict__ out, uint64_t r
 // each wave write adjacent output rows. Loads are software-pipelined: a
 // record's slot is refilled with the next round's record as soon as its
 // lookup is done.
-template <int NT, class T, int PI = kProbeItems>
+//
+// CL: the records are not contiguous but listed by chunk: record i is record
+// i % kChunkRecs of chunk clp[i / kChunkRecs] of Sp (chunks of kChunkRecs
+// records each). Lane l of a wave holds the chunk number of its slice's
+// chunk l, loaded two rounds ahead of the records it locates, and hands it
+// to the lanes that load that chunk's records.
+constexpr uint32_t kChunkRecs = 8;
+
+template <int NT, class T, int PI = kProbeItems, bool CL = false>
 __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restrict__ Sp, uint32_t s_cnt,
                                              uint32_t* __restrict__ out, uint64_t out_cap,
                                              unsigned long long* __restrict__ out_cursor, const OutFormat& fmt,
-                                             bool kbp, uint32_t* wsum, uint32_t* wbase_lds)
+                                             bool kbp, uint32_t* wsum, uint32_t* wbase_lds,
+                                             const uint32_t* __restrict__ clp = nullptr)
 {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     constexpr uint32_t kSlice = 64 * PI;
     constexpr uint32_t kRound = (NT / 64) * kSlice;
+    constexpr uint32_t kSliceChunks = kSlice / kChunkRecs;
+    static_assert(!CL || (kSlice % kChunkRecs == 0 && kSliceChunks <= 64), "a slice's chunks fit a wave");
+    // chunk number of the lane's chunk in this wave's slice of round r0
+    auto chunk_at = [&](uint32_t r0) -> uint32_t {
+        const uint32_t c = (r0 + wave * kSlice) / kChunkRecs + lane;
+        return lane < kSliceChunks && c * kChunkRecs < s_cnt ? clp[c] : 0u;
+    };
+    // address of item k of this lane in the slice whose chunk numbers are in ch
+    auto rec_at = [&](uint32_t ch, int k) -> const uint4* {
+        const uint32_t c = __shfl(ch, k * (64 / kChunkRecs) + (lane / kChunkRecs), 64);
+        return Sp + ((uint64_t)c * kChunkRecs + (lane % kChunkRecs));
+    };
+    uint32_t ch_next = 0;   // CL: chunk numbers of the next round's slice
     uint4 nxt[PI];
+    if (CL) {
+        const uint32_t ch0 = chunk_at(0);
+        ch_next = chunk_at(kRound);
 #pragma unroll
-    for (int k = 0; k < PI; ++k) {
-        const uint32_t i = wave * kSlice + k * 64 + lane;
-        if (i < s_cnt)
-            nxt[k] = ld_stream(Sp + i);
+        for (int k = 0; k < PI; ++k) {
+            const uint32_t i = wave * kSlice + k * 64 + lane;
+            const uint4* a = rec_at(ch0, k);
+            if (i < s_cnt)
+                nxt[k] = ld_stream(a);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < PI; ++k) {
+            const uint32_t i = wave * kSlice + k * 64 + lane;
+            if (i < s_cnt)
+                nxt[k] = ld_stream(Sp + i);
+        }
     }
     for (uint32_t r0 = 0; r0 < s_cnt; r0 += kRound) {
         const uint32_t t0 = r0 + wave * kSlice;
+        const uint32_t ch_after = CL ? chunk_at(r0 + 2 * kRound) : 0u;
         uint4 s[PI];
         uint32_t nm[PI], first[PI];
         bool multi = false;
@@ -100,9 +135,15 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
             if (i < s_cnt)
                 tab.lookup(s[k], nm[k], first[k]);
             multi |= nm[k] > 1;
-            if (i + kRound < s_cnt)
+            if (CL) {
+                const uint4* a = rec_at(ch_next, k);
+                if (i + kRound < s_cnt)
+                    nxt[k] = ld_stream(a);
+            } else if (i + kRound < s_cnt) {
                 nxt[k] = ld_stream(Sp + i + kRound);
+            }
         }
+        ch_next = ch_after;
         const bool wmulti = __any(multi);
         uint64_t m[PI];
         uint32_t mine = 0, pre = 0, wtotal = 0;

@@ -25,6 +25,8 @@ PJ_MAX_KEYS = 8
 PART = {"radix": 0, "learned": 1, "auto": 2}
 PJ_FLAG_TIMING = 1
 PJ_FLAG_NARROW_ROW_HASH = 2
+PJ_FLAG_LAYOUT_EXACT = 4
+PJ_FLAG_LAYOUT_APPEND = 8
 
 
 class Schema(C.Structure):
@@ -45,7 +47,8 @@ class Timings(C.Structure):
                 ("join_ms", C.c_float), ("model_ms", C.c_float), ("total_ms", C.c_float),
                 ("partitions", C.c_uint32), ("subpartitions", C.c_uint32),
                 ("work_items", C.c_uint32), ("used_learned", C.c_uint32),
-                ("spilled_items", C.c_uint64), ("build_hist_ms", C.c_float), ("probe_hist_ms", C.c_float)]
+                ("spilled_items", C.c_uint64), ("build_hist_ms", C.c_float), ("probe_hist_ms", C.c_float),
+                ("probe_appended", C.c_uint32)]
 
 
 PJ_MODEL_LEAVES = 4096
@@ -225,8 +228,9 @@ def make_schema(cols):
 
 
 def make_desc(build_cols, probe_cols, buildjattr, probejattr, projection,
-              partitioner="radix", radix_bits=-1, timing=False):
-    """projection: list of "B$i" / "P$j" strings, as in a plan file."""
+              partitioner="radix", radix_bits=-1, timing=False, flags=0):
+    """projection: list of "B$i" / "P$j" strings, as in a plan file; flags:
+    PJ_FLAG_* bits besides PJ_FLAG_TIMING."""
     d = JoinDesc()
     d.build = make_schema(build_cols)
     d.probe = make_schema(probe_cols)
@@ -241,7 +245,7 @@ def make_desc(build_cols, probe_cols, buildjattr, probejattr, projection,
         d.proj_attr[i] = int(attr)
     d.partitioner = PART[partitioner]
     d.radix_bits = radix_bits
-    d.flags = PJ_FLAG_TIMING if timing else 0
+    d.flags = (PJ_FLAG_TIMING if timing else 0) | flags
     return d
 
 

@@ -698,19 +698,39 @@ struct DenseTab {
 // each record at its bucket's cursor; afterwards dir entry b holds the end of
 // bucket b. Two 16-bit entries share a 32-bit word. bucket_of(r) returns the
 // bucket of record r, or ~0 to skip it; put(j, r, i) stores record r (index
-// i of R) at slot j.
+// i of R) at slot j. Each pass loads kBuildBatch records a thread before
+// using any (one memory latency a batch, not one a record), and pass 2
+// issues a batch's cursor atomics back to back before its stores.
+constexpr uint32_t kBuildBatch = 8;
+
 template <class BucketOf, class Put>
 __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint32_t r_off, uint32_t r_cnt,
                                               uint32_t* dirw, uint32_t* wsum, BucketOf bucket_of, Put put)
 {
     const uint32_t tid = threadIdx.x;
+    constexpr uint32_t BU = kBuildBatch;
+    // batch i0: record i0 + u * kJoinThreads + tid in r[u]; indexes past
+    // r_cnt (r_cnt > 0 inside the loops) load the last record again, so that
+    // every load is unconditional
+    auto load = [&](uint32_t i0, uint4(&r)[BU]) {
+#pragma unroll
+        for (uint32_t u = 0; u < BU; ++u) {
+            const uint32_t i = i0 + u * kJoinThreads + tid;
+            r[u] = R[r_off + (i < r_cnt ? i : r_cnt - 1)];
+        }
+    };
     for (uint32_t w = tid; w <= kTableBuckets / 2; w += kJoinThreads)
         dirw[w] = 0;
     __syncthreads();
-    for (uint32_t i = tid; i < r_cnt; i += kJoinThreads) {
-        const uint32_t b = bucket_of(R[r_off + i]);
-        if (b != ~0u)
-            atomicAdd(&dirw[b >> 1], 1u << ((b & 1u) * 16u));
+    for (uint32_t i0 = 0; i0 < r_cnt; i0 += BU * kJoinThreads) {
+        uint4 r[BU];
+        load(i0, r);
+#pragma unroll
+        for (uint32_t u = 0; u < BU; ++u) {
+            const uint32_t b = i0 + u * kJoinThreads + tid < r_cnt ? bucket_of(r[u]) : ~0u;
+            if (b != ~0u)
+                atomicAdd(&dirw[b >> 1], 1u << ((b & 1u) * 16u));
+        }
     }
     __syncthreads();
     {
@@ -723,14 +743,23 @@ __device__ __forceinline__ void build_buckets(const uint4* __restrict__ R, uint3
         dirw[2 * tid + 1] = (pre + c0 + c1) | ((pre + c0 + c1 + c2) << 16);
     }
     __syncthreads();
-    for (uint32_t i = tid; i < r_cnt; i += kJoinThreads) {
-        const uint4 r = R[r_off + i];
-        const uint32_t b = bucket_of(r);
-        if (b != ~0u) {
-            const uint32_t sh = (b & 1u) * 16u;
-            const uint32_t j = (atomicAdd(&dirw[b >> 1], 1u << sh) >> sh) & 0xFFFFu;
-            put(j, r, r_off + i);
-        }
+    for (uint32_t i0 = 0; i0 < r_cnt; i0 += BU * kJoinThreads) {
+        uint4 r[BU];
+        load(i0, r);
+        uint32_t b[BU], j[BU];
+#pragma unroll
+        for (uint32_t u = 0; u < BU; ++u)
+            b[u] = i0 + u * kJoinThreads + tid < r_cnt ? bucket_of(r[u]) : ~0u;
+#pragma unroll
+        for (uint32_t u = 0; u < BU; ++u)
+            if (b[u] != ~0u) {
+                const uint32_t sh = (b[u] & 1u) * 16u;
+                j[u] = (atomicAdd(&dirw[b[u] >> 1], 1u << sh) >> sh) & 0xFFFFu;
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < BU; ++u)
+            if (b[u] != ~0u)
+                put(j[u], r[u], r_off + i0 + u * kJoinThreads + tid);
     }
     __syncthreads();
 }
@@ -764,18 +793,10 @@ __device__ __forceinline__ uint32_t plan_count(const PlanView& v, uint32_t p, ui
     return r && sn ? *splits * probe_pieces(sn, v.chunk) : 0u;
 }
 
-// Work item `it` (< offs[FR]).
-__device__ __forceinline__ Item plan_item(const PlanView& v, uint32_t it)
+// Work item `it` (< offs[FR]) of build partition p (offs[p] <= it < offs[p + 1]).
+__device__ __forceinline__ Item plan_item_of(const PlanView& v, uint32_t it, uint32_t p)
 {
-    uint32_t lo = 0, hi = v.FR;   // the last p with offs[p] <= it
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (v.offs[mid] <= it)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const uint32_t p = lo, q = it - v.offs[p];
+    const uint32_t q = it - v.offs[p];
     const uint32_t ps = plan_probe_part(v, p);
     const uint32_t r0 = v.rst[p], r = v.rst[p + 1] - r0;
     const uint32_t s0 = v.sst[ps];
@@ -800,6 +821,34 @@ __device__ __forceinline__ Item plan_item(const PlanView& v, uint32_t it)
     I.hi = v.pp ? v.pp[p].hi : 0ull;
     I.base = v.pp ? v.pp[p].base : 0ull;
     return I;
+}
+
+// Work item `it` (< offs[FR]) by binary search over offs.
+__device__ __forceinline__ Item plan_item(const PlanView& v, uint32_t it)
+{
+    uint32_t lo = 0, hi = v.FR;   // the last p with offs[p] <= it
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (v.offs[mid] <= it)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return plan_item_of(v, it, lo);
+}
+
+// Work item `it` found by the whole block at once (it block-uniform): each
+// thread tests its share of the build partitions, the one whose item range
+// holds `it` leaves its number in *slot (LDS). One memory latency instead of
+// a binary search's chain of dependent loads. Ends with a barrier; *slot
+// must not be rewritten before the block's next barrier after this one.
+__device__ __forceinline__ Item plan_item_block(const PlanView& v, uint32_t it, uint32_t* slot)
+{
+    for (uint32_t p = threadIdx.x; p < v.FR; p += blockDim.x)
+        if (v.offs[p] <= it && it < v.offs[p + 1])
+            *slot = p;
+    __syncthreads();
+    return plan_item_of(v, it, *slot);
 }
 
 constexpr uint32_t kPlanThreads = 1024;
@@ -871,7 +920,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
 {
     extern __shared__ uint64_t lds64[];
     const uint32_t tid = threadIdx.x;
-    uint32_t* misc = reinterpret_cast<uint32_t*>(lds64);   // [0] item, [1] flag, [2..33] wave bases
+    uint32_t* misc = reinterpret_cast<uint32_t*>(lds64);   // [0] item, [1] flag, [2..33] wave bases, [34] partition
     uint32_t* wsum = misc + 40;                            // 17
     uint64_t* table = lds64 + kMiscWords / 2;
     const bool kbp = fmt.words == 6 && fmt.src[0] == SRC_BK0 && fmt.src[1] == SRC_BK1 && fmt.src[2] == SRC_BP0 &&
@@ -887,7 +936,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
         const uint32_t it = misc[0];
         if (it >= nitems)
             break;
-        const Item I = plan_item(plan, it);
+        const Item I = plan_item_block(plan, it, misc + 34);
         // CL: the probe records are listed by chunk from plan.cl (whole
         // chunks, so I.s_off is a multiple of the chunk's records)
         const uint4* Sp = CL ? S : S + I.s_off;
@@ -994,7 +1043,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
         const uint32_t it = misc[0];
         if (it >= nitems)
             break;
-        const Item I = plan_item(plan, it);
+        const Item I = plan_item_block(plan, it, misc + 34);
         const uint4* Sp = S + I.s_off;
         uint64_t* tkey = table;                                            // kTableCap
         uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);          // kTableCap
@@ -1017,11 +1066,12 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
         constexpr uint32_t kSlice = 64 * kGjUnroll, kRound = (kJoinThreads / 64) * kSlice;
         const uint32_t lane = tid & 63, wave = tid >> 6;
         uint4 nxt[kGjUnroll];
+        if (I.s_cnt) {   // unconditional first loads (past s_cnt: the last record again, never looked up)
 #pragma unroll
-        for (uint32_t u = 0; u < kGjUnroll; ++u) {
-            const uint32_t i = wave * kSlice + u * 64 + lane;
-            if (i < I.s_cnt)
-                nxt[u] = ld_stream(Sp + i);
+            for (uint32_t u = 0; u < kGjUnroll; ++u) {
+                const uint32_t i = wave * kSlice + u * 64 + lane;
+                nxt[u] = ld_stream(Sp + (i < I.s_cnt ? i : I.s_cnt - 1));
+            }
         }
         for (uint32_t r0 = 0; r0 < I.s_cnt; r0 += kRound) {
             const uint32_t t0 = r0 + wave * kSlice;

@@ -102,22 +102,20 @@ __device__ __forceinline__ void probe_rounds(const T& tab, const uint4* __restri
     };
     uint32_t ch_next = 0;   // CL: chunk numbers of the next round's slice
     uint4 nxt[PI];
+    // The first round's loads are unconditional, so that they leave back to
+    // back: a lane past s_cnt loads the last record again (the list's first
+    // chunk when CL), and its result is never looked up.
     if (CL) {
         const uint32_t ch0 = chunk_at(0);
         ch_next = chunk_at(kRound);
 #pragma unroll
-        for (int k = 0; k < PI; ++k) {
-            const uint32_t i = wave * kSlice + k * 64 + lane;
-            const uint4* a = rec_at(ch0, k);
-            if (i < s_cnt)
-                nxt[k] = ld_stream(a);
-        }
-    } else {
+        for (int k = 0; k < PI; ++k)
+            nxt[k] = ld_stream(rec_at(ch0, k));
+    } else if (s_cnt) {
 #pragma unroll
         for (int k = 0; k < PI; ++k) {
             const uint32_t i = wave * kSlice + k * 64 + lane;
-            if (i < s_cnt)
-                nxt[k] = ld_stream(Sp + i);
+            nxt[k] = ld_stream(Sp + (i < s_cnt ? i : s_cnt - 1));
         }
     }
     for (uint32_t r0 = 0; r0 < s_cnt; r0 += kRound) {

@@ -203,15 +203,13 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     uint4* buf = sm4;                                                          // FL * B
     uint32_t* cnt = reinterpret_cast<uint32_t*>(buf + FL * B);                 // FL: buffered records
     uint32_t* goff = cnt + FL;   // FL: next chunk's record index (APP: the round's first chunk slot)
-    uint32_t* nch = goff + FL;   // APP, FL: chunks written a partition
-    uint32_t* ndir = nch + FL;   // APP, FL: the round's overflow records that leave in whole chunks
+    uint32_t* ndir = goff + FL;  // APP, FL: the round's overflow records that leave in whole chunks
     uint32_t* ccur = ndir + FL;  // APP: chunk slots the block has used
+    uint32_t nch = 0;            // APP: chunks written of the thread's own partition
     const uint64_t rbase = APP ? (uint64_t)blockIdx.x * app.region : 0;   // APP: the block's first tag
     auto slot_rec = [&](uint64_t slot) -> uint64_t { return (rbase + slot) * B; };
     for (uint32_t p = threadIdx.x; p < pn; p += kWcThreads) {
-        if (APP)
-            nch[p] = 0;
-        else
+        if (!APP)
             goff[p] = offs[(uint64_t)global_part(p) * nblk + part];
         cnt[p] = 0;
     }
@@ -222,10 +220,12 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
     const uint32_t gl = threadIdx.x % B, grp = threadIdx.x / B;
     constexpr uint32_t ngrp = kWcThreads / B;
-    // APP: own_slots(n), called by every lane of a wave with its own
-    // partition's chunk count, claims the wave's slots with one LDS atomic
-    // and returns the lane's first slot; group g of the wave then writes the
-    // wave's partitions g, g + 64 / B, ... with the owners' slots and counts.
+    // APP: thread t owns partition t of the block (t < pn). own_slots(n),
+    // called by every lane of a wave with its own partition's chunk count,
+    // claims the wave's slots with one LDS atomic and returns the lane's
+    // first slot (the wave's lanes get consecutive runs, so the owners' tag
+    // stores coalesce); group g of the wave then writes the wave's partitions
+    // g, g + 64 / B, ... with the owners' slots and counts.
     const uint32_t wl = threadIdx.x & 63u, wq0 = threadIdx.x & ~63u, gi = wl / B;
     // The wave's exclusive scan of the counts runs bit by bit over ballots
     // (a few bits: counts are small), without LDS round trips.
@@ -299,17 +299,20 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         // full buffers leave; whole chunks of overflow go straight out
         if (APP) {
             // the owner of each partition claims its chunks' slots (the
-            // buffer's chunk, then the overflow's) and keeps them in LDS
+            // buffer's chunk, then the overflow's), keeps them in LDS and
+            // tags them
             {
-                const uint32_t qo = wq0 + wl;
+                const uint32_t qo = threadIdx.x;
                 const uint32_t co = qo < pn ? cnt[qo] : 0u;
                 const uint32_t no = co >= (uint32_t)B ? 1 + (co - B) / B : 0u;
                 const uint32_t so = own_slots(no);
                 if (no) {
                     goff[qo] = so;
                     ndir[qo] = (no - 1) * B;
-                    nch[qo] += no;
+                    nch += no;
                 }
+                for (uint32_t t = 0; t < no; ++t)
+                    app.cmap[rbase + so + t] = (uint16_t)global_part(qo);
             }
             // the groups read the slots their wave's owners wrote to LDS
             // (one wave's LDS accesses complete in order)
@@ -317,12 +320,8 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             for (uint32_t j = gi; j < 64; j += 64 / B) {
                 const uint32_t q = wq0 + j;
                 const uint32_t c = q < pn ? cnt[q] : 0u;
-                if (c >= (uint32_t)B) {
-                    const uint32_t slot = goff[q], nq = 1 + (c - B) / B;
-                    out[slot_rec(slot) + gl] = buf[q * B + gl];
-                    for (uint32_t t = gl; t < nq; t += B)
-                        app.cmap[rbase + slot + t] = (uint16_t)global_part(q);
-                }
+                if (c >= (uint32_t)B)
+                    out[slot_rec(goff[q]) + gl] = buf[q * B + gl];
             }
             __syncthreads();
             // overflow records: whole chunks leave at their slots, the rest
@@ -427,11 +426,11 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
         out[at + gl] = make_uint4(keep ? r.x : pad.x, keep ? r.y : pad.y, keep ? r.z : pad.z, keep ? r.w : pad.w);
     };
     if (APP) {
-        const uint32_t qo = wq0 + wl;
+        const uint32_t qo = threadIdx.x;
         const uint32_t no = qo < pn && cnt[qo] > 0 ? 1u : 0u;
         const uint32_t so = own_slots(no);
         if (no) {
-            nch[qo] += 1;
+            nch += 1;
             app.cmap[rbase + so] = (uint16_t)global_part(qo);
         }
         for (uint32_t j = gi; j < 64; j += 64 / B) {
@@ -440,14 +439,14 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                 tail(wq0 + j, slot_rec(slot));
         }
         __syncthreads();
-        // chunk counts partition-major; the partner block's partitions read 0
-        for (uint32_t gp = threadIdx.x; gp < F; gp += kWcThreads) {
-            uint32_t c = 0;
-            if (SPLIT == 1)
-                c = nch[gp];
-            else if ((gp >= FL) == (half != 0))
-                c = nch[gp - p0];
-            app.bhist[(uint64_t)gp * gridDim.x + blockIdx.x] = c;
+        // chunk counts partition-major: the owners' partitions, then 0 for
+        // the partner block's half
+        if (qo < pn)
+            app.bhist[(uint64_t)(p0 + qo) * gridDim.x + blockIdx.x] = nch;
+        if (SPLIT == 2) {
+            const uint32_t gp = (half ? 0u : FL) + qo;
+            if (gp < (half ? FL : F))
+                app.bhist[(uint64_t)gp * gridDim.x + blockIdx.x] = 0;
         }
         if (threadIdx.x == 0)
             app.nchunks[blockIdx.x] = *ccur;
@@ -1925,7 +1924,7 @@ int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, con
     const uint32_t FL = SPLIT == 1 ? pf.F : (pf.F + 1) / 2;
     if (SPLIT > 1 && nblk % 8)
         return fail(PJ_ERR_INVALID, "split scatter needs a multiple of 8 input chunks");
-    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4 + (APP ? 2 * (size_t)FL * 4 + 16 : 0);
+    const size_t lds = (size_t)FL * B * 16 + 2 * FL * 4 + (APP ? (size_t)FL * 4 + 16 : 0);
     uint4 pad_other, pad_zero;
     if (RANGE) {
         // ordered key 0 lies in partition 0, ordered key ~0 in the last one

@@ -487,15 +487,18 @@ __global__ __launch_bounds__(kPlaceThreads) void k_chunk_place(const uint16_t* _
         for (uint32_t p = tid; p <= F; p += kPlaceThreads)
             cnt[p] = 0;
         __syncthreads();
+        // all tag loads first (past tn: the tile's last tag again, unused),
+        // then the counting atomics back to back
         uint32_t part[PER], rank[PER];
 #pragma unroll
         for (uint32_t j = 0; j < PER; ++j) {
             const uint32_t i = j * kPlaceThreads + tid;
-            if (i < tn) {
-                part[j] = tags[t0 + i];
-                rank[j] = atomicAdd(&cnt[part[j]], 1u);
-            }
+            part[j] = tags[t0 + (i < tn ? i : tn - 1)];
         }
+#pragma unroll
+        for (uint32_t j = 0; j < PER; ++j)
+            if (j * kPlaceThreads + tid < tn)
+                rank[j] = atomicAdd(&cnt[part[j]], 1u);
         __syncthreads();
         block_scan_array<kPlaceThreads>(cnt, F, wsum);   // cnt[p]: the tile's first place of p
 #pragma unroll

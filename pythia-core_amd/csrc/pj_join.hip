@@ -2157,14 +2157,16 @@ static_assert(kAppendMaxF <= kPlaceMaxF, "the chunk list's tile sort holds every
 // partition once: listed chunks cost the join a record in flight a thread,
 // paid on every read), at least two partitions (one has no key outside it to
 // pad chunks with) and at most kAppendMaxF, 16-byte records or the row-id
-// form (the word-by-word variant spills registers), and not in a fixed
-// scratch (it needs about twice the room).
+// form (the word-by-word variant spills registers), fewer than 2^30 tuples
+// (slot numbers in 32 bits), and not in a fixed scratch (it needs about
+// twice the room: each block of a split pair reserves slots for its whole
+// input chunk).
 bool want_append(const pj_join_s* h, const void* d_probe, uint64_t np)
 {
     const SideLayout& L = h->lay.probe;
     const bool words = !L.hashed && !(L.ident && ((uintptr_t)d_probe % 16 == 0));
-    if (np == 0 || h->range || h->bS == 0 || (1u << h->bS) > kAppendMaxF || h->bR != h->bS || words ||
-        (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) || h->spart.arena)
+    if (np == 0 || np >= (1ull << 30) || h->range || h->bS == 0 || (1u << h->bS) > kAppendMaxF ||
+        h->bR != h->bS || words || (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) || h->spart.arena)
         return false;
     return (h->desc.flags & PJ_FLAG_LAYOUT_APPEND) || np >= kAppendMin;
 }

@@ -278,15 +278,27 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             }
         }
         // claim: the round's LDS atomics leave back to back, the buffer
-        // writes wait for their results after
+        // writes wait for their results after. (Range partitions, whose
+        // loop also carries the partition ids, claim and write record by
+        // record: the batched form spills registers there.)
+        if (RANGE) {
 #pragma unroll
-        for (int k = 0; k < IT; ++k)
-            if (v[k])
-                s[k] = atomicAdd(&cnt[p[k]], 1u);
+            for (int k = 0; k < IT; ++k)
+                if (v[k]) {
+                    s[k] = atomicAdd(&cnt[p[k]], 1u);
+                    if (s[k] < (uint32_t)B)
+                        buf[p[k] * B + s[k]] = cur[k];
+                }
+        } else {
 #pragma unroll
-        for (int k = 0; k < IT; ++k)
-            if (v[k] && s[k] < (uint32_t)B)
-                buf[p[k] * B + s[k]] = cur[k];
+            for (int k = 0; k < IT; ++k)
+                if (v[k])
+                    s[k] = atomicAdd(&cnt[p[k]], 1u);
+#pragma unroll
+            for (int k = 0; k < IT; ++k)
+                if (v[k] && s[k] < (uint32_t)B)
+                    buf[p[k] * B + s[k]] = cur[k];
+        }
         // APP: record k's partition and claim in one word (~0: none), which
         // frees the registers of both for the slot claims below
         uint32_t ps[IT];

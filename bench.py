@@ -440,7 +440,8 @@ def load_from_files(args, P, torch, db, nb, dp, npr, stream):
 
 def layout_flags(args, P):
     """PJ_FLAG_LAYOUT_* bits of --layout (auto: the library chooses)."""
-    return {"auto": 0, "exact": P.PJ_FLAG_LAYOUT_EXACT, "append": P.PJ_FLAG_LAYOUT_APPEND}[args.layout]
+    return {"auto": 0, "exact": P.PJ_FLAG_LAYOUT_EXACT, "append": P.PJ_FLAG_LAYOUT_APPEND,
+            "two_pass": P.PJ_FLAG_LAYOUT_TWO_PASS}[args.layout]
 
 
 def lib_sha():
@@ -784,6 +785,7 @@ def compare_learned(args, P, torch, dev, stream, with_cpu):
                      "used_learned": int(t.used_learned), "partitions": int(t.partitions),
                      "work_items": int(t.work_items),
                      "probe_layout": "appended" if t.probe_appended else "contiguous",
+                     "probe_passes": int(t.probe_passes),
                      "parity": {"count": n, "checksum": s, "matches_closed_form": (n, s) == (ec, es)}}
         _say("learned_vs_radix: %s %.2f ms/step" % (part, ms))
         j.close()
@@ -808,9 +810,9 @@ def main():
                     sorted(BATCH_WORKLOADS))
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the composite-key workload")
     ap.add_argument("--partitioner", default="radix", choices=["radix", "learned", "auto"])
-    ap.add_argument("--layout", default="auto", choices=["auto", "exact", "append"],
-                    help="probe-side partition layout: the library's choice, contiguous after a histogram pass, or "
-                         "appended chunks listed by partition")
+    ap.add_argument("--layout", default="auto", choices=["auto", "exact", "append", "two_pass"],
+                    help="probe-side partition layout: the library's choice, contiguous after a histogram pass, "
+                         "appended chunks listed by partition, or two radix passes")
     ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key", "count_by_key_fused"],
                     help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
                          "per build key (int64-key workloads); count_by_key_fused: the count per build key "
@@ -1050,7 +1052,8 @@ def main():
                    "partitioner": args.partitioner, "partitions": t.partitions,
                    "subpartitions": t.subpartitions, "work_items": t.work_items,
                    "used_learned": t.used_learned, "spilled_items": t.spilled_items,
-                   "probe_layout": "appended" if t.probe_appended else "contiguous"},
+                   "probe_layout": "appended" if t.probe_appended else "contiguous",
+                   "probe_passes": t.probe_passes},
         "hbm_gbs_algorithmic": job_gbs,
         "hbm_frac_of_peak": job_gbs / HBM_PEAK_GBS,
         "hbm_gbs_inputs_only": (nb + npr) * w_in / (ms * 1e-3) / 1e9,

@@ -83,6 +83,13 @@ typedef struct {
                                            probe tuples); by default probe sides of 2^22 tuples or more
                                            take it where allowed. It needs about twice the probe side's
                                            bytes of device memory for the partitioned copy */
+#define PJ_FLAG_LAYOUT_TWO_PASS 16u     /* where the build side has more partitions than one write-combining
+                                           pass gives the probe side (2 to 8 build partitions a probe
+                                           partition), partition the probe side in two radix passes, high bits
+                                           first, so that the join reads each probe record once instead of
+                                           once a build partition, whatever its size; by default probe sides
+                                           of 2^22 tuples or more take it. It needs another copy of the probe
+                                           side in device memory */
 
 typedef struct pj_join_s* pj_join_t;
 
@@ -101,6 +108,7 @@ typedef struct {
     float build_hist_ms;       /* of build_partition_ms: the histogram and its scan */
     float probe_hist_ms;       /* of probe_partition_ms: the histogram and its scan */
     uint32_t probe_appended;   /* 1 if the probe side took the appended layout (no histogram pass) */
+    uint32_t probe_passes;     /* radix passes over the probe side (2: PJ_FLAG_LAYOUT_TWO_PASS's split) */
 } pj_timings;
 
 /* Resumable probe cursor: the state HashJoinOp keeps between getNext calls

@@ -545,6 +545,84 @@ __global__ void k_chunk_bounds(const uint32_t* __restrict__ offs, uint32_t F, ui
         part_start[F] = *total * kChunkRecs;
 }
 
+// ---------------------------------------------------------------- two passes
+// A probe side whose build partitions are finer (2^bR) than the 2^bS
+// partitions one write-combining pass produces is partitioned in two passes,
+// high bits first: the write-combining scatter on the top bS bits of the key
+// mix, then k_subsplit on the next bR - bS bits within each of those
+// partitions. Both passes' offsets come from one histogram over the 2^bR
+// fine partitions.
+
+// coarse[p * nblk + b] = the fine counts of partition p's D = 2^dbits fine
+// partitions in input chunk b, summed and rounded up to whole chunks of pad.
+__global__ void k_coarsen(const uint32_t* __restrict__ fine, uint32_t F, uint32_t nblk, uint32_t dbits, uint32_t pad,
+                          uint32_t* __restrict__ coarse)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)F * nblk)
+        return;
+    const uint32_t p = (uint32_t)(i / nblk), b = (uint32_t)(i % nblk), D = 1u << dbits;
+    uint32_t c = 0;
+    for (uint32_t q = 0; q < D; ++q)
+        c += fine[((uint64_t)p * D + q) * nblk + b];
+    coarse[i] = (c + pad - 1) / pad * pad;
+}
+
+constexpr uint32_t kSubThreads = 1024, kSubItems = 8;
+
+// The second pass. Block b re-reads the segments that input chunk b's
+// scatter wrote: segment p at coff[p * nblk + b] (the scanned coarse counts;
+// its end is the next segment's start, or *ctotal), padding included. Each
+// record of partition p moves to its fine partition p * D + q (q: the next
+// dbits bits of the key mix) at foff[(p * D + q) * nblk + b] (the scanned
+// fine counts) plus its rank among the segment's records of q; padding
+// records (keys of other partitions) are dropped. Waves take segments
+// p = wave, wave + 16, ...; lane q < D keeps fine partition q's next place.
+__global__ __launch_bounds__(kSubThreads) void k_subsplit(const uint4* __restrict__ src,
+                                                           const uint32_t* __restrict__ coff,
+                                                           const uint32_t* __restrict__ ctotal, uint32_t F,
+                                                           uint32_t nblk, uint32_t bS, uint32_t dbits,
+                                                           const uint32_t* __restrict__ foff, uint4* __restrict__ dst)
+{
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, b = blockIdx.x;
+    const uint64_t below = lane ? ~0ull >> (64 - lane) : 0ull;
+    const uint32_t D = 1u << dbits, bR = bS + dbits;
+    const uint64_t m = (uint64_t)F * nblk;
+    for (uint32_t p = wave; p < F; p += kSubThreads / 64) {
+        const uint64_t idx = (uint64_t)p * nblk + b;
+        const uint32_t s0 = coff[idx];
+        const uint32_t len = (idx + 1 < m ? coff[idx + 1] : *ctotal) - s0;
+        uint32_t next = lane < D ? foff[((uint64_t)p * D + lane) * nblk + b] : 0u;
+        for (uint32_t i0 = 0; i0 < len; i0 += 64 * kSubItems) {
+            // unconditional loads (past the segment: its last record again)
+            uint4 r[kSubItems];
+#pragma unroll
+            for (uint32_t k = 0; k < kSubItems; ++k) {
+                const uint32_t i = i0 + k * 64 + lane;
+                r[k] = src[s0 + (i < len ? i : len - 1)];
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kSubItems; ++k) {
+                const uint32_t f = radix_part(key_mix(rec_key(r[k])), bR);
+                const bool v = i0 + k * 64 + lane < len && (f >> dbits) == p;
+                const uint32_t q = f & (D - 1);
+                uint32_t dest = 0;
+                for (uint32_t qq = 0; qq < D; ++qq) {
+                    const bool mine = v && q == qq;
+                    const uint64_t mq = __ballot(mine);
+                    const uint32_t base = __builtin_amdgcn_readlane(next, qq);
+                    if (mine)
+                        dest = base + (uint32_t)__popcll(mq & below);
+                    if (lane == qq)
+                        next += (uint32_t)__popcll(mq);
+                }
+                if (v)
+                    dst[dest] = r[k];
+            }
+        }
+    }
+}
+
 constexpr int kScatterItems = 16;
 
 // Scatter with an LDS-sorted tile, for fan-outs too wide for one LDS buffer
@@ -1824,6 +1902,10 @@ struct pj_join_s {
     // the probe side's appended layout: chunk tags, counts, chunk list
     DevBuf a_cmap, a_bhist, a_bsum, a_nchunks, a_clist;
     bool s_app = false;             // the last probe side took the appended layout
+    // the probe side's second radix pass: fine counts, their scan, the
+    // records in fine partitions and their starts
+    DevBuf t_fine, t_fsum, t_recs, t_starts;
+    bool s_two = false;             // the last probe side took two passes
     uint64_t pair_cap = 0;          // pairs the last probe phase could hold
     uint64_t pair_want = 0;         // pairs a previous run needed
 };
@@ -2250,6 +2332,83 @@ int partition_append(pj_join_s* h, const void* d_in, uint64_t n, const SideLayou
     return PJ_OK;
 }
 
+// Probe sides of at least this many tuples whose build partitions are finer
+// than their probe partitions take two passes by default
+// (PJ_FLAG_LAYOUT_EXACT / _TWO_PASS force one or two).
+constexpr uint64_t kTwoPassMin = 1ull << 22;
+
+// Whether the probe side takes two radix passes: radix partitions, 2 to 8
+// fine partitions a probe partition (bR - bS of 1 to 3 bits; the planner
+// takes at most 2^14 build partitions), not in a fixed scratch (the second
+// pass needs room for another copy).
+bool want_two_pass(const pj_join_s* h, uint64_t np)
+{
+    if (np == 0 || h->range || h->bR <= h->bS || h->bR - h->bS > 3 || (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) ||
+        h->spart.arena)
+        return false;
+    return (h->desc.flags & PJ_FLAG_LAYOUT_TWO_PASS) || np >= kTwoPassMin;
+}
+
+// Partitions the probe side in two radix passes (k_subsplit): one histogram
+// over the 2^bR fine partitions gives the first pass's per-chunk counts
+// (k_coarsen) and the second pass's places; h->t_recs then holds fine
+// partition f's records at [t_starts[f], t_starts[f + 1]), no padding.
+// Marks event 9 after the histogram and its scans.
+int partition_two_pass(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, hipStream_t st)
+{
+    int rc;
+    const uint32_t bS = h->bS, dbits = h->bR - h->bS;
+    const PartFn pf = radix_fn(bS), pff = radix_fn(h->bR);
+    const uint32_t F = pf.F, FF = pff.F;
+    Partitioned& P = h->S;
+    P.F = FF;
+    P.n = n;
+    P.start.clear();
+    if (n >= (1ull << 31))
+        return fail(PJ_ERR_UNSUPPORTED, "relations of 2^31 tuples or more");
+    const PartGeom g = part_geom(n, pf, false, true);
+    if (g.B != 8)
+        return fail(PJ_ERR_UNSUPPORTED, "two passes take a first pass of 8-record chunks (%u partitions)", F);
+    const uint64_t mf = (uint64_t)FF * g.nblk;
+    const uint32_t nsf = (uint32_t)((mf + kScanSeg - 1) / kScanSeg);
+    if (g.cap >= (1ull << 32) || mf >= (1ull << 32))
+        return fail(PJ_ERR_UNSUPPORTED, "partitioned relation exceeds 2^32 records");
+    if ((rc = h->spart.ensure(g.cap * 16)) || (rc = h->hist.ensure(g.m * 4)) ||
+        (rc = h->bsum.ensure(((size_t)g.nsb + 1) * 4)) || (rc = h->t_fine.ensure(mf * 4)) ||
+        (rc = h->t_fsum.ensure(((size_t)nsf + 1) * 4)) || (rc = h->t_recs.ensure(n * 16)) ||
+        (rc = h->t_starts.ensure(((size_t)FF + 1) * 4)))
+        return rc;
+    const uint32_t* in = (const uint32_t*)d_in;
+    const int rm = L.hashed ? REC_HASH : (L.ident && ((uintptr_t)d_in % 16 == 0)) ? REC_IDENT : REC_WORDS;
+    uint32_t* fine = (uint32_t*)h->t_fine.p;
+    uint32_t* fsum = (uint32_t*)h->t_fsum.p;
+    uint32_t* coarse = (uint32_t*)h->hist.p;
+    uint32_t* csum = (uint32_t*)h->bsum.p;
+    if ((rc = launch_hist<false>(rm, in, n, L, pff, g.chunk, g.nblk, 1, fine, nullptr, nullptr, nullptr, st)))
+        return rc;
+    hipLaunchKernelGGL(k_coarsen, dim3((uint32_t)((g.m + 255) / 256)), dim3(256), 0, st, (const uint32_t*)fine, F,
+                       g.nblk, dbits, g.B, coarse);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(g.nsb), dim3(kScanThreads), 0, st, coarse, g.m, csum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, csum, g.nsb);
+    hipLaunchKernelGGL(k_scan_down, dim3(g.nsb), dim3(kScanThreads), 0, st, coarse, g.m, (const uint32_t*)csum);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nsf), dim3(kScanThreads), 0, st, fine, mf, fsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, fsum, nsf);
+    hipLaunchKernelGGL(k_scan_down, dim3(nsf), dim3(kScanThreads), 0, st, fine, mf, (const uint32_t*)fsum);
+    hipLaunchKernelGGL(k_part_bounds, dim3((FF + 1 + 255) / 256), dim3(256), 0, st, (const uint32_t*)fine, FF, g.nblk,
+                       (const uint32_t*)(fsum + nsf), (uint32_t*)h->t_starts.p);
+    PJ_HIP(hipGetLastError());
+    if (h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING))
+        PJ_HIP(hipEventRecord(h->ev[9], st));
+    if ((rc = launch_wc<8>(rm, false, g.split, in, n, L, pf, g.chunk, g.nblk, coarse, (uint4*)h->spart.p, nullptr,
+                           st)))
+        return rc;
+    hipLaunchKernelGGL(k_subsplit, dim3(g.nblk), dim3(kSubThreads), 0, st, (const uint4*)h->spart.p,
+                       (const uint32_t*)coarse, (const uint32_t*)(csum + g.nsb), F, g.nblk, bS, dbits,
+                       (const uint32_t*)fine, (uint4*)h->t_recs.p);
+    PJ_HIP(hipGetLastError());
+    return PJ_OK;
+}
+
 // The build side (&h->R) marks event 8 after its histogram, the probe side 9.
 int partition_side(pj_join_s* h, const void* d_in, uint64_t n, const SideLayout& L, const PartFn& pf, bool range,
                    bool write_combine, DevBuf& dst, Partitioned& P, DevBuf& starts, bool host_copy,
@@ -2435,10 +2594,13 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     *empty = false;
     record(h, 2, st);
     h->s_app = want_append(h, d_probe, np);
+    h->s_two = !h->s_app && want_two_pass(h, np);
     const PartFn spf =
         h->range ? map_fn(h->plan.ncoarse, (const uint8_t*)h->map_dev.p, h->plan.div) : radix_fn(h->bS);
     if (h->s_app)
         rc = partition_append(h, d_probe, np, h->lay.probe, spf, st);
+    else if (h->s_two)
+        rc = partition_two_pass(h, d_probe, np, h->lay.probe, st);
     else
         rc = partition_side(h, d_probe, np, h->lay.probe, spf, h->range, true, h->spart, h->S, h->sstarts, false,
                             nullptr, st);
@@ -2454,9 +2616,9 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     // the work items, counted on the device from the partition starts
     PlanView v = {};
     v.FR = h->range ? h->plan.nfine : 1u << h->bR;
-    v.shift = h->range ? 0u : h->bR - h->bS;
+    v.shift = h->range || h->s_two ? 0u : h->bR - h->bS;   // two passes: probe partitions = build partitions
     v.rst = (const uint32_t*)h->rstarts.p;
-    v.sst = (const uint32_t*)h->sstarts.p;
+    v.sst = (const uint32_t*)(h->s_two ? h->t_starts.p : h->sstarts.p);
     v.coarse_of = h->range ? (const uint32_t*)h->coarse_of.p : nullptr;
     v.pp = h->range ? (const PlanPart*)h->plan_parts.p : nullptr;
     v.cl = h->s_app ? (const uint32_t*)h->a_clist.p : nullptr;
@@ -2495,8 +2657,8 @@ int probe_enqueue(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, u
     const uint32_t jbits = h->range ? 0u : h->bR, jshift = h->range ? 0u : bkt_shift;
     auto launch = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kJoinThreads), lds, st, (const uint4*)h->rpart.p,
-                           (const uint4*)h->spart.p, v, item_ctr, (uint32_t*)jout, jcap, cursor, h->lay.out, jbits,
-                           jshift, fail_ctr);
+                           (const uint4*)(h->s_two ? h->t_recs.p : h->spart.p), v, item_ctr, (uint32_t*)jout, jcap,
+                           cursor, h->lay.out, jbits, jshift, fail_ctr);
     };
     if (h->range)   // range plans keep the contiguous layout
         launch(k_join<true, false>);
@@ -2564,6 +2726,7 @@ int probe_once(pj_join_s* h, const void* d_probe, uint64_t np, void* d_out, uint
     t.used_learned = h->range ? 1u : 0u;
     t.spilled_items = produced[7];
     t.probe_appended = h->s_app ? 1u : 0u;
+    t.probe_passes = h->s_two ? 2u : 1u;
     return PJ_OK;
 }
 

@@ -27,6 +27,7 @@ PJ_FLAG_TIMING = 1
 PJ_FLAG_NARROW_ROW_HASH = 2
 PJ_FLAG_LAYOUT_EXACT = 4
 PJ_FLAG_LAYOUT_APPEND = 8
+PJ_FLAG_LAYOUT_TWO_PASS = 16
 
 
 class Schema(C.Structure):
@@ -48,7 +49,7 @@ class Timings(C.Structure):
                 ("partitions", C.c_uint32), ("subpartitions", C.c_uint32),
                 ("work_items", C.c_uint32), ("used_learned", C.c_uint32),
                 ("spilled_items", C.c_uint64), ("build_hist_ms", C.c_float), ("probe_hist_ms", C.c_float),
-                ("probe_appended", C.c_uint32)]
+                ("probe_appended", C.c_uint32), ("probe_passes", C.c_uint32)]
 
 
 PJ_MODEL_LEAVES = 4096

@@ -813,6 +813,8 @@ def main():
     ap.add_argument("--layout", default="auto", choices=["auto", "exact", "append", "two_pass"],
                     help="probe-side partition layout: the library's choice, contiguous after a histogram pass, "
                          "appended chunks listed by partition, or two radix passes")
+    ap.add_argument("--radix-bits", type=int, default=-1,
+                    help="radix bits of the build side (-1: the library's choice)")
     ap.add_argument("--aggregate", default="none", choices=["none", "sum", "count_by_key", "count_by_key_fused"],
                     help="after each join, aggregate its output in HBM: sum of the probe payload, or a count "
                          "per build key (int64-key workloads); count_by_key_fused: the count per build key "
@@ -869,7 +871,8 @@ def main():
         P.check(P.lib().pj_gen_clustered(table.ctypes.data_as(C.c_void_p), npr, cdf.ctypes.data_as(C.c_void_p),
                                          C.c_void_p(db.data_ptr()), C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P))
+                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P),
+                           radix_bits=args.radix_bits)
     elif zipf:
         seed, nb, npr = ZIPF_WORKLOADS[args.workload]
         nmiss, dom = 0, None
@@ -880,7 +883,7 @@ def main():
                                           C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["int", "int", "long"], ["int", "int", "long"], [0, 1], [0, 1],
                            ["B$0", "B$1", "B$2", "P$2"], partitioner=args.partitioner, timing=True,
-                           flags=layout_flags(args, P))
+                           flags=layout_flags(args, P), radix_bits=args.radix_bits)
     else:
         seed, dom, nb, npr, nmiss = WORKLOADS[args.workload]
         db = torch.empty(nb * 16, dtype=torch.uint8, device=dev)
@@ -888,7 +891,8 @@ def main():
         P.check(P.lib().pj_gen_pkfk(seed, dom, nb, npr, nmiss, C.c_void_p(db.data_ptr()),
                                     C.c_void_p(dp.data_ptr()), 1, C.c_void_p(stream)))
         desc = P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"],
-                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P))
+                           partitioner=args.partitioner, timing=True, flags=layout_flags(args, P),
+                           radix_bits=args.radix_bits)
     load_res = None
     if args.from_files:
         load_res = load_from_files(args, P, torch, db, nb, dp, npr, stream)

@@ -88,8 +88,9 @@ typedef struct {
                                            partition), partition the probe side in two radix passes, high bits
                                            first, so that the join reads each probe record once instead of
                                            once a build partition, whatever its size; by default probe sides
-                                           of 2^22 tuples or more take it. It needs another copy of the probe
-                                           side in device memory */
+                                           of 2^22 tuples or more with 4 to 8 build partitions a probe
+                                           partition take it. It needs another copy of the probe side in
+                                           device memory */
 
 typedef struct pj_join_s* pj_join_t;
 

@@ -2332,10 +2332,15 @@ int partition_append(pj_join_s* h, const void* d_in, uint64_t n, const SideLayou
     return PJ_OK;
 }
 
-// Probe sides of at least this many tuples whose build partitions are finer
-// than their probe partitions take two passes by default
-// (PJ_FLAG_LAYOUT_EXACT / _TWO_PASS force one or two).
+// Probe sides of at least this many tuples whose build partitions are at
+// least 4 times finer than their probe partitions take two passes by
+// default (PJ_FLAG_LAYOUT_EXACT / _TWO_PASS force one or two). The second
+// pass reads and writes the probe side once; it saves the join D - 1 reads
+// of it. Measured on 16M x 256M uniform: 2 build partitions a probe
+// partition 5.65 ms a step in one pass, 7.00 in two; 4 of them 7.68 in one,
+// 7.34 in two.
 constexpr uint64_t kTwoPassMin = 1ull << 22;
+constexpr uint32_t kTwoPassMinBits = 2;
 
 // Whether the probe side takes two radix passes: radix partitions, 2 to 8
 // fine partitions a probe partition (bR - bS of 1 to 3 bits; the planner
@@ -2346,7 +2351,8 @@ bool want_two_pass(const pj_join_s* h, uint64_t np)
     if (np == 0 || h->range || h->bR <= h->bS || h->bR - h->bS > 3 || (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) ||
         h->spart.arena)
         return false;
-    return (h->desc.flags & PJ_FLAG_LAYOUT_TWO_PASS) || np >= kTwoPassMin;
+    return (h->desc.flags & PJ_FLAG_LAYOUT_TWO_PASS) ||
+           (np >= kTwoPassMin && h->bR - h->bS >= kTwoPassMinBits);
 }
 
 // Partitions the probe side in two radix passes (k_subsplit): one histogram

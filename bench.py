@@ -498,17 +498,31 @@ def read_traffic(workload):
     return d["kernels"]
 
 
+PROBE_SCATTER = "k_scatter_wc probe"
+PROBE_SCATTER_APP = "k_scatter_wc probe (appended) + k_chunk_place"
+PROBE_SCATTER_TWO = "k_scatter_wc + k_subsplit probe (two passes)"
+
+
+def probe_scatter_label(appended, passes):
+    return PROBE_SCATTER_APP if appended else PROBE_SCATTER_TWO if passes == 2 else PROBE_SCATTER
+
+
 def _label_step_kernels(rows):
     """Dispatches of one join step in launch order -> {label: [counter values]}.
-    The step launches k_hist twice (build, then probe), the build side's
-    scatter, the probe side's write-combining scatter, k_plan, k_join."""
+    The step launches k_hist for the build side, then (unless the probe side
+    is appended) for the probe side, the build side's scatter, the probe
+    side's write-combining scatter (with k_chunk_place, or k_coarsen and
+    k_subsplit, by its layout), k_plan, k_join."""
+    names = [n for n, _ in rows]
+    scat = probe_scatter_label(any("k_chunk_place" in n for n in names), 2 if any("k_subsplit" in n for n in names)
+                               else 1)
     out, hist_seen = {}, 0
     for name, val in rows:
         if "k_hist" in name:
             label = "k_hist build" if hist_seen == 0 else "k_hist probe"
             hist_seen += 1
-        elif "k_scatter_wc" in name:
-            label = "k_scatter_wc probe"
+        elif any(k in name for k in ("k_scatter_wc", "k_chunk_place", "k_chunk_bounds", "k_subsplit", "k_coarsen")):
+            label = scat
         elif "k_scatter" in name:
             label = "k_scatter build"
         elif "k_join" in name:
@@ -1016,11 +1030,17 @@ def main():
     # times include their scans), algorithmic bytes, and the HBM bytes of
     # the current build's counter summary when there is one
     traffic = read_traffic(args.workload)
+    # the probe side by its layout: a histogram and one scatter; no
+    # histogram (appended chunks, then their list); or a histogram and two
+    # passes, each reading and writing every record
+    scat = probe_scatter_label(t.probe_appended, t.probe_passes)
     kern = [("k_hist build", phases["build_hist_ms"], nb * w_in),
-            ("k_scatter build", phases["build_partition_ms"] - phases["build_hist_ms"], 2 * nb * w_in),
-            ("k_hist probe", phases["probe_hist_ms"], npr * w_in),
-            ("k_scatter_wc probe", phases["probe_partition_ms"] - phases["probe_hist_ms"], 2 * npr * w_in),
-            ("k_join", phases["join_ms"], b_alg)]
+            ("k_scatter build", phases["build_partition_ms"] - phases["build_hist_ms"], 2 * nb * w_in)]
+    if not t.probe_appended:
+        kern.append(("k_hist probe", phases["probe_hist_ms"], npr * w_in))
+    kern += [(scat, phases["probe_partition_ms"] - (0.0 if t.probe_appended else phases["probe_hist_ms"]),
+              (4 if t.probe_passes == 2 else 2) * npr * w_in),
+             ("k_join", phases["join_ms"], b_alg)]
     kernels = []
     for name, kms, kbytes in kern:
         ach = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None

@@ -87,6 +87,24 @@ def test_step_kernels_are_labelled_in_launch_order():
                    "k_join": [50.0]}
 
 
+def test_step_kernels_of_the_appended_and_two_pass_layouts():
+    app = [("void pj::k_hist<1, false>(...)", 10.0), ("void pj::k_scatter<1, false>(...)", 20.0),
+           ("void pj::k_scatter_wc<1, 8, false, 2, true>(...)", 40.0), ("pj::k_scan_reduce", 0.5),
+           ("pj::k_chunk_place(...)", 3.0), ("pj::k_chunk_bounds(...)", 0.125), ("pj::k_plan", 0.25),
+           ("void pj::k_join<false, true>(...)", 50.0)]
+    got = bench._label_step_kernels(app)
+    assert got == {"k_hist build": [10.0], "k_scatter build": [20.0],
+                   bench.PROBE_SCATTER_APP: [40.0, 3.0, 0.125], "scans, partition starts, k_plan": [0.5, 0.25],
+                   "k_join": [50.0]}
+    two = [("void pj::k_hist<1, false>(...)", 10.0), ("void pj::k_scatter<1, false>(...)", 20.0),
+           ("void pj::k_hist<1, false>(...)", 30.0), ("pj::k_coarsen(...)", 0.5),
+           ("void pj::k_scatter_wc<1, 8, false, 2, false>(...)", 40.0), ("pj::k_subsplit(...)", 45.0),
+           ("void pj::k_join<false, false>(...)", 50.0)]
+    got = bench._label_step_kernels(two)
+    assert got == {"k_hist build": [10.0], "k_scatter build": [20.0], "k_hist probe": [30.0],
+                   bench.PROBE_SCATTER_TWO: [0.5, 40.0, 45.0], "k_join": [50.0]}
+
+
 def test_traffic_summary_is_used_only_for_its_library(tmp_path, monkeypatch):
     monkeypatch.setattr(bench, "PROFILES", str(tmp_path))
     d = {"library_sha256_16": bench.lib_sha(), "kernels": {"k_join": {"read": 1.0, "write": 2.0}}}

@@ -337,20 +337,43 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             }
             __syncthreads();
             // overflow records: whole chunks leave at their slots, the rest
-            // refill the buffer. (The two stores differ in kind: two plain
-            // stores of one value merge into a flat store to either address.)
+            // refill the buffer. Two plain stores of one value in one
+            // if / else merge into a flat store to either address, so: a
+            // split block (whose partner reads the same input from L2) writes
+            // them nontemporal, which keeps the stores apart and the partner's
+            // lines in L2 (at 2048 partitions few records leave this way); an
+            // unsplit block (fewer partitions, many such records, which
+            // nontemporal stores slow threefold) stores them in a loop of
+            // their own before the refill.
+            if (SPLIT == 2) {
 #pragma unroll
-            for (int k = 0; k < IT; ++k)
-                if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {
-                    const uint32_t pk = ps[k] & 0x7FFu, rank = (ps[k] >> 11) - B, direct = ndir[pk];
-                    if (rank < direct) {
-                        const u32x4 r = {cur[k].x, cur[k].y, cur[k].z, cur[k].w};
-                        __builtin_nontemporal_store(
-                            r, reinterpret_cast<u32x4*>(out + slot_rec(goff[pk] + 1 + rank / B) + rank % B));
-                    } else {
-                        buf[pk * B + rank - direct] = cur[k];
+                for (int k = 0; k < IT; ++k)
+                    if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {
+                        const uint32_t pk = ps[k] & 0x7FFu, rank = (ps[k] >> 11) - B, direct = ndir[pk];
+                        if (rank < direct) {
+                            const u32x4 r = {cur[k].x, cur[k].y, cur[k].z, cur[k].w};
+                            __builtin_nontemporal_store(
+                                r, reinterpret_cast<u32x4*>(out + slot_rec(goff[pk] + 1 + rank / B) + rank % B));
+                        } else {
+                            buf[pk * B + rank - direct] = cur[k];
+                        }
                     }
-                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < IT; ++k)
+                    if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {
+                        const uint32_t pk = ps[k] & 0x7FFu, rank = (ps[k] >> 11) - B;
+                        if (rank < ndir[pk])
+                            out[slot_rec(goff[pk] + 1 + rank / B) + rank % B] = cur[k];
+                    }
+#pragma unroll
+                for (int k = 0; k < IT; ++k)
+                    if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {
+                        const uint32_t pk = ps[k] & 0x7FFu, rank = (ps[k] >> 11) - B, direct = ndir[pk];
+                        if (rank >= direct)
+                            buf[pk * B + rank - direct] = cur[k];
+                    }
+            }
         } else {
             for (uint32_t q = grp; q < pn; q += ngrp)
                 if (cnt[q] >= (uint32_t)B)

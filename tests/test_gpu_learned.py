@@ -61,6 +61,34 @@ def test_sort_u64(n):
     assert np.array_equal(got, np.sort(k))
 
 
+@pytest.mark.parametrize("kind", ["low24", "low32", "high_fixed", "low_fixed", "one_digit", "all_equal",
+                                  "odd_passes"])
+def test_sort_u64_skips_constant_digits(kind):
+    """Digits that are equal in every key are not sorted on; the keys still
+    come back sorted, also when the passes left are odd in number (their
+    result then lies in the ping-pong buffer) or none."""
+    _say("case: sort with constant digits, %s" % kind)
+    rng = np.random.default_rng(len(kind))
+    n = 300001
+    k = rng.integers(0, 1 << 24, size=n, dtype=np.uint64)                        # 3 varying digits
+    if kind == "low32":
+        k = rng.integers(0, 1 << 32, size=n, dtype=np.uint64)                    # 4 (an even number)
+    elif kind == "high_fixed":
+        k = k | np.uint64(0xABCD_0000_0000_0000)
+    elif kind == "low_fixed":
+        k = (k << np.uint64(40)) | np.uint64(0x12_3456_789A)                  # 3 varying digits on top
+    elif kind == "one_digit":
+        k = (k & np.uint64(0xFF)) << np.uint64(32)
+    elif kind == "all_equal":
+        k = np.full(n, 0x0123_4567_89AB_CDEF, dtype=np.uint64)
+    elif kind == "odd_passes":
+        k = (k & np.uint64(0xFFFF)) | ((k & np.uint64(0xFF0000)) << np.uint64(32))
+    d = _dev_u64(k)
+    P.sort_u64(d.data_ptr(), n)
+    got = d.cpu().numpy().view(np.uint64)[:n]
+    assert np.array_equal(got, np.sort(k))
+
+
 def _fit(keys, iters):
     u = OL.ordered(keys)
     s = u[::100].copy()

@@ -855,7 +855,7 @@ def main():
     if args.ceilings:
         return measure_ceilings(args)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is "
+        print("bench.py measures one GPU: the library has no multi-GPU path (the sharding design is "
               "written up, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload in BATCH_WORKLOADS:

@@ -344,7 +344,11 @@ int seq_handle(pj_batch_s* h, uint32_t bits, uint32_t lane, pj_join_t* out)
     pj_join_desc d = h->desc;
     d.partitioner = PJ_PART_RADIX;
     d.radix_bits = (int32_t)bits;
-    d.flags = 0;
+    // One probe layout for every size: the graph mode captures these handles
+    // after running each one on the largest build and probe sides it will
+    // meet, which sizes its buffers for that layout only (a smaller probe
+    // side taking another layout would grow a buffer inside the capture).
+    d.flags = PJ_FLAG_LAYOUT_EXACT;
     int rc = pj_join_create(&d, out);
     if (rc)
         return rc;

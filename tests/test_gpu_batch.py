@@ -318,3 +318,40 @@ def test_full_batch_512_joins_tuned_graph():
         _say("mode %d: 512 joins match, %d output tuples" % (mode, int(cnt.sum())))
     bt.close()
     del keep
+
+
+def test_batch_of_16_byte_joins_large_and_small_probes_in_one_lane():
+    """(long, long) tuples, lanes holding a probe side of 4M tuples and a
+    smaller one at the same radix bits: every mode, the graph one included,
+    gives each join its closed-form count and the same rows."""
+    _say("case: batch of 16-byte joins, large and small probe sides")
+    sizes = [1 << 20, 1 << 18, 70000, 3000, 600000, 1 << 17, 50000, 2000]
+    keep, joins = [], []
+    for j, nb in enumerate(sizes):
+        npr = 4 * nb
+        nmiss = npr // 8
+        b = torch.empty(nb * 16, dtype=torch.uint8, device="cuda")
+        p = torch.empty(npr * 16, dtype=torch.uint8, device="cuda")
+        o = torch.empty(npr * 24, dtype=torch.uint8, device="cuda")
+        P.check(P.lib().pj_gen_pkfk(700 + j, 1 << 40, nb, npr, nmiss, C.c_void_p(b.data_ptr()),
+                                    C.c_void_p(p.data_ptr()), 1, None))
+        joins.append((b.data_ptr(), nb, p.data_ptr(), npr, o.data_ptr(), npr))
+        keep += [b, p, o]
+    torch.cuda.synchronize()
+    bt = P.Batch(P.make_desc(["long", "long"], ["long", "long"], [0], [0], ["B$0", "B$1", "P$1"]))
+    bt.prepare(joins)
+    sums = None
+    for mode in (P.Batch.SEQUENTIAL, P.Batch.CONCURRENT, P.Batch.CONCURRENT_GRAPH, P.Batch.CONCURRENT_GRAPH):
+        for j in range(len(joins)):
+            keep[3 * j + 2].fill_(0xAB)
+        bt.run(mode)
+        cnt, _ = bt.results()
+        got = []
+        for j, (_, nb, _, npr, o, _) in enumerate(joins):
+            assert int(cnt[j]) == npr - npr // 8, (mode, j)
+            got.append(P.checksum_device(o, int(cnt[j]), 24))
+        if sums is None:
+            sums = got
+        assert got == sums, mode
+    bt.close()
+

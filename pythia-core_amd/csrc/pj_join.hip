@@ -1138,8 +1138,11 @@ __global__ __launch_bounds__(kJoinThreads) void k_join(const uint4* __restrict__
 // table splits add up. A key that an earlier slot of its bucket already
 // holds, or a partition split over several tables, sets *dup: equal keys
 // then have several records, which the caller merges.
+// CL: the probe side is appended (its records listed by chunk in plan.cl,
+// read as probe_rounds reads them).
 constexpr uint32_t kGjUnroll = 8;
 
+template <bool CL>
 __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restrict__ R, const uint4* __restrict__ S,
                                                              PlanView plan, uint32_t* __restrict__ item_ctr,
                                                              uint32_t* __restrict__ rcount, uint32_t bitsR,
@@ -1159,7 +1162,8 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
         if (it >= nitems)
             break;
         const Item I = plan_item_block(plan, it, misc + 34);
-        const uint4* Sp = S + I.s_off;
+        const uint4* Sp = CL ? S : S + I.s_off;
+        const uint32_t* clp = CL ? plan.cl + I.s_off / kChunkRecs : nullptr;
         uint64_t* tkey = table;                                            // kTableCap
         uint2* tpay = reinterpret_cast<uint2*>(tkey + kTableCap);          // kTableCap
         uint32_t* dirw = reinterpret_cast<uint32_t*>(tpay + kTableCap);    // kTableBuckets / 2 + 1
@@ -1179,9 +1183,28 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
         // record's register is refilled with the next round's record as soon
         // as its lookup is done, so loads stay in flight while LDS work runs.
         constexpr uint32_t kSlice = 64 * kGjUnroll, kRound = (kJoinThreads / 64) * kSlice;
+        constexpr uint32_t kSliceChunks = kSlice / kChunkRecs;
+        static_assert(kSliceChunks <= 64, "a slice's chunks fit a wave");
         const uint32_t lane = tid & 63, wave = tid >> 6;
+        // CL: lane l holds the number of chunk l of its wave's slice of round
+        // r0 (loaded two rounds ahead) and hands it to the lanes that load it
+        auto chunk_at = [&](uint32_t r0) -> uint32_t {
+            const uint32_t c = (r0 + wave * kSlice) / kChunkRecs + lane;
+            return lane < kSliceChunks && c * kChunkRecs < I.s_cnt ? clp[c] : 0u;
+        };
+        auto rec_at = [&](uint32_t ch, uint32_t u) -> const uint4* {
+            const uint32_t c = __shfl(ch, u * (64 / kChunkRecs) + lane / kChunkRecs, 64);
+            return Sp + ((uint64_t)c * kChunkRecs + lane % kChunkRecs);
+        };
         uint4 nxt[kGjUnroll];
-        if (I.s_cnt) {   // unconditional first loads (past s_cnt: the last record again, never looked up)
+        uint32_t ch_next = 0;
+        if (CL) {   // unconditional first loads (a lane past s_cnt: the list's first chunk)
+            const uint32_t ch0 = chunk_at(0);
+            ch_next = chunk_at(kRound);
+#pragma unroll
+            for (uint32_t u = 0; u < kGjUnroll; ++u)
+                nxt[u] = ld_stream(rec_at(ch0, u));
+        } else if (I.s_cnt) {   // unconditional first loads (past s_cnt: the last record again, never looked up)
 #pragma unroll
             for (uint32_t u = 0; u < kGjUnroll; ++u) {
                 const uint32_t i = wave * kSlice + u * 64 + lane;
@@ -1190,6 +1213,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
         }
         for (uint32_t r0 = 0; r0 < I.s_cnt; r0 += kRound) {
             const uint32_t t0 = r0 + wave * kSlice;
+            const uint32_t ch_after = CL ? chunk_at(r0 + 2 * kRound) : 0u;
 #pragma unroll
             for (uint32_t u = 0; u < kGjUnroll; ++u) {
                 const uint32_t i = t0 + u * 64 + lane;
@@ -1197,8 +1221,13 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
                 uint32_t nm = 0, first = 0;
                 if (i < I.s_cnt)
                     tab.lookup(s, nm, first);
-                if (i + kRound < I.s_cnt)
+                if (CL) {
+                    const uint4* a = rec_at(ch_next, u);
+                    if (i + kRound < I.s_cnt)
+                        nxt[u] = ld_stream(a);
+                } else if (i + kRound < I.s_cnt) {
                     nxt[u] = ld_stream(Sp + i + kRound);
+                }
                 if (nm == 1) {
                     atomicAdd(&tpay[first].y, 1u);
                 } else if (nm) {
@@ -1210,6 +1239,7 @@ __global__ __launch_bounds__(kJoinThreads) void k_groupjoin(const uint4* __restr
                         }
                 }
             }
+            ch_next = ch_after;
         }
         __syncthreads();
         const uint32_t ntab = dir[kTableBuckets - 1];
@@ -3106,9 +3136,14 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
     h->raw_build = d_build;
     h->raw_nbuild = nbuild;
     h->plan = RangePlan();
-    if ((rc = prepare_radix(h, d_build, nbuild, st)) ||
-        (rc = partition_side(h, d_probe, nprobe, h->lay.probe, radix_fn(h->bS), false, true, h->spart, h->S,
-                             h->sstarts, false, nullptr, st)))
+    if ((rc = prepare_radix(h, d_build, nbuild, st)))
+        return rc;
+    h->s_app = want_append(h, d_probe, nprobe);
+    h->s_two = false;
+    rc = h->s_app ? partition_append(h, d_probe, nprobe, h->lay.probe, radix_fn(h->bS), st)
+                  : partition_side(h, d_probe, nprobe, h->lay.probe, radix_fn(h->bS), false, true, h->spart, h->S,
+                                   h->sstarts, false, nullptr, st);
+    if (rc)
         return rc;
     if (nbuild == 0 || nprobe == 0) {
         PJ_HIP(hipStreamSynchronize(st));
@@ -3119,6 +3154,7 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
     v.shift = h->bR - h->bS;
     v.rst = (const uint32_t*)h->rstarts.p;
     v.sst = (const uint32_t*)h->sstarts.p;
+    v.cl = h->s_app ? (const uint32_t*)h->a_clist.p : nullptr;
     v.chunk = probe_chunk(nprobe);
     if (v.FR > kPlanThreads * kPlanPer)
         return fail(PJ_ERR_UNSUPPORTED, "%u build partitions exceed the planner", v.FR);
@@ -3135,11 +3171,15 @@ int pj_join_count_by_key(pj_join_t h, const void* d_build, uint64_t nbuild, cons
                        0u);
     hipLaunchKernelGGL(k_plan, dim3(1), dim3(kPlanThreads), 0, st, v, (uint32_t*)h->planoffs.p, counters);
     const size_t lds = join_lds_bytes(false);
-    if ((rc = allow_lds((const void*)k_groupjoin, lds)))
+    const void* gfn = h->s_app ? (const void*)k_groupjoin<true> : (const void*)k_groupjoin<false>;
+    if ((rc = allow_lds(gfn, lds)))
         return rc;
-    hipLaunchKernelGGL(k_groupjoin, dim3((uint32_t)device_count_cus()), dim3(kJoinThreads), lds, st,
-                       (const uint4*)h->rpart.p, (const uint4*)h->spart.p, v, counters, rcount, h->bR,
-                       64 - h->bR - 12, counters + 1);
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((uint32_t)device_count_cus()), dim3(kJoinThreads), lds, st,
+                           (const uint4*)h->rpart.p, (const uint4*)h->spart.p, v, counters, rcount, h->bR,
+                           64 - h->bR - 12, counters + 1);
+    };
+    h->s_app ? launch(k_groupjoin<true>) : launch(k_groupjoin<false>);
     PJ_HIP(hipGetLastError());
     PJ_HIP(hipMemcpyAsync(h->host_ctr, counters, 16, hipMemcpyDeviceToHost, st));
     PJ_HIP(hipStreamSynchronize(st));

@@ -166,3 +166,25 @@ def test_padding_and_extreme_keys():
     for bits in (-1, 11, 14):
         check(cols, cols, [0], [0], build.view(np.uint8), probe.view(np.uint8), bkeys.size, pkeys.size,
               radix_bits=bits)
+
+
+@pytest.mark.parametrize("bits", [1, 6, 11])
+def test_appended_probe_side(bits):
+    """The probe side appended (PJ_FLAG_LAYOUT_APPEND): the count per key
+    reads its probe records through the chunk list; padding and extreme keys,
+    a hot key and a key held by many build records."""
+    _say("case: count_by_key, appended probe side, bits=%d" % bits)
+    from test_gpu_join import _fin64
+    pk = next(k for k in range(1, 1 << 20) if _fin64(k) >> 63)
+    special = np.array([0, pk, -1, -(1 << 63), (1 << 63) - 1], np.int64)
+    rng = np.random.default_rng(70 + bits)
+    bkeys = np.concatenate([special, np.full(300, 12345), rng.choice(1 << 40, 40000, replace=False)])
+    pkeys = np.concatenate([np.repeat(special, 200), np.full(50000, bkeys[900]), np.full(100, 12345),
+                            rng.choice(bkeys, 250000)])
+    rng.shuffle(pkeys)
+    build = np.stack([bkeys, np.arange(bkeys.size, dtype=np.int64)], 1).reshape(-1)
+    probe = np.stack([pkeys, np.arange(pkeys.size, dtype=np.int64)], 1).reshape(-1)
+    cols = ["long", "long"]
+    check(cols, cols, [0], [0], build.view(np.uint8), probe.view(np.uint8), bkeys.size, pkeys.size,
+          radix_bits=bits, flags=P.PJ_FLAG_LAYOUT_APPEND)
+

@@ -86,7 +86,12 @@ struct pj_batch_s {
     // one join handle per (radix bits, lane): the lanes of the concurrent
     // mode each run on a stream of their own, with buffers of their own
     std::map<std::pair<uint32_t, uint32_t>, pj_join_t> seq;
-    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint64_t, uint64_t>> seq_warm;   // largest (build, probe) run
+    // largest sides a handle has run: build, probe of the appended layout,
+    // probe of the contiguous one
+    struct Warm {
+        uint64_t build = 0, probe = 0, probe_small = 0;
+    };
+    std::map<std::pair<uint32_t, uint32_t>, Warm> seq_warm;
     std::vector<hipStream_t> lanes;   // created for the concurrent mode, default priority
     hipGraphExec_t lane_exec = nullptr;     // the concurrent mode captured (PJ_BATCH_CONCURRENT_GRAPH)
     std::vector<uint32_t> lane_graph_bits;  // radix bits of each join in that capture
@@ -344,11 +349,7 @@ int seq_handle(pj_batch_s* h, uint32_t bits, uint32_t lane, pj_join_t* out)
     pj_join_desc d = h->desc;
     d.partitioner = PJ_PART_RADIX;
     d.radix_bits = (int32_t)bits;
-    // One probe layout for every size: the graph mode captures these handles
-    // after running each one on the largest build and probe sides it will
-    // meet, which sizes its buffers for that layout only (a smaller probe
-    // side taking another layout would grow a buffer inside the capture).
-    d.flags = PJ_FLAG_LAYOUT_EXACT;
+    d.flags = 0;
     int rc = pj_join_create(&d, out);
     if (rc)
         return rc;
@@ -402,18 +403,32 @@ int prepare_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const 
         h->seq_nslots = nj;
     }
     int rc;
-    std::map<std::pair<uint32_t, uint32_t>, std::pair<size_t, size_t>> big;   // most build, most probe tuples
+    // Per handle (radix bits, lane): the joins with the most build tuples,
+    // the most probe tuples, and the most probe tuples below kAppendMin. The
+    // graph mode captures the handles after these runs, which size every
+    // buffer a handle's joins need: the probe sides of kAppendMin tuples or
+    // more take the appended layout, the smaller ones the contiguous one,
+    // and each layout's buffers grow with its largest probe side.
+    struct Big {
+        size_t build, probe, probe_small;
+    };
+    constexpr size_t kNone = ~(size_t)0;
+    std::map<std::pair<uint32_t, uint32_t>, Big> big;
     for (size_t i = 0; i < nj; ++i) {
         const std::pair<uint32_t, uint32_t> key{bits[i], (uint32_t)(i % nlanes)};
+        const bool small = joins[i].nprobe < kAppendMin;
         auto it = big.find(key);
         if (it == big.end()) {
-            big[key] = {i, i};
+            big[key] = {i, i, small ? i : kNone};
             continue;
         }
-        if (joins[i].nbuild > joins[it->second.first].nbuild)
-            it->second.first = i;
-        if (joins[i].nprobe > joins[it->second.second].nprobe)
-            it->second.second = i;
+        Big& g = it->second;
+        if (joins[i].nbuild > joins[g.build].nbuild)
+            g.build = i;
+        if (joins[i].nprobe > joins[g.probe].nprobe)
+            g.probe = i;
+        if (small && (g.probe_small == kNone || joins[i].nprobe > joins[g.probe_small].nprobe))
+            g.probe_small = i;
     }
     bool waited = false;
     for (const auto& kv : big) {
@@ -422,9 +437,11 @@ int prepare_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const 
             return rc;
         auto& w = h->seq_warm[kv.first];
         const hipStream_t ls = nlanes > 1 ? h->lanes[kv.first.second] : st;
-        for (size_t i : {kv.second.first, kv.second.second}) {
+        for (size_t i : {kv.second.build, kv.second.probe, kv.second.probe_small}) {
+            if (i == kNone)
+                continue;
             const pj_batch_join& q = joins[i];
-            if (q.nbuild <= w.first && q.nprobe <= w.second)
+            if (q.nbuild <= w.build && q.nprobe <= (q.nprobe < kAppendMin ? w.probe_small : w.probe))
                 continue;
             if (!waited) {   // the inputs may still be in flight on st
                 PJ_HIP(hipStreamSynchronize(st));
@@ -434,8 +451,11 @@ int prepare_lanes(pj_batch_s* h, const std::vector<pj_batch_join>& joins, const 
             rc = pj_join_run_device(jh, q.build, q.nbuild, q.probe, q.nprobe, q.out, q.out_cap, &n, ls);
             if (rc && rc != PJ_ERR_CAPACITY)
                 return rc;
-            w.first = std::max<uint64_t>(w.first, q.nbuild);
-            w.second = std::max<uint64_t>(w.second, q.nprobe);
+            w.build = std::max<uint64_t>(w.build, q.nbuild);
+            if (q.nprobe < kAppendMin)
+                w.probe_small = std::max<uint64_t>(w.probe_small, q.nprobe);
+            else
+                w.probe = std::max<uint64_t>(w.probe, q.nprobe);
         }
     }
     return PJ_OK;

@@ -221,6 +221,10 @@ struct DevBuf {
 // of the mixed key, partition p at records [starts[p], starts[p + 1]) of
 // recs (device words). A partition holds padding records whose key belongs
 // to another partition; consumers skip them by recomputing the partition.
+// Probe sides of at least this many tuples take the appended layout by
+// default (PJ_FLAG_LAYOUT_EXACT / _APPEND force either).
+constexpr uint64_t kAppendMin = 1ull << 22;
+
 struct RadixRecords {
     DevBuf recs, starts, hist, bsum, pids;
 };

@@ -2088,9 +2088,9 @@ int launch_wc_t(int rm, const uint32_t* in, uint64_t n, const SideLayout& L, con
     if (rm == REC_IDENT)
         return launch_wc_rm<REC_IDENT, B, RANGE, SPLIT, APP>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero,
                                                              pid, app, lds, st);
-    if (!APP && (RANGE || rm == REC_WORDS))   // (the appended layout takes no word-by-word tuples)
-        return launch_wc_rm<REC_WORDS, B, RANGE, SPLIT, false>(in, n, L, pf, chunk, nblk, offs, out, pad_other,
-                                                               pad_zero, pid, app, lds, st);
+    if (RANGE || rm == REC_WORDS)
+        return launch_wc_rm<REC_WORDS, B, RANGE, SPLIT, APP>(in, n, L, pf, chunk, nblk, offs, out, pad_other,
+                                                             pad_zero, pid, app, lds, st);
     return launch_wc_rm<REC_HASH, B, false, SPLIT, APP>(in, n, L, pf, chunk, nblk, offs, out, pad_other, pad_zero,
                                                         pid, app, lds, st);
 }
@@ -2291,9 +2291,6 @@ int partition_bufs(DevBuf& histb, DevBuf& bsumb, DevBuf& pidsb, const void* d_in
     return PJ_OK;
 }
 
-// Probe sides of at least this many tuples take the appended layout by
-// default (PJ_FLAG_LAYOUT_EXACT / _APPEND force either).
-constexpr uint64_t kAppendMin = 1ull << 22;
 // A scatter block's lanes own its partitions, one each: at most 1024 a
 // block, 2048 over a split pair.
 constexpr uint32_t kAppendMaxF = 2 * kWcThreads;
@@ -2303,17 +2300,15 @@ static_assert(kAppendMaxF <= kPlaceMaxF, "the chunk list's tile sort holds every
 // build partitions are the probe partitions (the join reads each probe
 // partition once: listed chunks cost the join a record in flight a thread,
 // paid on every read), at least two partitions (one has no key outside it to
-// pad chunks with) and at most kAppendMaxF, 16-byte records or the row-id
-// form (the word-by-word variant spills registers), fewer than 2^30 tuples
-// (slot numbers in 32 bits), and not in a fixed scratch (it needs about
-// twice the room: each block of a split pair reserves slots for its whole
-// input chunk).
+// pad chunks with) and at most kAppendMaxF, fewer than 2^30 tuples (slot
+// numbers in 32 bits), and not in a fixed scratch (it needs about twice the
+// room: each block of a split pair reserves slots for its whole input
+// chunk).
 bool want_append(const pj_join_s* h, const void* d_probe, uint64_t np)
 {
-    const SideLayout& L = h->lay.probe;
-    const bool words = !L.hashed && !(L.ident && ((uintptr_t)d_probe % 16 == 0));
+    (void)d_probe;
     if (np == 0 || np >= (1ull << 30) || h->range || h->bS == 0 || (1u << h->bS) > kAppendMaxF ||
-        h->bR != h->bS || words || (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) || h->spart.arena)
+        h->bR != h->bS || (h->desc.flags & PJ_FLAG_LAYOUT_EXACT) || h->spart.arena)
         return false;
     return (h->desc.flags & PJ_FLAG_LAYOUT_APPEND) || np >= kAppendMin;
 }
@@ -2358,8 +2353,6 @@ int partition_append(pj_join_s* h, const void* d_in, uint64_t n, const SideLayou
         return rc;
     const uint32_t* in = (const uint32_t*)d_in;
     const int rm = L.hashed ? REC_HASH : (L.ident && ((uintptr_t)d_in % 16 == 0)) ? REC_IDENT : REC_WORDS;
-    if (rm == REC_WORDS)
-        return fail(PJ_ERR_INVALID, "the appended layout takes 16-byte records or the row-id form");
     if (h->ev_ok && (h->desc.flags & PJ_FLAG_TIMING))
         PJ_HIP(hipEventRecord(h->ev[9], st));
     AppendOut app;

@@ -179,3 +179,23 @@ def test_handle_reused_across_sizes():
         got = out[: n * j.out_width].cpu().numpy().reshape(-1, j.out_width)
         assert np.array_equal(O.sorted_rows(got), O.sorted_rows(exp)), (nb, npr)
     j.close()
+
+
+@pytest.mark.parametrize("bits", [3, 9, 11])
+def test_word_by_word_records(bits):
+    """12-byte (int64 key, int32 payload) tuples, read word by word into the
+    canonical records, and a probe side whose buffer is not 16-byte aligned."""
+    _say("case: appended layout, 12-byte tuples, bits=%d" % bits)
+    rng = np.random.default_rng(80 + bits)
+    nb, npr = 60000, 500000
+    bkeys = rng.choice(1 << 40, nb, replace=False)
+    pkeys = np.where(rng.random(npr) < 0.8, rng.choice(bkeys, npr), rng.integers(0, 1 << 40, npr))
+    bt = np.zeros(nb, dtype=[("k", "<i8"), ("v", "<i4")])
+    bt["k"], bt["v"] = bkeys, np.arange(nb)
+    pt = np.zeros(npr, dtype=[("k", "<i8"), ("v", "<i4")])
+    pt["k"], pt["v"] = pkeys, np.arange(npr)
+    args = dict(build_cols=["long", "int"], probe_cols=["long", "int"], buildjattr=[0], probejattr=[0],
+                projection=["B$0", "B$1", "P$1"])
+    exp = _both(args, bt.view(np.uint8), pt.view(np.uint8), nb, npr, radix_bits=bits)
+    assert exp.shape[0] > npr // 2
+

@@ -79,8 +79,8 @@ typedef struct {
 #define PJ_FLAG_LAYOUT_APPEND 8u        /* partition the probe side without a histogram pass (each block
                                            appends whole chunks, then the chunks are listed by partition),
                                            whatever its size, where the join allows it (radix partitions
-                                           read once, 16-byte records or the row-id form, fewer than 2^30
-                                           probe tuples); by default probe sides of 2^22 tuples or more
+                                           read once, fewer than 2^30 probe tuples); by default probe
+                                           sides of 2^22 tuples or more
                                            take it where allowed. It needs about twice the probe side's
                                            bytes of device memory for the partitioned copy */
 #define PJ_FLAG_LAYOUT_TWO_PASS 16u     /* where the build side has more partitions than one write-combining

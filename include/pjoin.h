@@ -80,9 +80,9 @@ typedef struct {
                                            appends whole chunks, then the chunks are listed by partition),
                                            whatever its size, where the join allows it (radix partitions
                                            read once, fewer than 2^30 probe tuples); by default probe
-                                           sides of 2^22 tuples or more
-                                           take it where allowed. It needs about twice the probe side's
-                                           bytes of device memory for the partitioned copy */
+                                           sides of 2^22 tuples or more take it where allowed. It needs
+                                           about twice the probe side's bytes of device memory for the
+                                           partitioned copy */
 #define PJ_FLAG_LAYOUT_TWO_PASS 16u     /* where the build side has more partitions than one write-combining
                                            pass gives the probe side (2 to 8 build partitions a probe
                                            partition), partition the probe side in two radix passes, high bits

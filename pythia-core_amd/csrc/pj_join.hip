@@ -139,6 +139,10 @@ __global__ void k_part_bounds(const uint32_t* __restrict__ offs, uint32_t F, uin
 }
 
 constexpr int kWcThreads = 1024;
+// Appended 2048-way scatter: a round of a block whose partitions fill fewer
+// whole chunks of overflow than this (uniform keys: about five a round)
+// stores them nontemporal.
+constexpr uint32_t kWcHotChunks = 16;
 
 // Software write-combining scatter: every partition owns a B-record buffer
 // in LDS and leaves HBM only in aligned chunks of B records. A round claims
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
     uint32_t* goff = cnt + FL;   // FL: next chunk's record index (APP: the round's first chunk slot)
     uint32_t* ndir = goff + FL;  // APP, FL: the round's overflow records that leave in whole chunks
     uint32_t* ccur = ndir + FL;  // APP: chunk slots the block has used
+    uint32_t* hot = ccur + 1;    // APP: the round's whole chunks of overflow (all partitions)
     uint32_t nch = 0;            // APP: chunks written of the thread's own partition
     const uint64_t rbase = APP ? (uint64_t)blockIdx.x * app.region : 0;   // APP: the block's first tag
     auto slot_rec = [&](uint64_t slot) -> uint64_t { return (rbase + slot) * B; };
@@ -213,8 +218,10 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             goff[p] = offs[(uint64_t)global_part(p) * nblk + part];
         cnt[p] = 0;
     }
-    if (APP && threadIdx.x == 0)
+    if (APP && threadIdx.x == 0) {
         *ccur = 0;
+        *hot = 0;
+    }
     __syncthreads();
     const uint64_t beg = (uint64_t)part * chunk;
     const uint64_t end = beg + chunk < n ? beg + chunk : n;
@@ -306,6 +313,9 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
 #pragma unroll
             for (int k = 0; k < IT; ++k)
                 ps[k] = v[k] ? p[k] | (s[k] << 11) : ~0u;
+            // (the previous round's reads of the count ended at its last barrier)
+            if (SPLIT == 2 && threadIdx.x == 0)
+                *hot = 0;
         }
         __syncthreads();
         // full buffers leave; whole chunks of overflow go straight out
@@ -323,6 +333,8 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
                     ndir[qo] = (no - 1) * B;
                     nch += no;
                 }
+                if (SPLIT == 2 && no > 1)
+                    atomicAdd(hot, no - 1);
                 for (uint32_t t = 0; t < no; ++t)
                     app.cmap[rbase + so + t] = (uint16_t)global_part(qo);
             }
@@ -338,14 +350,15 @@ __global__ __launch_bounds__(kWcThreads) void k_scatter_wc(const uint32_t* __res
             __syncthreads();
             // overflow records: whole chunks leave at their slots, the rest
             // refill the buffer. Two plain stores of one value in one
-            // if / else merge into a flat store to either address, so: a
-            // split block (whose partner reads the same input from L2) writes
-            // them nontemporal, which keeps the stores apart and the partner's
-            // lines in L2 (at 2048 partitions few records leave this way); an
-            // unsplit block (fewer partitions, many such records, which
-            // nontemporal stores slow threefold) stores them in a loop of
-            // their own before the refill.
-            if (SPLIT == 2) {
+            // if / else merge into a flat store to either address, so: where
+            // few whole chunks leave (uniform keys: a few a round, which a
+            // 2048-way split block writes nontemporal, keeping the stores
+            // apart and its partner's input lines in L2) one loop stores
+            // them nontemporal; where many leave (skewed keys, or fewer
+            // partitions: nontemporal stores of many partial lines run up to
+            // three times slower) a loop of their own stores them before the
+            // refill.
+            if (SPLIT == 2 && *hot < kWcHotChunks) {
 #pragma unroll
                 for (int k = 0; k < IT; ++k)
                     if (ps[k] != ~0u && (ps[k] >> 11) >= (uint32_t)B) {

@@ -102,6 +102,27 @@ def test_one_probe_key_fills_its_partition(bits):
     assert exp.shape[0] == pkeys.size
 
 
+@pytest.mark.parametrize("bits", [10, 11])
+def test_hot_and_cold_rounds(bits):
+    # the probe side alternates runs of 6000 uniform probes with runs of
+    # 6000 probes of which a third are four hot keys: the scatter's rounds
+    # with few whole chunks of overflow and those with many (stored by two
+    # different loops) follow each other within every block
+    _say("case: appended layout, hot and cold rounds, bits=%d" % bits)
+    rng = np.random.default_rng(60 + bits)
+    bkeys = rng.choice(1 << 40, 200000, replace=False)
+    runs = []
+    for r in range(300):
+        run = rng.choice(bkeys, 6000)
+        if r % 2:
+            hot = rng.random(6000) < 1 / 3
+            run[hot] = bkeys[rng.integers(0, 4, int(hot.sum()))]
+        runs.append(run)
+    pkeys = np.concatenate(runs)
+    exp = _both(HEADLINE, _tuples(bkeys), _tuples(pkeys, 1 << 35), bkeys.size, pkeys.size, radix_bits=bits)
+    assert exp.shape[0] == pkeys.size
+
+
 def _fin64(x):
     m = (1 << 64) - 1
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9 & m
